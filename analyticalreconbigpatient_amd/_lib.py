@@ -1,0 +1,121 @@
+"""ctypes binding of libarecon.so (the gfx950 HIP library).
+
+The library is loaded from the package directory only.  There is no fallback:
+if ``libarecon.so`` is missing or does not export a declared symbol,
+``lib()`` raises, so nothing can silently run a CPU substitute.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libarecon.so"
+
+ARECON_OK = 0
+ARECON_F32, ARECON_F16, ARECON_BF16 = 0, 1, 2
+ARECON_LAYOUT_VYZ, ARECON_LAYOUT_VZY = 0, 1
+
+_f = C.c_float
+_d = C.c_double
+_i = C.c_int
+_p = C.c_void_p
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+
+
+class BPGeom(C.Structure):
+    """Mirror of ``arecon_bp_geom`` (include/arecon.h)."""
+
+    _fields_ = [
+        ("N_2pi", _i), ("XN", _i), ("YN", _i), ("ZN", _i), ("YL", _i), ("ZL", _i), ("PN", _i),
+        ("XNC", _d), ("YNC", _d), ("ZNC", _d),
+        ("dx", _d), ("dy", _d), ("dz", _d),
+        ("YLC", _d), ("ZLC", _d),
+        ("dYL", _d), ("dZL", _d),
+        ("h", _d),
+        ("BetaS", _d), ("DeltaFai", _d),
+        ("N_pi", _d), ("HSCoef", _d),
+        ("SO", _d), ("SD", _d), ("k1", _d),
+        ("nz", _i),
+        ("order", _i),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/arecon.h.
+SIGNATURES = {
+    "ParallelRebinningCBCurve_GPU": (None, [_fp, _fp, _i, _i, _i, _f, _f, _f, _f, _f, _f, _f]),
+    "filtering": (None, [_fp, _fp, _i, _i, _i, _f, _f, _f, _f, _f]),
+    "ParallekFDKHelical3DWeighting_GPU_float": (
+        None, [_fp, _fp, _fp, _i, _i, _i, _i, _f, _f, _f, _f, _f, _f, _i, _i, _i,
+               _f, _f, _f, _f, _f, _f, _f, _f, _f, _f, _f, _f]),
+    "ParallekFDKHelical3DWeighting_GPU_double": (
+        None, [_dp, _dp, _dp, _i, _i, _i, _i, _d, _d, _d, _d, _d, _d, _i, _i, _i,
+               _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d]),
+    "ParallelFDKHelical3DWeighting": (
+        None, [_dp, _dp, _d, _d, _i, _i, _d, _d, _d, _d, _d, _d, _i, _i, _d, _i, _i, _i,
+               _d, _d, _d, _i, _d, _d, _i]),
+    "arecon_last_error": (_i, [C.c_char_p, C.c_size_t]),
+    "arecon_version": (_i, []),
+    "arecon_device_count": (_i, []),
+    "arecon_rebin_dev": (_i, [_p, _p, _i, _i, _i, _f, _f, _f, _f, _f, _f, _f, _i, _p]),
+    "arecon_filter_dev": (_i, [_p, _i, _p, _i, _i, _i, _f, _f, _f, _f, _f, _p]),
+    "arecon_bp_plan_create": (_i, [C.POINTER(BPGeom), _dp, C.POINTER(_p)]),
+    "arecon_bp_plan_destroy": (_i, [_p]),
+    "arecon_bp_plan_execute": (_i, [_p, _p, _p, _i, _p]),
+    "arecon_bp_dev_f64": (_i, [_p, _p, C.POINTER(BPGeom), _dp, _p]),
+    "arecon_fdk_weights": (_i, [_dp, _i, _d, _i]),
+    "arecon_project_ellipsoids_dev": (_i, [_p, _fp, _i, _i, _i, _i, _d, _d, _d, _d, _d, _d,
+                                           _d, _d, _d, _p]),
+}
+
+_lock = threading.Lock()
+_lib: C.CDLL | None = None
+
+
+class AreconError(RuntimeError):
+    """A call into libarecon.so reported a failure."""
+
+
+def lib() -> C.CDLL:
+    """Load libarecon.so (once) and bind every declared symbol."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise AreconError(
+                    f"{LIB_PATH} is missing: build it with "
+                    "`python -c 'import __graft_entry__ as g; g.build()'`")
+            handle = C.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = handle
+        return _lib
+
+
+def last_error() -> tuple[int, str]:
+    buf = C.create_string_buffer(2048)
+    code = lib().arecon_last_error(buf, len(buf))
+    return code, buf.value.decode(errors="replace")
+
+
+def check(status: int | None, what: str) -> None:
+    """Raise AreconError if ``status`` (or, for void drop-ins, the thread's
+    last-error slot) reports a failure."""
+    if status is None:
+        code, msg = last_error()
+    else:
+        code = status
+        msg = last_error()[1] if status != ARECON_OK else ""
+    if code != ARECON_OK:
+        raise AreconError(f"{what}: {msg} (code {code})")
+
+
+def fptr(arr) -> C.POINTER(C.c_float):
+    return arr.ctypes.data_as(_fp)
+
+
+def dptr(arr) -> C.POINTER(C.c_double):
+    return arr.ctypes.data_as(_dp)

@@ -1,0 +1,100 @@
+"""Build recipe for the gfx950 library (libarecon.so) and the CPU oracle.
+
+``build_library()`` compiles every ``csrc/*.hip`` translation unit with hipcc for
+gfx950 (one process per file, in parallel) and links ``libarecon.so`` next to
+this file.  ``build_oracle()`` compiles ``oracle/arecon_oracle.c`` with gcc into
+``oracle/liboracle.so`` (test infrastructure; the library never links it).
+
+Run ``python -m analyticalreconbigpatient_amd.build`` to build both.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+BUILD = ROOT / "_build"
+LIB = PKG / "libarecon.so"
+ORACLE_SRC = ROOT / "oracle" / "arecon_oracle.c"
+ORACLE_LIB = ROOT / "oracle" / "liboracle.so"
+
+ARCH = "gfx950"
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Wno-unused-result",
+    # Packed FP32 instructions stay off: hipcc must not form v_pk_{add,mul,fma}_f32.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{INCLUDE}",
+    f"-I{CSRC}",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found (ROCm is required to build libarecon.so)")
+
+
+def _run_parallel(cmds: list[list[str]]) -> None:
+    procs = [(c, subprocess.Popen(c, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)) for c in cmds]
+    failed = []
+    for c, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            failed.append((c, out.decode(errors="replace")))
+        elif out:
+            sys.stderr.write(out.decode(errors="replace"))
+    if failed:
+        msg = "\n\n".join(" ".join(c) + "\n" + o for c, o in failed)
+        raise RuntimeError(f"compilation failed:\n{msg}")
+
+
+def build_library(verbose: bool = False) -> Path:
+    """Compile csrc/*.hip for gfx950 and link libarecon.so (always rebuilds)."""
+    hipcc = _hipcc()
+    BUILD.mkdir(exist_ok=True)
+    sources = sorted(CSRC.glob("*.hip"))
+    objs = [BUILD / (s.stem + ".o") for s in sources]
+    cmds = [[hipcc, *HIPCC_FLAGS, "-c", str(s), "-o", str(o)] for s, o in zip(sources, objs)]
+    if verbose:
+        for c in cmds:
+            print(" ".join(c))
+    _run_parallel(cmds)
+    tmp = LIB.with_suffix(".so.tmp")
+    link = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    _run_parallel([link])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(verbose: bool = False) -> Path:
+    """Compile the CPU oracle (test infrastructure) into oracle/liboracle.so."""
+    cc = os.environ.get("CC", "gcc")
+    tmp = ORACLE_LIB.with_suffix(".so.tmp")
+    cmd = [cc, "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+           "-std=c99", "-D_GNU_SOURCE", str(ORACLE_SRC), "-o", str(tmp), "-lm"]
+    if verbose:
+        print(" ".join(cmd))
+    _run_parallel([cmd])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def build_all(verbose: bool = False) -> None:
+    build_oracle(verbose)
+    build_library(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)

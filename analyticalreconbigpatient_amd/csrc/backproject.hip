@@ -1,0 +1,435 @@
+// 3D-weighted helical FDK backprojection in the cone-parallel geometry (gfx950).
+//
+// Reference behaviour: src/Parallel_FDK_Helical_3DWeighting.cpp:42-113 (CPU path;
+// the view loop covers the voxel's window [s1, s2], :71).  For voxel (x, y, z):
+//   s0 = ceil((2 pi z / h - BetaS) / DeltaFai - 0.5), window s0 -+ W,
+//   W = ceil(N_pi HSCoef), clamped to [0, PN-1], skipped unless s0-W < PN and
+//   s0+W > 0 (:63-70).  For each view p of the window, with V = BetaS + p DeltaFai:
+//     UU = -x cosV - y sinV,  Yr = -x sinV + y cosV,  temp1 = sqrt(SO^2 - Yr^2),
+//     temp2 = z - h V / 2pi,  Zr = temp2 SO SD / (UU temp1 + SO^2 - Yr^2),
+//     bilinear sample at (Yr/dYL + YLC, Zr/dZL + ZLC) if 0 < ceil(.) < YL / ZL,
+//     weight = w1 Gc^k1 / (w2 G^k1 + w1 Gc^k1),  G = |temp2/(temp1+UU)|,
+//     Gc = |(temp2 -+ 0.5)/(temp1-UU)|  (- before s0, + from s0 on),
+//     w1 = w[N_pi - (s0 - p)], w2 = w[w1 index +- N_pi]   (:73-104).
+//
+// Design (MI355X-first).
+//  * One wavefront = an 8x8 patch of (x, y) columns at the same slices, so the
+//    window, the per-view table entries and the weight-table entries are
+//    wavefront-uniform (scalar loads, scalar branches); the 64 gathers of a load
+//    instruction fall on a few neighbouring detector cache lines.
+//  * Each thread owns NZ consecutive slices of its column.  Everything that
+//    depends only on (column, view) -- UU, Yr, temp1, the channel and its blend
+//    weight, the magnification and the ratio (temp1-UU)/(temp1+UU) -- is computed
+//    once per view and reused by the NZ slices; a slice whose window does not
+//    contain the view skips it with a uniform branch, so every voxel still sees
+//    exactly its own window.
+//  * FP64 on the host for everything that cancels over thousands of views: the
+//    per-view cos/sin table, the per-slice s0 and the slice's offset from its
+//    central view's source height, z - h V(s0) / 2pi.  In the kernel
+//    temp2 = offset - (p - s0) h DeltaFai / 2pi, both terms small.
+//  * The 3D weight uses one ratio: weight = w1 / (w1 + w2 (G/Gc)^k1), with
+//    (G/Gc)^k1 = exp2(k1 log2 ratio).  Inside the window's centre (no conjugate
+//    view in the window) w2 = 0 and the weight is exactly 1.
+//  * Workgroups are remapped so that each of the 8 XCDs walks a contiguous run of
+//    the logical (slice set, column tile) sequence and keeps its own L2 warm.
+// Roofline: 16 B of gathered FP32 samples (8 B FP16/BF16) per voxel-update,
+// served from L2/MALL; compulsory HBM traffic is the filtered sinogram once and
+// the volume once.  VALU (transcendentals) is the co-bound.
+#include "arecon_internal.h"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+namespace arecon {
+
+namespace {
+
+struct BPZ {        // per slice
+    int s0;         // central view
+    int s1, s2;     // clamped window; s1 > s2 when the slice gets no view
+    float dz;       // z - h (BetaS + s0 DeltaFai) / 2pi
+};
+
+struct BPArgs {
+    int XN, YN, ZN, YL, ZL, PN, W;
+    int tilesx, ntiles, nzsets, order, k1zero;
+    int vec4;                         // float4 stores allowed (ZN % 4 == 0, aligned output)
+    float XNC, YNC, dx, dy;
+    float YLC, ZLC, invdYL, KSD;      // KSD = SO SD / dZL
+    float YLm1, ZLm1;                 // YL-1, ZL-1 as floats
+    float SO2, hstep, k1, DeltaFai;   // hstep = h DeltaFai / 2pi
+};
+
+// Two adjacent detector samples (channels U-1, U of one row) as floats.
+template <class T> __device__ __forceinline__ float2 load2(const T* p);
+
+typedef float f32x2_u __attribute__((ext_vector_type(2), aligned(4)));
+template <> __device__ __forceinline__ float2 load2<float>(const float* p) {
+    const f32x2_u v = *reinterpret_cast<const f32x2_u*>(p);
+    return make_float2(v.x, v.y);
+}
+template <> __device__ __forceinline__ float2 load2<__half>(const __half* p) {
+    const unsigned short* q = reinterpret_cast<const unsigned short*>(p);
+    return make_float2(__half2float(__ushort_as_half(q[0])), __half2float(__ushort_as_half(q[1])));
+}
+template <> __device__ __forceinline__ float2 load2<__hip_bfloat16>(const __hip_bfloat16* p) {
+    const unsigned short* q = reinterpret_cast<const unsigned short*>(p);
+    return make_float2(__uint_as_float((unsigned)q[0] << 16), __uint_as_float((unsigned)q[1] << 16));
+}
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+template <int NZ, class T>
+__global__ __launch_bounds__(256) void bp_fdk3dw(float* __restrict__ img,
+                                                 const T* __restrict__ fp,
+                                                 const float2* __restrict__ vtab,
+                                                 const float4* __restrict__ ktab,
+                                                 const BPZ* __restrict__ ztab, BPArgs A) {
+    // XCD-aware remap: hardware workgroup b runs on XCD b % 8; give each XCD a
+    // contiguous run of logical workgroups.
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int qx = nb >> 3, rx = nb & 7, xcd = b & 7;
+    const int lb = xcd * qx + min(xcd, rx) + (b >> 3);
+    int zs, tile;
+    if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
+    else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
+    const int tyb = tile / A.tilesx, txb = tile - tyb * A.tilesx;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int xi = txb * 16 + (wave & 1) * 8 + (lane & 7);
+    const int yi = tyb * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const float x = ((float)xi - A.XNC) * A.dx;
+    const float y = ((float)yi - A.YNC) * A.dy;
+    const int z0 = zs * NZ;
+
+    int s0[NZ], s1[NZ], s2[NZ];
+    float dzs[NZ], acc[NZ];
+    int pmin = INT_MAX, pmax = INT_MIN;
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+        acc[j] = 0.f;
+        if (z0 + j < A.ZN) {
+            const BPZ t = ztab[z0 + j];
+            s0[j] = t.s0; s1[j] = t.s1; s2[j] = t.s2; dzs[j] = t.dz;
+        } else {
+            s0[j] = 0; s1[j] = 1; s2[j] = 0; dzs[j] = 0.f;
+        }
+        if (s1[j] <= s2[j]) { pmin = min(pmin, s1[j]); pmax = max(pmax, s2[j]); }
+    }
+
+    const long long view_stride = (long long)A.ZL * A.YL;
+    for (int p = pmin; p <= pmax; ++p) {
+        const float2 cs = vtab[p];
+        const float UU = -x * cs.x - y * cs.y;
+        const float Yr = -x * cs.y + y * cs.x;
+        const float t1sq = A.SO2 - Yr * Yr;
+        const float temp1 = fast_sqrt(t1sq);
+        const float U1 = Yr * A.invdYL + A.YLC;
+        const bool uok = (U1 > 0.f) && (U1 <= A.YLm1);   // 0 < ceil(U1) < YL
+        const float Uc = ceilf(U1);
+        const float Dey = Uc - U1;
+        const int U = uok ? (int)Uc : 1;
+        const float K = A.KSD * fast_rcp(fmaf(UU, temp1, t1sq));
+        const float R = fabsf((temp1 - UU) * fast_rcp(temp1 + UU));
+        const T* col = fp + (long long)p * view_stride + (U - 1);
+#pragma unroll
+        for (int j = 0; j < NZ; ++j) {
+            if (p >= s1[j] && p <= s2[j]) {
+                const int k = p - s0[j];
+                const float4 kt = ktab[k + A.W];   // w1, w2, -+0.5, w1/(w1+w2)
+                const float temp2 = fmaf(-(float)k, A.hstep, dzs[j]);
+                const float V1 = fmaf(temp2, K, A.ZLC);
+                if (uok && V1 > 0.f && V1 <= A.ZLm1) {       // 0 < ceil(V1) < ZL
+                    const float Vc = ceilf(V1);
+                    const float Dez = Vc - V1;
+                    const T* q0 = col + ((int)Vc - 1) * A.YL;
+                    const float2 r0 = load2<T>(q0);          // row V-1: U-1, U
+                    const float2 r1 = load2<T>(q0 + A.YL);   // row V:   U-1, U
+                    const float top = fmaf(Dey, r0.x - r0.y, r0.y);
+                    const float bot = fmaf(Dey, r1.x - r1.y, r1.y);
+                    const float tou = fmaf(Dez, top - bot, bot);
+                    float weight;
+                    if (kt.y == 0.f) {
+                        weight = 1.f;
+                    } else if (A.k1zero) {
+                        weight = kt.w;
+                    } else {
+                        const float ratio = fabsf(temp2) * R * fast_rcp(fabsf(temp2 + kt.z));
+                        const float m = fast_exp2(A.k1 * fast_log2(ratio));
+                        weight = kt.x * fast_rcp(fmaf(kt.y, m, kt.x));
+                    }
+                    acc[j] = fmaf(weight, tou, acc[j]);
+                }
+            }
+        }
+    }
+    if (xi < A.XN && yi < A.YN) {
+        float* o = img + ((long long)yi * A.XN + xi) * A.ZN + z0;
+        bool done = false;
+        if constexpr (NZ == 4) {
+            if (A.vec4 && z0 + 4 <= A.ZN) {
+                *reinterpret_cast<float4*>(o) =
+                    make_float4(acc[0] * A.DeltaFai, acc[1] * A.DeltaFai,
+                                acc[2] * A.DeltaFai, acc[3] * A.DeltaFai);
+                done = true;
+            }
+        }
+        if (!done) {
+#pragma unroll
+            for (int j = 0; j < NZ; ++j)
+                if (z0 + j < A.ZN) o[j] = acc[j] * A.DeltaFai;
+        }
+    }
+}
+
+// FP64 path: the same math as the reference, one thread per voxel, in double.
+__global__ __launch_bounds__(256) void bp_fdk3dw_f64(double* __restrict__ img,
+                                                     const double* __restrict__ fp,
+                                                     const double* __restrict__ w,
+                                                     arecon_bp_geom g, double invPI,
+                                                     double pi_ref) {
+    const long long n = (long long)g.XN * g.YN * g.ZN;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int zi = (int)(id % g.ZN);
+    const long long xy = id / g.ZN;
+    const int xi = (int)(xy % g.XN);
+    const int yi = (int)(xy / g.XN);
+    const double x = (xi - g.XNC) * g.dx;
+    const double y = (yi - g.YNC) * g.dy;
+    const double z = (zi - g.ZNC) * g.dz;
+    const double Beta0 = 2.0 * pi_ref * z / g.h;
+    const int s0 = (int)ceil((Beta0 - g.BetaS) / g.DeltaFai - 0.5);
+    int s1 = (int)(s0 - ceil(g.N_pi * g.HSCoef));
+    int s2 = (int)(s0 + ceil(g.N_pi * g.HSCoef));
+    double res = 0;
+    if ((s1 < g.PN) && (s2 > 0)) {
+        s1 = (s1 < 0) ? 0 : s1;
+        s2 = (s2 > g.PN - 1) ? g.PN - 1 : s2;
+        for (int p = s1; p <= s2; ++p) {
+            const double View = g.BetaS + p * g.DeltaFai;
+            const int d1 = (int)(g.N_pi - (s0 - p));
+            const int d2 = (int)((p < s0) ? (d1 + g.N_pi) : (d1 - g.N_pi));
+            double sv, cv;
+            sincos(View, &sv, &cv);
+            const double UU = -x * cv - y * sv;
+            const double Yr = -x * sv + y * cv;
+            const double temp1 = sqrt(g.SO * g.SO - Yr * Yr);
+            const double temp2 = z - g.h * View * invPI;
+            const double Zr = temp2 * (g.SO * g.SD) / (UU * temp1 + g.SO * g.SO - Yr * Yr);
+            const double U1 = Yr / g.dYL + g.YLC;
+            const int U = (int)ceil(U1);
+            const double V1 = Zr / g.dZL + g.ZLC;
+            const int V = (int)ceil(V1);
+            const double Dey = U - U1, Dez = V - V1;
+            if ((U > 0) && (U < g.YL) && (V > 0) && (V < g.ZL)) {
+                const double* f = fp + (long long)p * g.ZL * g.YL;
+                const double tou = Dey * Dez * f[(V - 1) * g.YL + (U - 1)] +
+                                   Dey * (1.0 - Dez) * f[V * g.YL + (U - 1)] +
+                                   (1.0 - Dey) * Dez * f[(V - 1) * g.YL + U] +
+                                   (1.0 - Dey) * (1.0 - Dez) * f[V * g.YL + U];
+                const double w1 = w[d1], w2 = w[d2];
+                const double G = fabs(temp2 / (temp1 + UU));
+                const double Gc = (p < s0) ? fabs((temp2 - 0.5) / (temp1 - UU))
+                                           : fabs((temp2 + 0.5) / (temp1 - UU));
+                const double m1 = pow(G, g.k1), m2 = pow(Gc, g.k1);
+                res += (w1 * m2) / (w2 * m1 + w1 * m2) * tou * g.DeltaFai;
+            }
+        }
+    }
+    img[((long long)yi * g.XN + xi) * g.ZN + zi] = res;
+}
+
+// File-level pi of the reference (src/Parallel_FDK_Helical_3DWeighting.cpp:16),
+// used for Beta0 (:62); invPI uses the full constant (:51).
+constexpr double kRefPi = 3.14159265358979;
+constexpr double kInvTwoPi = 1.0 / (2.0 * 3.141592653589793);
+
+// Index arithmetic of :74-75 in the reference's types (double N_pi, int result).
+inline int weight_index1(double N_pi, int k) { return (int)(N_pi + k); }
+inline int weight_index2(double N_pi, int k) {
+    const int d1 = weight_index1(N_pi, k);
+    return (int)((k < 0) ? (d1 + N_pi) : (d1 - N_pi));
+}
+
+void validate_geom(const arecon_bp_geom& g) {
+    ARECON_REQUIRE(g.XN > 0 && g.YN > 0 && g.ZN > 0, "bp: XN, YN and ZN must be positive");
+    ARECON_REQUIRE(g.YL >= 2 && g.ZL >= 2, "bp: YL and ZL must be at least 2");
+    ARECON_REQUIRE(g.PN > 0 && g.N_2pi > 0, "bp: PN and N_2pi must be positive");
+    ARECON_REQUIRE(g.h != 0.0, "bp: helical pitch h must be nonzero");
+    ARECON_REQUIRE(g.DeltaFai > 0.0, "bp: DeltaFai must be positive");
+    ARECON_REQUIRE(g.HSCoef >= 0.0, "bp: HSCoef must be non-negative");
+    ARECON_REQUIRE(g.SO > 0.0 && g.dYL != 0.0 && g.dZL != 0.0, "bp: SO, dYL, dZL invalid");
+    const int W = (int)std::ceil(g.N_pi * g.HSCoef);
+    for (int k = -W; k <= W; ++k) {
+        const int d1 = weight_index1(g.N_pi, k), d2 = weight_index2(g.N_pi, k);
+        ARECON_REQUIRE(d1 >= 0 && d1 < g.N_2pi && d2 >= 0 && d2 < g.N_2pi,
+                       "bp: N_pi*HSCoef puts a weight index outside w[0, N_2pi) "
+                       "(the reference reads out of bounds there)");
+    }
+}
+
+int auto_nz(const std::vector<BPZ>& z, int W) {
+    // Largest NZ whose union of NZ consecutive windows stays within ~15% of one
+    // window (slices further apart share too few views to amortise the column
+    // work).
+    const int n = (int)z.size();
+    double ds = 0.0;
+    if (n > 1) ds = std::fabs((double)(z[n - 1].s0 - z[0].s0)) / (n - 1);
+    const double L = 2.0 * W + 1.0;
+    for (int nz : {8, 4, 2})
+        if ((nz - 1) * ds <= 0.15 * L && n >= nz) return nz;
+    return 1;
+}
+
+}  // namespace
+
+struct BPPlan {
+    arecon_bp_geom g;
+    int device = 0;
+    float2* vtab = nullptr;
+    float4* ktab = nullptr;
+    BPZ* ztab = nullptr;
+    BPArgs a{};
+    int nz = 1;
+    int grid = 0;
+};
+
+BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
+    validate_geom(g);
+    ARECON_REQUIRE(w != nullptr, "bp: null weight table");
+    const int W = (int)std::ceil(g.N_pi * g.HSCoef);
+    // Per-view table (FP64 -> FP32).
+    std::vector<float2> vt(g.PN);
+    for (int p = 0; p < g.PN; ++p) {
+        const double V = g.BetaS + p * g.DeltaFai;
+        vt[p] = make_float2((float)std::cos(V), (float)std::sin(V));
+    }
+    // Per-offset table k = p - s0 in [-W, W].
+    std::vector<float4> kt(2 * W + 1);
+    for (int k = -W; k <= W; ++k) {
+        const double w1 = w[weight_index1(g.N_pi, k)], w2 = w[weight_index2(g.N_pi, k)];
+        const double r = (w1 + w2 != 0.0) ? w1 / (w1 + w2) : 0.0;
+        kt[k + W] = make_float4((float)w1, (float)w2, k < 0 ? -0.5f : 0.5f, (float)r);
+    }
+    // Per-slice table.
+    std::vector<BPZ> zt(g.ZN);
+    for (int zi = 0; zi < g.ZN; ++zi) {
+        const double z = (zi - g.ZNC) * g.dz;
+        const double Beta0 = 2.0 * kRefPi * z / g.h;
+        const int s0 = (int)std::ceil((Beta0 - g.BetaS) / g.DeltaFai - 0.5);
+        int s1 = s0 - W, s2 = s0 + W;
+        BPZ e;
+        e.s0 = s0;
+        if ((s1 < g.PN) && (s2 > 0)) {
+            e.s1 = std::max(s1, 0);
+            e.s2 = std::min(s2, g.PN - 1);
+        } else {
+            e.s1 = 1;
+            e.s2 = 0;
+        }
+        e.dz = (float)(z - g.h * (g.BetaS + s0 * g.DeltaFai) * kInvTwoPi);
+        zt[zi] = e;
+    }
+    BPPlan* P = new BPPlan();
+    P->g = g;
+    ARECON_HIP(hipGetDevice(&P->device));
+    P->nz = g.nz > 0 ? g.nz : auto_nz(zt, W);
+    ARECON_REQUIRE(P->nz == 1 || P->nz == 2 || P->nz == 4 || P->nz == 8,
+                   "bp: nz must be 0, 1, 2, 4 or 8");
+    ARECON_REQUIRE(g.order == 0 || g.order == 1, "bp: order must be 0 or 1");
+    BPArgs& A = P->a;
+    A.XN = g.XN; A.YN = g.YN; A.ZN = g.ZN; A.YL = g.YL; A.ZL = g.ZL; A.PN = g.PN; A.W = W;
+    A.tilesx = (g.XN + 15) / 16;
+    const int tilesy = (g.YN + 15) / 16;
+    A.ntiles = A.tilesx * tilesy;
+    A.nzsets = (g.ZN + P->nz - 1) / P->nz;
+    A.order = g.order;
+    A.k1zero = (g.k1 == 0.0);
+    A.XNC = (float)g.XNC; A.YNC = (float)g.YNC; A.dx = (float)g.dx; A.dy = (float)g.dy;
+    A.YLC = (float)g.YLC; A.ZLC = (float)g.ZLC; A.invdYL = (float)(1.0 / g.dYL);
+    A.KSD = (float)(g.SO * g.SD / g.dZL);
+    A.YLm1 = (float)(g.YL - 1); A.ZLm1 = (float)(g.ZL - 1);
+    A.SO2 = (float)(g.SO * g.SO); A.hstep = (float)(g.h * g.DeltaFai * kInvTwoPi);
+    A.k1 = (float)g.k1; A.DeltaFai = (float)g.DeltaFai;
+    const long long grid = (long long)A.ntiles * A.nzsets;
+    ARECON_REQUIRE(grid < (1LL << 31), "bp: volume too large for one launch");
+    P->grid = (int)grid;
+    try {
+        ARECON_HIP(hipMalloc(&P->vtab, sizeof(float2) * vt.size()));
+        ARECON_HIP(hipMalloc(&P->ktab, sizeof(float4) * kt.size()));
+        ARECON_HIP(hipMalloc(&P->ztab, sizeof(BPZ) * zt.size()));
+        ARECON_HIP(hipMemcpy(P->vtab, vt.data(), sizeof(float2) * vt.size(), hipMemcpyHostToDevice));
+        ARECON_HIP(hipMemcpy(P->ktab, kt.data(), sizeof(float4) * kt.size(), hipMemcpyHostToDevice));
+        ARECON_HIP(hipMemcpy(P->ztab, zt.data(), sizeof(BPZ) * zt.size(), hipMemcpyHostToDevice));
+    } catch (...) {
+        bp_plan_destroy(P);
+        throw;
+    }
+    return P;
+}
+
+void bp_plan_destroy(BPPlan* P) {
+    if (!P) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(P->device);
+    if (P->vtab) (void)hipFree(P->vtab);
+    if (P->ktab) (void)hipFree(P->ktab);
+    if (P->ztab) (void)hipFree(P->ztab);
+    (void)hipSetDevice(cur);
+    delete P;
+}
+
+template <int NZ, class T>
+static void launch_nz(const BPPlan* P, float* img, const T* fp, hipStream_t s) {
+    BPArgs a = P->a;
+    a.vec4 = (a.ZN % 4 == 0) && ((reinterpret_cast<uintptr_t>(img) & 15) == 0);
+    hipLaunchKernelGGL((bp_fdk3dw<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp, P->vtab,
+                       P->ktab, P->ztab, a);
+}
+
+template <class T>
+static void launch_t(const BPPlan* P, float* img, const T* fp, hipStream_t s) {
+    switch (P->nz) {
+    case 1: launch_nz<1, T>(P, img, fp, s); break;
+    case 2: launch_nz<2, T>(P, img, fp, s); break;
+    case 4: launch_nz<4, T>(P, img, fp, s); break;
+    default: launch_nz<8, T>(P, img, fp, s); break;
+    }
+}
+
+void bp_plan_launch(const BPPlan* P, float* img, const void* fp, int fp_dtype, hipStream_t s) {
+    ARECON_REQUIRE(P != nullptr, "bp: null plan");
+    ARECON_REQUIRE(img != nullptr && fp != nullptr, "bp: null buffer");
+    switch (fp_dtype) {
+    case ARECON_F32: launch_t<float>(P, img, static_cast<const float*>(fp), s); break;
+    case ARECON_F16: launch_t<__half>(P, img, static_cast<const __half*>(fp), s); break;
+    case ARECON_BF16:
+        launch_t<__hip_bfloat16>(P, img, static_cast<const __hip_bfloat16*>(fp), s);
+        break;
+    default: fail(ARECON_ERR_ARG, "bp: unknown fp dtype");
+    }
+    ARECON_HIP(hipGetLastError());
+}
+
+void bp_launch_f64(double* img, const double* fp, const arecon_bp_geom& g, const double* w,
+                   hipStream_t s) {
+    validate_geom(g);
+    ARECON_REQUIRE(img != nullptr && fp != nullptr && w != nullptr, "bp: null buffer");
+    const long long n = (long long)g.XN * g.YN * g.ZN;
+    const long long grid = (n + 255) / 256;
+    ARECON_REQUIRE(grid < (1LL << 31), "bp: volume too large for one launch");
+    hipLaunchKernelGGL(bp_fdk3dw_f64, dim3((unsigned)grid), dim3(256), 0, s, img, fp, w, g,
+                       kInvTwoPi, kRefPi);
+    ARECON_HIP(hipGetLastError());
+}
+
+}  // namespace arecon

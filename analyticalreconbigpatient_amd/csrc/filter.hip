@@ -1,0 +1,327 @@
+// Cone/row preweighting fused with the ramp filter (gfx950).
+//
+// Reference behaviour (src/filtering.cu:187-229 and :86-175; CPU twin
+// src/analyticalReconBigPatient.m:107-134): every detector row of the
+// [ViewN][ZL][YL] array is multiplied by
+//     SO^2 / sqrt(SO^2 (SO^2 + b^2) - b^2 t^2),  t = (j-PLC) dYL, b = (k-ZLC) dZL,
+// and then linearly convolved with h(n) = -2 / (pi^2 (4n^2 - 1)) / dYL
+// (src/filtering.cu:23).  The reference zero-pads each row to 2^ceil(log2 3YL)
+// complex samples in a separate HBM buffer and runs batched C2C FFTs; the
+// result equals the exact linear convolution restricted to the first YL outputs.
+//
+// Design.  Output j of a row only sees taps |n| <= YL-1, so any transform length
+// N >= 2YL gives the same linear convolution; this kernel uses
+// N = 2^ceil(log2 2YL) (2048 for YL = 888).  The tap sequence is real and even,
+// so its spectrum H is real: two detector rows are packed as the real and
+// imaginary parts of one complex row, and H * FFT(a + ib) transforms back to
+// (a*h) + i(b*h) with no cross talk.  Per pair of rows the whole transform lives
+// in LDS:
+//   load (global, preweight fused) -> radix-4 Stockham passes (+1 radix-2 pass
+//   when log2 N is odd) -> multiply by H/N and conjugate (fused into the last
+//   forward pass) -> the same forward passes again (conj(FFT(conj(Y))) is the
+//   inverse) -> store of the first YL outputs (fused into the last pass),
+//   in FP32, FP16 or BF16.
+// The first pass reads only the nonzero half of the padded row (N >= 2YL means
+// inputs N/2..N-1 are zero).  Twiddles exp(-2 pi i m/N) and H are computed in
+// FP64 on the host, cached per (device, YL, dYL), and staged in LDS once per
+// workgroup; workgroups loop over row pairs.
+// HBM roofline: 8 B per sample FP32 -> FP32 (6 B with a 16-bit output).
+#include "arecon_internal.h"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+namespace arecon {
+namespace {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Radix-4 DFT butterfly (forward sign).
+__device__ __forceinline__ void bfly4(float2& v0, float2& v1, float2& v2, float2& v3) {
+    const float2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
+    const float2 d = csub(v1, v3);
+    const float2 a3 = make_float2(d.y, -d.x);   // -i * d
+    v0 = cadd(a0, a2);
+    v1 = cadd(a1, a3);
+    v2 = csub(a0, a2);
+    v3 = csub(a1, a3);
+}
+
+template <class T> __device__ __forceinline__ T to_out(float v);
+template <> __device__ __forceinline__ float to_out<float>(float v) { return v; }
+template <> __device__ __forceinline__ __half to_out<__half>(float v) { return __float2half(v); }
+template <> __device__ __forceinline__ __hip_bfloat16 to_out<__hip_bfloat16>(float v) {
+    return __float2bfloat16(v);
+}
+
+// Epilogue of a pass: 0 = LDS store, 1 = LDS store of H[d]*conj(v),
+// 2 = store of the first YL outputs of both rows to global memory.
+template <int EPI, class OutT>
+__device__ __forceinline__ void emit(float2* dst, int d, float2 v, const float* H,
+                                     OutT* oa, OutT* ob, int YL, bool has_b) {
+    if constexpr (EPI == 0) {
+        dst[d] = v;
+    } else if constexpr (EPI == 1) {
+        const float hs = H[d];
+        dst[d] = make_float2(hs * v.x, -hs * v.y);
+    } else {
+        if (d < YL) {
+            oa[d] = to_out<OutT>(v.x);
+            if (has_b) ob[d] = to_out<OutT>(-v.y);
+        }
+    }
+}
+
+template <int EPI, class OutT>
+__device__ __forceinline__ void pass4(const float2* src, float2* dst, int N, int Ns,
+                                      const float2* tw, const float* H, OutT* oa,
+                                      OutT* ob, int YL, bool has_b) {
+    const int q = N >> 2;
+    const int tstride = N / (Ns * 4);
+    for (int j = threadIdx.x; j < q; j += blockDim.x) {
+        const int jm = j & (Ns - 1);
+        float2 v0 = src[j], v1 = src[j + q], v2 = src[j + 2 * q], v3 = src[j + 3 * q];
+        const int t = jm * tstride;
+        v1 = cmul(v1, tw[t]);
+        v2 = cmul(v2, tw[2 * t]);
+        v3 = cmul(v3, tw[3 * t]);
+        bfly4(v0, v1, v2, v3);
+        const int d = (j - jm) * 4 + jm;
+        emit<EPI, OutT>(dst, d, v0, H, oa, ob, YL, has_b);
+        emit<EPI, OutT>(dst, d + Ns, v1, H, oa, ob, YL, has_b);
+        emit<EPI, OutT>(dst, d + 2 * Ns, v2, H, oa, ob, YL, has_b);
+        emit<EPI, OutT>(dst, d + 3 * Ns, v3, H, oa, ob, YL, has_b);
+    }
+}
+
+// Final radix-2 pass (Ns = N/2).
+template <int EPI, class OutT>
+__device__ __forceinline__ void pass2_last(const float2* src, float2* dst, int N,
+                                           const float2* tw, const float* H, OutT* oa,
+                                           OutT* ob, int YL, bool has_b) {
+    const int hN = N >> 1;
+    for (int j = threadIdx.x; j < hN; j += blockDim.x) {
+        const float2 v0 = src[j];
+        const float2 v1 = cmul(src[j + hN], tw[j]);
+        emit<EPI, OutT>(dst, j, cadd(v0, v1), H, oa, ob, YL, has_b);
+        emit<EPI, OutT>(dst, j + hN, csub(v0, v1), H, oa, ob, YL, has_b);
+    }
+}
+
+// Passes 2.. of one transform (the first pass has been done into `cur`).
+// Returns the buffer holding the result when EPI_LAST writes to LDS.
+template <int EPI_LAST, class OutT>
+__device__ __forceinline__ float2* passes_after_first(float2* cur, float2* nxt, int N,
+                                                      const float2* tw, const float* H,
+                                                      OutT* oa, OutT* ob, int YL,
+                                                      bool has_b) {
+    int Ns = 4;
+    while (Ns * 4 <= N) {
+        const bool last = (Ns * 4 == N);
+        __syncthreads();
+        if (last)
+            pass4<EPI_LAST, OutT>(cur, nxt, N, Ns, tw, H, oa, ob, YL, has_b);
+        else
+            pass4<0, OutT>(cur, nxt, N, Ns, tw, H, oa, ob, YL, has_b);
+        float2* t = cur; cur = nxt; nxt = t;
+        Ns *= 4;
+    }
+    if (Ns < N) {   // N = 2 * 4^k
+        __syncthreads();
+        pass2_last<EPI_LAST, OutT>(cur, nxt, N, tw, H, oa, ob, YL, has_b);
+        float2* t = cur; cur = nxt; nxt = t;
+    }
+    return cur;
+}
+
+__device__ __forceinline__ float preweight(int j, float b, float PLC, float dYL, float SO) {
+    // src/filtering.cu:202-204 (FP32, as the reference's GPU path computes it)
+    const float t = ((float)j - PLC) * dYL;
+    return SO * SO / sqrtf(SO * SO * (SO * SO + b * b) - b * t * b * t);
+}
+
+template <class OutT>
+__global__ __launch_bounds__(256) void filter_rows(
+    OutT* out, const float* in, const float2* __restrict__ twg,
+    const float* __restrict__ Hg, int YL, int ZL, long long nrows, int N, float PLC,
+    float ZLC, float dYL, float dZL, float SO) {
+    extern __shared__ float2 smem[];
+    float2* bufA = smem;
+    float2* bufB = smem + N;
+    float2* tw = smem + 2 * N;
+    float* H = reinterpret_cast<float*>(smem + 3 * N);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        tw[i] = twg[i];
+        H[i] = Hg[i];
+    }
+    const long long npairs = (nrows + 1) >> 1;
+    const int q = N >> 2;
+    for (long long pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
+        const long long ra = 2 * pair;
+        const bool has_b = (ra + 1) < nrows;
+        const float* ia = in + ra * YL;
+        const float* ib = ia + YL;
+        const int ka = (int)(ra % ZL);
+        const int kb = (ka + 1 == ZL) ? 0 : ka + 1;
+        const float ba = ((float)ka - ZLC) * dZL;
+        const float bb = ((float)kb - ZLC) * dZL;
+        __syncthreads();   // previous pair's readers of bufA/bufB are done
+        // First forward pass (Ns = 1) straight from global memory.  Inputs
+        // n >= N/2 >= YL are zero, so v2 = v3 = 0.
+        for (int j = threadIdx.x; j < q; j += blockDim.x) {
+            float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
+            if (j < YL) {
+                v0.x = ia[j] * preweight(j, ba, PLC, dYL, SO);
+                if (has_b) v0.y = ib[j] * preweight(j, bb, PLC, dYL, SO);
+            }
+            const int j1 = j + q;
+            if (j1 < YL) {
+                v1.x = ia[j1] * preweight(j1, ba, PLC, dYL, SO);
+                if (has_b) v1.y = ib[j1] * preweight(j1, bb, PLC, dYL, SO);
+            }
+            const float2 nv1 = make_float2(v1.y, -v1.x);   // -i * v1
+            bufA[4 * j + 0] = cadd(v0, v1);
+            bufA[4 * j + 1] = cadd(v0, nv1);
+            bufA[4 * j + 2] = csub(v0, v1);
+            bufA[4 * j + 3] = csub(v0, nv1);
+        }
+        OutT* oa = out + ra * YL;
+        OutT* ob = oa + YL;
+        float2* spec = passes_after_first<1, OutT>(bufA, bufB, N, tw, H, oa, ob, YL, has_b);
+        float2* other = (spec == bufA) ? bufB : bufA;
+        // Inverse: forward passes over conj(H X / N); first pass from LDS.
+        __syncthreads();
+        for (int j = threadIdx.x; j < q; j += blockDim.x) {
+            float2 v0 = spec[j], v1 = spec[j + q], v2 = spec[j + 2 * q], v3 = spec[j + 3 * q];
+            bfly4(v0, v1, v2, v3);
+            other[4 * j + 0] = v0;
+            other[4 * j + 1] = v1;
+            other[4 * j + 2] = v2;
+            other[4 * j + 3] = v3;
+        }
+        passes_after_first<2, OutT>(other, spec, N, tw, H, oa, ob, YL, has_b);
+    }
+}
+
+struct FilterTables {
+    float2* tw = nullptr;
+    float* H = nullptr;
+    int N = 0;
+};
+
+std::mutex g_filter_mu;
+std::map<std::tuple<int, int, uint64_t>, FilterTables> g_filter_cache;
+
+int filter_length(int YL) {
+    int N = 64;
+    while (N < 2 * YL) N <<= 1;
+    return N;
+}
+
+// Twiddles and the real spectrum of the tap sequence, FP64 on the host.
+const FilterTables& filter_tables(int device, int YL, float dYL_f) {
+    const double dYL = (double)dYL_f;
+    uint64_t key_bits;
+    std::memcpy(&key_bits, &dYL, sizeof(key_bits));
+    std::lock_guard<std::mutex> lk(g_filter_mu);
+    auto key = std::make_tuple(device, YL, key_bits);
+    auto it = g_filter_cache.find(key);
+    if (it != g_filter_cache.end()) return it->second;
+    const int N = filter_length(YL);
+    std::vector<double> c(N), s(N);
+    for (int m = 0; m < N; ++m) {
+        c[m] = std::cos(2.0 * M_PI * m / N);
+        s[m] = std::sin(2.0 * M_PI * m / N);
+    }
+    std::vector<float2> tw(N);
+    for (int m = 0; m < N; ++m) tw[m] = make_float2((float)c[m], (float)-s[m]);
+    // h(n), src/filtering.cu:23 (pi^2 literal of the reference), |n| <= YL-1.
+    std::vector<double> h(YL);
+    for (int n = 0; n < YL; ++n)
+        h[n] = -2.0 / (9.869604401089358 * (4.0 * n * n - 1.0)) / dYL;
+    std::vector<float> H(N);
+    for (int f = 0; f < N; ++f) {
+        double acc = h[0];
+        long long idx = 0;
+        for (int n = 1; n < YL; ++n) {
+            idx += f;
+            if (idx >= N) idx -= N;
+            acc += 2.0 * h[n] * c[idx];
+        }
+        H[f] = (float)(acc / N);
+    }
+    FilterTables t;
+    t.N = N;
+    ARECON_HIP(hipMalloc(&t.tw, sizeof(float2) * N));
+    ARECON_HIP(hipMalloc(&t.H, sizeof(float) * N));
+    ARECON_HIP(hipMemcpy(t.tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+    ARECON_HIP(hipMemcpy(t.H, H.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    return g_filter_cache.emplace(key, t).first->second;
+}
+
+}  // namespace
+
+void launch_filter(void* out, int out_dtype, const float* in, int YL, int ZL, int ViewN,
+                   float PLC, float ZLC, float dYL, float dZL, float SO, hipStream_t s) {
+    ARECON_REQUIRE(YL > 0 && ZL > 0 && ViewN > 0, "filter: YL, ZL and ViewN must be positive");
+    ARECON_REQUIRE(YL <= 2048, "filter: YL above 2048 is not supported");
+    ARECON_REQUIRE(out != nullptr && in != nullptr, "filter: null buffer");
+    ARECON_REQUIRE(out_dtype == ARECON_F32 || out_dtype == ARECON_F16 || out_dtype == ARECON_BF16,
+                   "filter: unknown output dtype");
+    ARECON_REQUIRE(out_dtype == ARECON_F32 || out != (const void*)in,
+                   "filter: a 16-bit output cannot alias the FP32 input");
+    ARECON_REQUIRE(dYL != 0.f && SO > 0.f, "filter: dYL must be nonzero and SO positive");
+    int dev = 0;
+    ARECON_HIP(hipGetDevice(&dev));
+    const FilterTables& t = filter_tables(dev, YL, dYL);
+    const int N = t.N;
+    const size_t lds = sizeof(float2) * 3 * (size_t)N + sizeof(float) * (size_t)N;
+    const long long nrows = (long long)ZL * ViewN;
+    const long long npairs = (nrows + 1) / 2;
+    int cus = 0;
+    ARECON_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
+    const long long grid = std::min<long long>(npairs, (long long)cus * per_cu);
+    static std::once_flag lds_once;
+    std::call_once(lds_once, [] {
+        // Transforms of 4096 points need more than the default 64 KB of LDS.
+        const int max_lds = 160 * 1024;
+        (void)hipFuncSetAttribute((const void*)filter_rows<float>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+        (void)hipFuncSetAttribute((const void*)filter_rows<__half>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+        (void)hipFuncSetAttribute((const void*)filter_rows<__hip_bfloat16>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    });
+    switch (out_dtype) {
+    case ARECON_F32:
+        hipLaunchKernelGGL(filter_rows<float>, dim3((unsigned)grid), dim3(256), lds, s,
+                           static_cast<float*>(out), in, t.tw, t.H, YL, ZL, nrows, N, PLC,
+                           ZLC, dYL, dZL, SO);
+        break;
+    case ARECON_F16:
+        hipLaunchKernelGGL(filter_rows<__half>, dim3((unsigned)grid), dim3(256), lds, s,
+                           static_cast<__half*>(out), in, t.tw, t.H, YL, ZL, nrows, N, PLC,
+                           ZLC, dYL, dZL, SO);
+        break;
+    default:
+        hipLaunchKernelGGL(filter_rows<__hip_bfloat16>, dim3((unsigned)grid), dim3(256), lds,
+                           s, static_cast<__hip_bfloat16*>(out), in, t.tw, t.H, YL, ZL,
+                           nrows, N, PLC, ZLC, dYL, dZL, SO);
+        break;
+    }
+    ARECON_HIP(hipGetLastError());
+}
+
+}  // namespace arecon

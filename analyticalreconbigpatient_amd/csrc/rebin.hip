@@ -1,0 +1,276 @@
+// Fan-to-parallel rebinning on the curved detector (gfx950).
+//
+// Behaviour follows the reference's CPU path, src/ParallelRebinningCBCurve.cpp:22-97:
+// for parallel view i and channel j,
+//   Theta = i*DeltaTheta, t = (j-PLC)*DeltaT, Beta = asin(t/SO),
+//   FaiIndex = (Theta+Beta)/DeltaFai, UIndex = atan(t/sqrt(SO^2-t^2))/dYA + YLC,
+// a 2x2 linear blend of fan views ceil(FaiIndex)-1/ceil(FaiIndex) and channels
+// ceil(UIndex)-1/ceil(UIndex), with the CPU path's ">" edge clamps (:55-85), and
+// rows passed straight through (:87-94).
+//
+// Design.  Everything that depends on the channel only (Beta, the channel pair
+// and its weights) is computed once per call on the host, in the reference's
+// own arithmetic (double libm rounded to float), and read from a small table.
+// The kernels do the view-dependent part with round-to-nearest intrinsics and no
+// FMA contraction, so their result is bit-identical to the float CPU path.
+//
+//   rebin_vyz  output [ViewN][YL][ZL] (reference layout): one thread per
+//              4 consecutive rows of one (view, channel), float4 loads/stores.
+//   rebin_vzy  output [ViewN][ZL][YL] (the filter's layout): one workgroup per
+//              (view, 64-channel block); the ZL-contiguous input vectors are
+//              blended with coalesced loads into an LDS tile and written out
+//              transposed with coalesced stores.
+// HBM roofline: 8 B per sample (one read, one write); the 4-tap gather is
+// served by L2 (neighbouring channels share fan views).
+#include "arecon_internal.h"
+
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace arecon {
+namespace {
+
+struct ChanCoef {
+    float beta;   // asin(t/SO), rounded to float
+    float cyb;    // coeYB
+    float cyu;    // coeYU
+    int iyb;      // IndexYB
+    int iyu;      // IndexYU
+    int pad[3];
+};
+static_assert(sizeof(ChanCoef) == 32, "ChanCoef layout");
+
+// Host: channel table in the reference's arithmetic (:37-48, :71-85).
+std::vector<ChanCoef> channel_table(int YL, float YLC, float dYA, float PLC,
+                                    float DeltaT, float SO) {
+    std::vector<ChanCoef> c(YL);
+    for (int j = 0; j < YL; ++j) {
+        volatile float t = ((float)j - PLC) * DeltaT;
+        volatile float q = t / SO;
+        volatile float beta = (float)std::asin((double)q);
+        volatile float r2 = SO * SO - t * t;
+        volatile float a = (float)std::atan((double)t / std::sqrt((double)r2));
+        volatile float ad = a / dYA;
+        volatile float uindex = ad + YLC;
+        const int UI = (int)std::ceil((double)uindex);
+        volatile float cyb = (float)UI - uindex;
+        volatile float cyu = (float)(1.0 - (double)cyb);
+        ChanCoef e{};
+        e.beta = beta;
+        e.cyb = cyb;
+        e.cyu = cyu;
+        if (UI <= 0) { e.iyb = 0; e.iyu = 0; }
+        else if (UI > YL - 1) { e.iyb = YL - 1; e.iyu = YL - 1; }
+        else { e.iyb = UI - 1; e.iyu = UI; }
+        c[j] = e;
+    }
+    return c;
+}
+
+// Device copies of channel tables, keyed by (device, geometry bits).  A table
+// is uploaded once with a blocking copy and kept for the life of the process
+// (the cache is flushed, after a device synchronisation, when it grows past
+// 64 geometries).
+std::mutex g_chan_mu;
+std::map<std::array<uint32_t, 7>, ChanCoef*> g_chan_cache;
+
+uint32_t bits(float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    return u;
+}
+
+const ChanCoef* channel_table_dev(int YL, float YLC, float dYA, float PLC, float DeltaT,
+                                  float SO) {
+    int dev = 0;
+    ARECON_HIP(hipGetDevice(&dev));
+    const std::array<uint32_t, 7> key{(uint32_t)dev, (uint32_t)YL, bits(YLC), bits(dYA),
+                                      bits(PLC), bits(DeltaT), bits(SO)};
+    std::lock_guard<std::mutex> lk(g_chan_mu);
+    auto it = g_chan_cache.find(key);
+    if (it != g_chan_cache.end()) return it->second;
+    if (g_chan_cache.size() >= 64) {
+        ARECON_HIP(hipDeviceSynchronize());
+        for (auto& kv : g_chan_cache) {
+            (void)hipSetDevice((int)kv.first[0]);
+            (void)hipFree(kv.second);
+        }
+        g_chan_cache.clear();
+        ARECON_HIP(hipSetDevice(dev));
+    }
+    const std::vector<ChanCoef> tab = channel_table(YL, YLC, dYA, PLC, DeltaT, SO);
+    ChanCoef* d = nullptr;
+    ARECON_HIP(hipMalloc(&d, sizeof(ChanCoef) * YL));
+    ARECON_HIP(hipMemcpy(d, tab.data(), sizeof(ChanCoef) * YL, hipMemcpyHostToDevice));
+    g_chan_cache.emplace(key, d);
+    return d;
+}
+
+// View-dependent part of the blend weights (:34,:39,:41,:43,:45-46,:55-69),
+// with explicit round-to-nearest operations so no FMA is formed.
+struct Blend {
+    float c00, c10, c01, c11;   // (XB,YB) (XU,YB) (XB,YU) (XU,YU)
+    int ixb, ixu;
+};
+
+__device__ __forceinline__ Blend view_blend(int i, const ChanCoef& cc, int ViewN,
+                                            float DeltaTheta, float DeltaFai) {
+    const float theta = __fmul_rn((float)i, DeltaTheta);
+    const float fai = __fadd_rn(theta, cc.beta);
+    const float fidx = __fdiv_rn(fai, DeltaFai);
+    const float fic = ceilf(fidx);
+    const int FI = (int)fic;
+    const float cxb = __fsub_rn(fic, fidx);
+    const float cxu = __fsub_rn(1.0f, cxb);
+    Blend b;
+    if (FI <= 0) { b.ixb = 0; b.ixu = 0; }
+    else if (FI > ViewN - 1) { b.ixb = ViewN - 1; b.ixu = ViewN - 1; }
+    else { b.ixb = FI - 1; b.ixu = FI; }
+    b.c00 = __fmul_rn(cxb, cc.cyb);
+    b.c10 = __fmul_rn(cxu, cc.cyb);
+    b.c01 = __fmul_rn(cxb, cc.cyu);
+    b.c11 = __fmul_rn(cxu, cc.cyu);
+    return b;
+}
+
+// ((c00*p0 + c10*p1) + c01*p2) + c11*p3, each operation rounded (:89-93).
+__device__ __forceinline__ float blend4(const Blend& b, float p0, float p1, float p2,
+                                        float p3) {
+    float acc = __fadd_rn(__fmul_rn(b.c00, p0), __fmul_rn(b.c10, p1));
+    acc = __fadd_rn(acc, __fmul_rn(b.c01, p2));
+    return __fadd_rn(acc, __fmul_rn(b.c11, p3));
+}
+
+// [ViewN][YL][ZL] output, ZL % 4 == 0: one thread per float4 of rows.
+__global__ __launch_bounds__(256) void rebin_vyz_v4(
+    float4* __restrict__ out, const float4* __restrict__ in,
+    const ChanCoef* __restrict__ chan, int YL, int ZL4, int ViewN, float DeltaTheta,
+    float DeltaFai, long long total) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const int kq = (int)(gid % ZL4);
+    const long long ij = gid / ZL4;
+    const int j = (int)(ij % YL);
+    const int i = (int)(ij / YL);
+    const ChanCoef cc = chan[j];
+    const Blend b = view_blend(i, cc, ViewN, DeltaTheta, DeltaFai);
+    const float4 p0 = in[((long long)b.ixb * YL + cc.iyb) * ZL4 + kq];
+    const float4 p1 = in[((long long)b.ixu * YL + cc.iyb) * ZL4 + kq];
+    const float4 p2 = in[((long long)b.ixb * YL + cc.iyu) * ZL4 + kq];
+    const float4 p3 = in[((long long)b.ixu * YL + cc.iyu) * ZL4 + kq];
+    float4 o;
+    o.x = blend4(b, p0.x, p1.x, p2.x, p3.x);
+    o.y = blend4(b, p0.y, p1.y, p2.y, p3.y);
+    o.z = blend4(b, p0.z, p1.z, p2.z, p3.z);
+    o.w = blend4(b, p0.w, p1.w, p2.w, p3.w);
+    out[gid] = o;
+}
+
+// [ViewN][YL][ZL] output, any ZL: one thread per sample.
+__global__ __launch_bounds__(256) void rebin_vyz_s(
+    float* __restrict__ out, const float* __restrict__ in,
+    const ChanCoef* __restrict__ chan, int YL, int ZL, int ViewN, float DeltaTheta,
+    float DeltaFai, long long total) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const int k = (int)(gid % ZL);
+    const long long ij = gid / ZL;
+    const int j = (int)(ij % YL);
+    const int i = (int)(ij / YL);
+    const ChanCoef cc = chan[j];
+    const Blend b = view_blend(i, cc, ViewN, DeltaTheta, DeltaFai);
+    const float p0 = in[((long long)b.ixb * YL + cc.iyb) * ZL + k];
+    const float p1 = in[((long long)b.ixu * YL + cc.iyb) * ZL + k];
+    const float p2 = in[((long long)b.ixb * YL + cc.iyu) * ZL + k];
+    const float p3 = in[((long long)b.ixu * YL + cc.iyu) * ZL + k];
+    out[gid] = blend4(b, p0, p1, p2, p3);
+}
+
+// [ViewN][ZL][YL] output.  Block = (view i, channels j0..j0+JB-1).
+// LDS tile[ZL][JB+1] (padded against bank conflicts on the transposed read).
+template <int JB>
+__global__ __launch_bounds__(256) void rebin_vzy(
+    float* __restrict__ out, const float* __restrict__ in,
+    const ChanCoef* __restrict__ chan, int YL, int ZL, int ViewN, float DeltaTheta,
+    float DeltaFai) {
+    extern __shared__ float tile[];             // ZL * (JB + 1)
+    __shared__ Blend sb[JB];
+    __shared__ int sbase[JB][4];
+    const int i = blockIdx.y;
+    const int j0 = blockIdx.x * JB;
+    const int nj = min(JB, YL - j0);
+    const int tid = threadIdx.x;
+    if (tid < nj) {
+        const ChanCoef cc = chan[j0 + tid];
+        const Blend b = view_blend(i, cc, ViewN, DeltaTheta, DeltaFai);
+        sb[tid] = b;
+        sbase[tid][0] = b.ixb * YL + cc.iyb;
+        sbase[tid][1] = b.ixu * YL + cc.iyb;
+        sbase[tid][2] = b.ixb * YL + cc.iyu;
+        sbase[tid][3] = b.ixu * YL + cc.iyu;
+    }
+    __syncthreads();
+    const int n = nj * ZL;
+    for (int e = tid; e < n; e += blockDim.x) {
+        const int jj = e / ZL;
+        const int k = e - jj * ZL;
+        const Blend b = sb[jj];
+        const float p0 = in[(long long)sbase[jj][0] * ZL + k];
+        const float p1 = in[(long long)sbase[jj][1] * ZL + k];
+        const float p2 = in[(long long)sbase[jj][2] * ZL + k];
+        const float p3 = in[(long long)sbase[jj][3] * ZL + k];
+        tile[k * (JB + 1) + jj] = blend4(b, p0, p1, p2, p3);
+    }
+    __syncthreads();
+    for (int e = tid; e < ZL * JB; e += blockDim.x) {
+        const int k = e / JB;
+        const int jj = e - k * JB;
+        if (jj < nj)
+            out[((long long)i * ZL + k) * YL + j0 + jj] = tile[k * (JB + 1) + jj];
+    }
+}
+
+}  // namespace
+
+void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN, float YLC,
+                  float dYA, float DeltaTheta, float PLC, float DeltaT, float DeltaFai,
+                  float SO, int out_layout, hipStream_t s) {
+    ARECON_REQUIRE(YL > 0 && ZL > 0 && ViewN > 0, "rebin: YL, ZL and ViewN must be positive");
+    ARECON_REQUIRE(out != nullptr && in != nullptr, "rebin: null buffer");
+    ARECON_REQUIRE(out != in, "rebin: input and output must not alias");
+    ARECON_REQUIRE((long long)YL * ViewN < (1LL << 31),
+                   "rebin: YL*ViewN must fit in 31 bits");
+    ARECON_REQUIRE(out_layout == ARECON_LAYOUT_VYZ || out_layout == ARECON_LAYOUT_VZY,
+                   "rebin: unknown output layout");
+    const ChanCoef* chan = channel_table_dev(YL, YLC, dYA, PLC, DeltaT, SO);
+    if (out_layout == ARECON_LAYOUT_VYZ) {
+        if (ZL % 4 == 0) {
+            const long long total = (long long)ViewN * YL * (ZL / 4);
+            const unsigned grid = (unsigned)((total + 255) / 256);
+            hipLaunchKernelGGL(rebin_vyz_v4, dim3(grid), dim3(256), 0, s,
+                               reinterpret_cast<float4*>(out),
+                               reinterpret_cast<const float4*>(in), chan,
+                               YL, ZL / 4, ViewN, DeltaTheta, DeltaFai, total);
+        } else {
+            const long long total = (long long)ViewN * YL * ZL;
+            const unsigned grid = (unsigned)((total + 255) / 256);
+            hipLaunchKernelGGL(rebin_vyz_s, dim3(grid), dim3(256), 0, s, out, in,
+                               chan, YL, ZL, ViewN, DeltaTheta, DeltaFai,
+                               total);
+        }
+    } else {
+        constexpr int JB = 64;
+        const size_t lds = sizeof(float) * (size_t)ZL * (JB + 1);
+        ARECON_REQUIRE(lds <= 64 * 1024, "rebin: ZL too large for the transposing kernel");
+        dim3 grid((YL + JB - 1) / JB, ViewN);
+        hipLaunchKernelGGL(rebin_vzy<JB>, grid, dim3(256), lds, s, out, in,
+                           chan, YL, ZL, ViewN, DeltaTheta, DeltaFai);
+    }
+    ARECON_HIP(hipGetLastError());
+}
+
+}  // namespace arecon

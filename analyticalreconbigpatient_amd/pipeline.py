@@ -1,0 +1,176 @@
+"""Device-resident helical FDK pipeline: rebin -> preweight + filter -> backproject.
+
+The reference runs its three stages as separate MEX calls, each copying the
+whole sinogram host -> device -> host, with host-side permutes in between
+(src/analyticalReconBigPatient.m:79,138-146, src/Parallel_Rebinning_CB_Curve.m:20-23).
+Here the sinogram stays in HBM: the rebinning writes the filter's
+``[ViewN][ZL][YL]`` layout directly, the filter runs in place (FP32) or into a
+16-bit buffer, and the backprojector reads the trimmed view range by pointer
+offset.  torch supplies device memory and the stream; all compute is in
+libarecon.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .geometry import (BPParams, ReconSetup, bp_params, bp_scan_geom, filter_args, rebin_args,
+                       voxel_updates)
+
+STORAGE = {"f32": (_lib.ARECON_F32, torch.float32), "f16": (_lib.ARECON_F16, torch.float16),
+           "bf16": (_lib.ARECON_BF16, torch.bfloat16)}
+
+
+def _stream_ptr(stream: torch.cuda.Stream | None) -> C.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def fdk_weights(N_2pi: int, HSCoef: float, delta: int) -> np.ndarray:
+    """The reference's weight table (src/Parallel_FDK_Helical_3DWeighting.cpp:156-170),
+    built by libarecon.so."""
+    w = np.empty(N_2pi, dtype=np.float64)
+    _lib.check(_lib.lib().arecon_fdk_weights(_lib.dptr(w), N_2pi, HSCoef, delta),
+               "arecon_fdk_weights")
+    return w
+
+
+class BPPlan:
+    """Owner of an ``arecon_bp_plan`` (tables built in FP64, stored FP32)."""
+
+    def __init__(self, params: BPParams, *, dz: float | None = None, nz: int = 0,
+                 order: int = 0, w: np.ndarray | None = None):
+        self.params = params
+        self.geom = params.to_geom(dz=dz, nz=nz, order=order)
+        self.w = fdk_weights(params.N_2pi, params.HSCoef, params.delta) if w is None else \
+            np.ascontiguousarray(w, dtype=np.float64)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().arecon_bp_plan_create(C.byref(self.geom), _lib.dptr(self.w),
+                                                    C.byref(h)), "arecon_bp_plan_create")
+        self._h = h
+
+    def execute(self, img: torch.Tensor, fp: torch.Tensor, stream=None) -> None:
+        g = self.geom
+        dtype = {torch.float32: _lib.ARECON_F32, torch.float16: _lib.ARECON_F16,
+                 torch.bfloat16: _lib.ARECON_BF16}[fp.dtype]
+        assert img.is_cuda and fp.is_cuda and img.dtype == torch.float32
+        assert img.is_contiguous() and fp.is_contiguous()
+        assert img.numel() == g.XN * g.YN * g.ZN, "volume size does not match the plan"
+        assert fp.numel() >= g.PN * g.ZL * g.YL, "fp holds fewer than PN views"
+        _lib.check(_lib.lib().arecon_bp_plan_execute(self._h, C.c_void_p(img.data_ptr()),
+                                                     C.c_void_p(fp.data_ptr()), dtype,
+                                                     _stream_ptr(stream)),
+                   "arecon_bp_plan_execute")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().arecon_bp_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rebin_dev(out: torch.Tensor, raw: torch.Tensor, setup_geom, layout: int, stream=None):
+    a = rebin_args(setup_geom)
+    assert raw.dtype == torch.float32 and raw.is_contiguous() and out.is_contiguous()
+    assert raw.numel() == a.YL * a.ZL * a.ViewN and out.numel() == raw.numel()
+    _lib.check(_lib.lib().arecon_rebin_dev(
+        C.c_void_p(out.data_ptr()), C.c_void_p(raw.data_ptr()), a.YL, a.ZL, a.ViewN, a.YLC0,
+        a.dYA, a.DeltaTheta, a.PLC0, a.DeltaT, a.DeltaFai, a.SO, layout, _stream_ptr(stream)),
+        "arecon_rebin_dev")
+
+
+def filter_dev(out: torch.Tensor, inp: torch.Tensor, setup: ReconSetup, stream=None):
+    fa = filter_args(setup)
+    ViewN, ZL, YL = inp.shape
+    dtype = {torch.float32: _lib.ARECON_F32, torch.float16: _lib.ARECON_F16,
+             torch.bfloat16: _lib.ARECON_BF16}[out.dtype]
+    assert inp.dtype == torch.float32 and inp.is_contiguous() and out.is_contiguous()
+    assert out.numel() == inp.numel()
+    _lib.check(_lib.lib().arecon_filter_dev(
+        C.c_void_p(out.data_ptr()), dtype, C.c_void_p(inp.data_ptr()), YL, ZL, ViewN, fa.PLC0,
+        fa.ZLC0, fa.dYL, fa.dZL, fa.SO, _stream_ptr(stream)), "arecon_filter_dev")
+
+
+@dataclass
+class StageTimes:
+    rebin_ms: float
+    filter_ms: float
+    bp_ms: float
+
+    @property
+    def total_ms(self) -> float:
+        return self.rebin_ms + self.filter_ms + self.bp_ms
+
+
+class HelicalFDK:
+    """One segment's reconstruction, all stages on one device.
+
+    ``first_view``/``n_views`` select the backprojected views of the rebinned
+    sinogram (the segment trimming of src/analyticalReconBigPatient.m:80-93);
+    ``bp_geom`` is the ScanGeom after trimming (BetaS shifted accordingly).
+    ``storage`` is the element type of the filtered sinogram.
+    """
+
+    def __init__(self, setup: ReconSetup, *, storage: str = "f32", first_view: int = 0,
+                 n_views: int | None = None, bp_geom=None, dz: float | None = None,
+                 nz: int = 0, order: int = 0, device: str = "cuda"):
+        self.setup = setup
+        self.ra = rebin_args(setup.geom)
+        self.ViewN, self.YL, self.ZL = self.ra.ViewN, self.ra.YL, self.ra.ZL
+        self.first_view = first_view
+        self.n_views = self.ViewN - first_view if n_views is None else n_views
+        g = bp_scan_geom(setup, geom=bp_geom, n_views=self.n_views)
+        self.params = bp_params(g)
+        self.dz = dz
+        self.plan = BPPlan(self.params, dz=dz, nz=nz, order=order)
+        self.storage = storage
+        code, tdtype = STORAGE[storage]
+        dev = torch.device(device)
+        shape = (self.ViewN, self.ZL, self.YL)
+        self.rebinned = torch.empty(shape, dtype=torch.float32, device=dev)
+        self.filtered = self.rebinned if storage == "f32" else torch.empty(shape, dtype=tdtype,
+                                                                           device=dev)
+        p = self.params
+        self.volume = torch.empty((p.YN, p.XN, p.ZN), dtype=torch.float32, device=dev)
+        self.updates = voxel_updates(p, dz)
+
+    def rebin(self, raw: torch.Tensor, stream=None) -> torch.Tensor:
+        rebin_dev(self.rebinned, raw, self.setup.geom, _lib.ARECON_LAYOUT_VZY, stream)
+        return self.rebinned
+
+    def filter(self, stream=None) -> torch.Tensor:
+        filter_dev(self.filtered, self.rebinned, self.setup, stream)
+        return self.filtered
+
+    def backproject(self, stream=None) -> torch.Tensor:
+        fp = self.filtered[self.first_view:self.first_view + self.n_views]
+        self.plan.execute(self.volume, fp, stream)
+        return self.volume
+
+    def run(self, raw: torch.Tensor, stream=None) -> torch.Tensor:
+        self.rebin(raw, stream)
+        self.filter(stream)
+        return self.backproject(stream)
+
+    def run_timed(self, raw: torch.Tensor) -> StageTimes:
+        """One pass with HIP events around each stage on the current stream."""
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        self.rebin(raw)
+        ev[1].record()
+        self.filter()
+        ev[2].record()
+        self.backproject()
+        ev[3].record()
+        ev[3].synchronize()
+        return StageTimes(ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]),
+                          ev[2].elapsed_time(ev[3]))

@@ -1,0 +1,200 @@
+/*
+ * arecon.h -- C ABI of the MI355X (gfx950) helical cone-beam FDK library.
+ *
+ * Two layers:
+ *
+ *  1. Drop-ins for the reference's extern "C" host functions.  Same names
+ *     (including the reference's "Parallek" spelling), same argument lists,
+ *     same memory layouts, host pointers, synchronous on return.  They run on
+ *     the device named by the ARECON_DEVICE environment variable (default 0)
+ *     on a library-owned stream, and report failures through
+ *     arecon_last_error() (the reference's functions return void and check
+ *     nothing; these never call exit()).
+ *
+ *  2. Native device-resident entry points (arecon_*): device pointers, an
+ *     explicit hipStream_t passed as void*, int status (0 = ok).  These are
+ *     what a device-resident pipeline calls; nothing is copied to the host.
+ *
+ * Layouts (all row-major, last index fastest):
+ *   fan / rebinned sinogram, reference MEX layout   [ViewN][YL][ZL]
+ *   filter input / output, BP input                 [ViewN][ZL][YL]
+ *   reconstructed volume                            [YN][XN][ZN]
+ */
+#ifndef ARECON_H_
+#define ARECON_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define ARECON_OK 0
+#define ARECON_ERR_ARG 1      /* invalid argument or unsupported shape */
+#define ARECON_ERR_HIP 2      /* a HIP runtime call failed */
+#define ARECON_ERR_NOGPU 3    /* no usable gfx950 device */
+#define ARECON_ERR_INTERNAL 4
+
+/* ---- element types for filtered projections ------------------------------- */
+#define ARECON_F32 0
+#define ARECON_F16 1
+#define ARECON_BF16 2
+
+/* ---- rebinning output layouts --------------------------------------------- */
+#define ARECON_LAYOUT_VYZ 0   /* [ViewN][YL][ZL], the reference's rebin layout */
+#define ARECON_LAYOUT_VZY 1   /* [ViewN][ZL][YL], the filter's input layout */
+
+/* ========================================================================== */
+/* 1. Reference drop-ins                                                      */
+/* ========================================================================== */
+
+/* Fan-to-parallel rebinning on the curved detector.
+ * Replaces src/ParallelRebinningCBCurve.h:4-8 (definition
+ * src/ParallelRebinningCBCurveGPU.cu:106-115).  YLC and PLC are 0-based, as
+ * the MEX gateway passes them (src/ParallelRebinningCBCurve.cpp:121-122).
+ * Layout [ViewN][YL][ZL] in and out.  Edge clamps follow the reference's CPU
+ * path (src/ParallelRebinningCBCurve.cpp:55-85). */
+void ParallelRebinningCBCurve_GPU(float* outputProj, const float* Proj, const int YL,
+                                  const int ZL, const int ViewN, const float YLC,
+                                  const float dYA, const float DeltaTheta,
+                                  const float PLC, const float DeltaT,
+                                  const float DeltaFai, const float SO);
+
+/* Cone/row preweighting followed by the Shepp-Logan-type ramp filter
+ * h(n) = -2/(pi^2 (4n^2-1))/dYL as an exact linear convolution.
+ * Replaces src/filtering.h:12-18 (definition src/filtering.cu:270-287).
+ * PLC and ZLC are 0-based (src/analyticalReconBigPatient.m:139).
+ * Layout [ViewN][ZL][YL] in and out. */
+void filtering(float* hfpwd, const float* hProj, const int YL, const int ZL,
+               const int ViewN, const float PLC, const float ZLC, const float dYL,
+               const float dZL, const float SO);
+
+/* 3D-weighted helical FDK backprojection, FP32.
+ * Replaces src/ParallelFDKHelical3DWeightingGPU.h:13-22 (definition
+ * src/ParallelFDKHelical3DWeightingGPU.cu:144-159).  The view loop covers the
+ * voxel's window [s1, s2] as the reference's CPU path does
+ * (src/Parallel_FDK_Helical_3DWeighting.cpp:71).  hw holds N_2pi weights. */
+void ParallekFDKHelical3DWeighting_GPU_float(
+    float* hctimage, const float* hfp, const float* hw, const int N_2pi, const int XN,
+    const int YN, const int ZN, const float XNC, const float YNC, const float ZNC,
+    const float dx, const float dy, const float dz, const int YL, const int ZL,
+    const int PN, const float YLC, const float ZLC, const float dYL, const float dZL,
+    const float h, const float BetaS, const float DeltaFai, const float N_pi,
+    const float HSCoef, const float SO, const float SD, const float k1);
+
+/* Same in FP64 throughout.
+ * Replaces src/ParallelFDKHelical3DWeightingGPU.h:4-12 (definition
+ * src/ParallelFDKHelical3DWeightingGPU.cu:125-140). */
+void ParallekFDKHelical3DWeighting_GPU_double(
+    double* hctimage, const double* hfp, const double* hw, const int N_2pi,
+    const int XN, const int YN, const int ZN, const double XNC, const double YNC,
+    const double ZNC, const double dx, const double dy, const double dz, const int YL,
+    const int ZL, const int PN, const double YLC, const double ZLC, const double dYL,
+    const double dZL, const double h, const double BetaS, const double DeltaFai,
+    const double N_pi, const double HSCoef, const double SO, const double SD,
+    const double k1);
+
+/* Mid-level entry: derives dx, dy, dz (= 2 ObjR / N), dYL, dZL, DeltaFai and
+ * SD, builds the weight table w[N_2pi], and backprojects.
+ * Replaces src/Parallel_FDK_Helical_3DWeighting.cpp:117-143.  useGPU == 1
+ * runs the FP32 GPU path exactly as the reference's useGPU == 1 branch does
+ * (FP32 copy of fp, result widened to double).  Any other useGPU value is an
+ * error: this library is the GPU path and has no CPU implementation. */
+void ParallelFDKHelical3DWeighting(double* ctimage, const double* fp, const double SO,
+                                   const double DO, const int YL, const int ZL,
+                                   const double DecWidth, const double DecHeigh,
+                                   const double YLC, const double ZLC, const double h,
+                                   const double BetaS, const int PN, const int N_2pi,
+                                   const double ObjR, const int XN, const int YN,
+                                   const int ZN, const double XNC, const double YNC,
+                                   const double ZNC, const int delta,
+                                   const double HSCoef, const double k1,
+                                   const int useGPU);
+
+/* ========================================================================== */
+/* 2. Native API                                                              */
+/* ========================================================================== */
+
+/* Last error of the calling thread: returns its code (0 = none) and copies a
+ * NUL-terminated message into buf (if buf != NULL and len > 0). */
+int arecon_last_error(char* buf, size_t len);
+
+/* Library version as 10000*major + 100*minor + patch. */
+int arecon_version(void);
+
+/* Number of HIP devices visible (0 if none), without creating contexts. */
+int arecon_device_count(void);
+
+/* Rebinning on device buffers.  out_layout selects ARECON_LAYOUT_VYZ (the
+ * reference layout) or ARECON_LAYOUT_VZY (written transposed, ready for the
+ * filter; removes the host permutes of src/Parallel_Rebinning_CB_Curve.m:20,23
+ * and src/analyticalReconBigPatient.m:138).  in and out must not alias. */
+int arecon_rebin_dev(float* d_out, const float* d_in, int YL, int ZL, int ViewN,
+                     float YLC, float dYA, float DeltaTheta, float PLC, float DeltaT,
+                     float DeltaFai, float SO, int out_layout, void* stream);
+
+/* Preweight + ramp filter on device buffers, layout [ViewN][ZL][YL].
+ * d_out may equal d_in when out_dtype is ARECON_F32. */
+int arecon_filter_dev(void* d_out, int out_dtype, const float* d_in, int YL, int ZL,
+                      int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
+                      void* stream);
+
+/* Backprojection geometry: the derived quantities the reference passes to its
+ * kernel (src/ParallelFDKHelical3DWeightingGPU.h:13-22), kept in FP64. */
+typedef struct arecon_bp_geom {
+    int N_2pi, XN, YN, ZN, YL, ZL, PN;
+    double XNC, YNC, ZNC;       /* 0-based volume centre */
+    double dx, dy, dz;
+    double YLC, ZLC;            /* 0-based detector centre */
+    double dYL, dZL;
+    double h;                   /* helical pitch (table travel per 2 pi) */
+    double BetaS, DeltaFai;     /* first view angle, view step */
+    double N_pi, HSCoef;        /* N_2pi / 2, half-scan coefficient */
+    double SO, SD, k1;
+    int nz;                     /* slices per thread: 0 = automatic, else 1/2/4/8 */
+    int order;                  /* workgroup order: 0 = slice-major, 1 = column-major */
+} arecon_bp_geom;
+
+typedef struct arecon_bp_plan arecon_bp_plan;
+
+/* Builds the per-view, per-slice and per-offset tables (FP64 on the host,
+ * stored FP32 on the device of the current HIP context) for geometry g and
+ * weight table w[N_2pi].  *out receives the plan. */
+int arecon_bp_plan_create(const arecon_bp_geom* g, const double* w,
+                          arecon_bp_plan** out);
+int arecon_bp_plan_destroy(arecon_bp_plan* plan);
+
+/* Backprojects d_fp ([PN][ZL][YL], element type fp_dtype) into d_img
+ * ([YN][XN][ZN], FP32; every voxel is written). */
+int arecon_bp_plan_execute(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
+                           int fp_dtype, void* stream);
+
+/* FP64 backprojection on device buffers (same math in double throughout).
+ * Synchronises `stream` before returning (w is a host buffer). */
+int arecon_bp_dev_f64(double* d_img, const double* d_fp, const arecon_bp_geom* g,
+                      const double* w, void* stream);
+
+/* Synthetic data (no counterpart in the reference, which ships no data):
+ * exact line integrals of nE ellipsoids (rows of 8 floats: cx, cy, cz, a, b,
+ * c, phi [rad, rotation about z], mu [1/mm]) along the rays of a helical
+ * curved-detector scan, written as [ViewN][YL][ZL] FP32 (the rebinning input
+ * layout, channels already in the flipped order of
+ * src/analyticalReconBigPatient.m:30).  View i has source angle
+ * BetaS + i*dBeta and source height h*angle/(2 pi); channel c has fan angle
+ * (c - YLC0)*dYA; row k has height (k - ZLC0)*dZL at distance SD.
+ * Synchronises `stream` before returning. */
+int arecon_project_ellipsoids_dev(float* d_out, const float* ellipsoids, int nE, int YL, int ZL,
+                                  int ViewN, double BetaS, double dBeta, double SO, double SD,
+                                  double h, double YLC0, double dYA, double ZLC0, double dZL,
+                                  void* stream);
+
+/* Builds the reference's weight table w[N_2pi]
+ * (src/Parallel_FDK_Helical_3DWeighting.cpp:156-170) on the host. */
+int arecon_fdk_weights(double* w, int N_2pi, double HSCoef, int delta);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARECON_H_ */
