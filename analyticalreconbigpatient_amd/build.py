@@ -51,10 +51,13 @@ def _run_parallel(cmds: list[list[str]]) -> None:
     failed = []
     for c, p in procs:
         out, _ = p.communicate()
+        text = out.decode(errors="replace")
+        # hipcc's host pass does not know the GPU-only feature flag: harmless
+        text = "\n".join(l for l in text.splitlines() if "packed-fp32-ops" not in l)
         if p.returncode != 0:
-            failed.append((c, out.decode(errors="replace")))
-        elif out:
-            sys.stderr.write(out.decode(errors="replace"))
+            failed.append((c, text))
+        elif text.strip():
+            sys.stderr.write(text + "\n")
     if failed:
         msg = "\n\n".join(" ".join(c) + "\n" + o for c, o in failed)
         raise RuntimeError(f"compilation failed:\n{msg}")
@@ -91,9 +94,40 @@ def build_oracle(verbose: bool = False) -> Path:
     return ORACLE_LIB
 
 
+MEX = PKG / "mex"
+GATEWAYS = ("ParallelRebinningCBCurve", "reWeigAdFiltr_GPU", "Parallel_FDK_Helical_3DWeighting")
+
+
+def build_mex(verbose: bool = False) -> list[Path]:
+    """Build the mx API shim (libmxshim.so) and the three MEX gateways, each a
+    shared library exporting mexFunction, linked against libarecon.so.  With a
+    MATLAB installation, compile the same gateway sources with `mex` instead."""
+    cxx = os.environ.get("CXX", "g++")
+    shim = MEX / "libmxshim.so"
+    cmds = [[cxx, "-O2", "-fPIC", "-shared", "-std=c++17", f"-I{MEX / 'shim'}",
+             str(MEX / "shim" / "mx_shim.cpp"), "-o", str(shim)]]
+    if verbose:
+        print(" ".join(cmds[0]))
+    _run_parallel(cmds)
+    outs, cmds = [], []
+    for g in GATEWAYS:
+        out = MEX / f"{g}.so"
+        outs.append(out)
+        cmds.append([cxx, "-O2", "-fPIC", "-shared", "-std=c++17", f"-I{MEX / 'shim'}",
+                     f"-I{MEX}", f"-I{INCLUDE}", str(MEX / f"{g}.cpp"), "-o", str(out),
+                     f"-L{PKG}", "-larecon", f"-L{MEX}", "-lmxshim",
+                     "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."])
+    if verbose:
+        for c in cmds:
+            print(" ".join(c))
+    _run_parallel(cmds)
+    return outs
+
+
 def build_all(verbose: bool = False) -> None:
     build_oracle(verbose)
     build_library(verbose)
+    build_mex(verbose)
 
 
 if __name__ == "__main__":

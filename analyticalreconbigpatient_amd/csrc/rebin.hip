@@ -200,8 +200,8 @@ __global__ __launch_bounds__(256) void rebin_vzy(
     extern __shared__ float tile[];             // ZL * (JB + 1)
     __shared__ Blend sb[JB];
     __shared__ int sbase[JB][4];
-    const int i = blockIdx.y;
-    const int j0 = blockIdx.x * JB;
+    const int i = blockIdx.x;
+    const int j0 = blockIdx.y * JB;
     const int nj = min(JB, YL - j0);
     const int tid = threadIdx.x;
     if (tid < nj) {
@@ -266,7 +266,7 @@ void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN, float 
         constexpr int JB = 64;
         const size_t lds = sizeof(float) * (size_t)ZL * (JB + 1);
         ARECON_REQUIRE(lds <= 64 * 1024, "rebin: ZL too large for the transposing kernel");
-        dim3 grid((YL + JB - 1) / JB, ViewN);
+        dim3 grid(ViewN, (YL + JB - 1) / JB);
         hipLaunchKernelGGL(rebin_vzy<JB>, grid, dim3(256), lds, s, out, in,
                            chan, YL, ZL, ViewN, DeltaTheta, DeltaFai);
     }

@@ -1,0 +1,240 @@
+"""Benchmark: GVoxel-updates/s of the end-to-end helical FDK reconstruction
+(rebin -> preweight + filter -> 3D-weighted backprojection) at 512 x 512 x 400
+from an 888 x 64 x 11,808-view helical scan, on one MI355X.
+
+One step = one pass of the three stages over the device-resident sinogram.
+value = voxel-updates of one pass (512*512*400 voxels x 671-view windows =
+7.036e10) / measured seconds per pass.  Prints one JSON line.
+
+Also reported: per-stage times (HIP events on the launch stream), the FP16
+filtered-storage variant, the dominant kernel's roofline, the HU RMSE against
+the FP64 oracle on sampled slices, and the CPU baseline (the oracle, timed on
+this host on a bounded sample and scaled to the whole workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "GVoxel-updates/s end-to-end at 512²×400, chosen vs plain; HU RMSE vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and HU RMSE")
+    ap.add_argument("--cpu-slices", type=int, default=2, help="slices in the CPU sample")
+    ap.add_argument("--nz", type=int, default=0)
+    ap.add_argument("--order", type=int, default=0)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_threads() -> int:
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, min(n or (os.cpu_count() or 1), 16))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from analyticalreconbigpatient_amd import geometry as G
+    from analyticalreconbigpatient_amd import phantom
+    from analyticalreconbigpatient_amd.pipeline import HelicalFDK
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    wl = G.body_clinical()
+    s = wl.setup()
+    p0 = G.bp_params(G.bp_scan_geom(s))
+    z_mid = ((wl.zn - 1) * 0.5 - p0.ZNC) * p0.dz
+    log(f"workload {wl.name}; projecting phantom")
+    E = phantom.random_body(seed=2, fov_radius=250.0, z_centre=z_mid, z_half=250.0)
+    raw = phantom.project(E, s, wl.hdr, device=str(dev))
+    torch.cuda.synchronize()
+
+    results = {}
+    for storage in ("f32", "f16"):
+        fdk = HelicalFDK(s, storage=storage, nz=args.nz, order=args.order)
+        for _ in range(args.warmup):
+            fdk.run(raw)
+        torch.cuda.synchronize()
+        log(f"{storage}: warmup done, timing {args.steps} steps")
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            e = ev[k]
+            e[0].record()
+            fdk.rebin(raw)
+            e[1].record()
+            fdk.filter()
+            e[2].record()
+            fdk.backproject()
+            e[3].record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = np.array([[e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]),
+                        e[2].elapsed_time(e[3])] for e in ev])
+        results[storage] = dict(fdk=fdk, wall=wall, stages=st.mean(axis=0),
+                                ms=wall * 1e3 / args.steps)
+        log(f"{storage}: {results[storage]['ms']:.2f} ms/step, stages (ms) "
+            f"{np.round(st.mean(axis=0), 3).tolist()}")
+
+    plain = results["f32"]
+    fdk = plain["fdk"]
+    updates = fdk.updates
+    value = updates / (plain["ms"] * 1e-3) / 1e9
+    chosen = results["f16"]
+    chosen_value = updates / (chosen["ms"] * 1e-3) / 1e9
+
+    # Roofline of the dominant kernel (backprojection): algorithmic bytes per
+    # launch = 16 B of gathered FP32 samples per voxel-update + 4 B per voxel.
+    bp_ms = float(plain["stages"][2])
+    p = fdk.params
+    nvox = p.XN * p.YN * p.ZN
+    alg_bytes = 16.0 * updates + 4.0 * nvox
+    achieved = alg_bytes / (bp_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("bp_fdk3dw_f32_clinical_bytes")
+        except (ValueError, OSError):
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "bp_fdk3dw<4,float>", "kernel_ms": round(bp_ms, 3),
+                "bytes_per_update": 16, "note": "gathers are served from L2/MALL; "
+                "compulsory HBM bytes are the filtered sinogram once and the volume once"}
+
+    stage_roof = {
+        "rebin_GBs": round(8.0 * p.YL * p.ZL * fdk.ViewN / (plain["stages"][0] * 1e-3) / 1e9, 1),
+        "filter_GBs": round(8.0 * p.YL * p.ZL * fdk.ViewN / (plain["stages"][1] * 1e-3) / 1e9, 1),
+    }
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GVoxel-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(plain["ms"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: random-ellipsoid body phantom (seed 2), exact line integrals",
+        "config": {"workload": wl.name, "detector": f"{p.YL}x{p.ZL}", "views": fdk.ViewN,
+                   "volume": f"{p.XN}x{p.YN}x{p.ZN}", "voxel_updates": updates,
+                   "storage": "f32", "path": "plain"},
+        "stages_ms": {"rebin": round(float(plain["stages"][0]), 3),
+                      "filter": round(float(plain["stages"][1]), 3),
+                      "backproject": round(bp_ms, 3)},
+        "stage_hbm_GBs": stage_roof,
+        "chosen": {"storage": "f16", "value": round(chosen_value, 2),
+                   "ms_per_step": round(chosen["ms"], 3),
+                   "stages_ms": [round(float(x), 3) for x in chosen["stages"]],
+                   "speedup_vs_plain": round(chosen_value / value, 3)},
+        "roofline": roofline,
+    }
+
+    if not args.no_cpu:
+        out["cpu_baseline"], out["hu_rmse"] = cpu_leg(s, fdk, results, raw, args)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_leg(s, fdk, results, raw, args):
+    """Time the oracle on this host on a bounded sample and compare sampled
+    slices of the GPU volumes with the FP64 oracle chain."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from analyticalreconbigpatient_amd import geometry as G
+
+    nth = host_threads()
+    p = fdk.params
+    W = G.window_half_width(p.N_2pi, p.HSCoef)
+    zs = np.linspace(p.ZN * 0.45, p.ZN * 0.55, args.cpu_slices).astype(int)
+    # views needed: windows of the sampled slices plus the rebinning's fan margin
+    s0 = [math.ceil((2 * 3.14159265358979 * (z - p.ZNC) * p.dz / p.h - p.BetaS) / p.DeltaFai - 0.5)
+          for z in zs]
+    margin = int(math.ceil(math.asin(min(1.0, 1.5 * math.tan(s.DecAngle / 2))) / p.DeltaFai)) + 4
+    v_lo = max(0, min(s0) - W - margin)
+    v_hi = min(fdk.ViewN, max(s0) + W + margin + 1)
+    a = G.rebin_args(s.geom)
+    raw_np = raw[v_lo:v_hi].cpu().numpy()
+    log(f"cpu leg: views [{v_lo}, {v_hi}), slices {zs.tolist()}, {nth} threads")
+
+    t = time.perf_counter()
+    reb = oracle.rebin_f32(raw_np, a.YLC0, a.dYA, a.DeltaTheta, a.PLC0, a.DeltaT, a.DeltaFai,
+                           a.SO)
+    t_reb = time.perf_counter() - t
+    fa = G.filter_args(s)
+    t = time.perf_counter()
+    filt = oracle.filter_f64(np.ascontiguousarray(reb.transpose(0, 2, 1)), float(fa.PLC0),
+                             float(fa.ZLC0), float(fa.dYL), float(fa.dZL), float(fa.SO),
+                             nthreads=nth)
+    t_filt = time.perf_counter() - t
+    # backproject the sampled slices from the view sub-range: shift BetaS and
+    # the slice window by v_lo (views are independent).
+    sub = G.BPParams(**{**p.__dict__, "PN": v_hi - v_lo,
+                        "BetaS": p.BetaS + v_lo * p.DeltaFai})
+    w = oracle.fdk_weights(p.N_2pi, p.HSCoef, p.delta)
+    ref = np.zeros((p.YN, p.XN, p.ZN))
+    t = time.perf_counter()
+    for z in zs:
+        oracle.fdk_core(filt, w, N_2pi=sub.N_2pi, XN=sub.XN, YN=sub.YN, ZN=sub.ZN, XNC=sub.XNC,
+                        YNC=sub.YNC, ZNC=sub.ZNC, dx=sub.dx, dy=sub.dy, dz=sub.dz, YLC=sub.YLC,
+                        ZLC=sub.ZLC, dYL=sub.dYL, dZL=sub.dZL, h=sub.h, BetaS=sub.BetaS,
+                        DeltaFai=sub.DeltaFai, N_pi=sub.N_pi, HSCoef=sub.HSCoef, SO=sub.SO,
+                        SD=sub.SD, k1=sub.k1, zi_range=(int(z), int(z) + 1), nthreads=nth,
+                        out=ref)
+    t_bp = time.perf_counter() - t
+    nv = v_hi - v_lo
+    upd_sample = len(zs) * p.XN * p.YN * (2 * W + 1)
+    t_total = (t_reb / nv * fdk.ViewN + t_filt / nv * fdk.ViewN
+               + t_bp / upd_sample * fdk.updates)
+    cpu = {"value": round(fdk.updates / t_total / 1e9, 5), "unit": "GVoxel-updates/s",
+           "cores": nth, "kind": "port",
+           "sample": (f"oracle (rebin FP32, filter FP64, backprojection FP64) on views "
+                      f"[{v_lo},{v_hi}) and {len(zs)} slices of 512x512; measured "
+                      f"{t_reb:.2f}+{t_filt:.2f}+{t_bp:.2f} s, scaled per view / per "
+                      f"voxel-update to the whole workload ({t_total:.0f} s)")}
+    # HU RMSE (1000/mu_w) RMSE(mu) inside the FOV cylinder on the sampled slices
+    yy, xx = np.meshgrid((np.arange(p.YN) - p.YNC) * p.dy, (np.arange(p.XN) - p.XNC) * p.dx,
+                         indexing="ij")
+    mask = (xx ** 2 + yy ** 2) <= (0.98 * 250.0) ** 2
+    hu = {}
+    for name, res in results.items():
+        vol = res["fdk"].run(raw)
+        torch.cuda.synchronize()
+        g = vol[:, :, torch.as_tensor(zs, device=vol.device)].cpu().numpy().astype(np.float64)
+        r = ref[:, :, zs]
+        diff = (g - r)[mask]
+        hu[name] = round(float(1000.0 / 0.02 * np.sqrt(np.mean(diff ** 2))), 4)
+        hu[name + "_rel_l2"] = float(np.linalg.norm(g - r) / np.linalg.norm(r))
+    log(f"cpu leg done: {cpu['value']} GVU/s, HU RMSE {hu}")
+    return cpu, hu
+
+
+if __name__ == "__main__":
+    main()
