@@ -39,6 +39,7 @@ class BPGeom(C.Structure):
         ("SO", _d), ("SD", _d), ("k1", _d),
         ("nz", _i),
         ("order", _i),
+        ("variant", _i),
     ]
 
 
@@ -62,6 +63,7 @@ SIGNATURES = {
     "arecon_filter_dev": (_i, [_p, _i, _p, _i, _i, _i, _f, _f, _f, _f, _f, _p]),
     "arecon_bp_plan_create": (_i, [C.POINTER(BPGeom), _dp, C.POINTER(_p)]),
     "arecon_bp_plan_destroy": (_i, [_p]),
+    "arecon_bp_plan_nz": (_i, [_p]),
     "arecon_bp_plan_execute": (_i, [_p, _p, _p, _i, _p]),
     "arecon_bp_dev_f64": (_i, [_p, _p, C.POINTER(BPGeom), _dp, _p]),
     "arecon_fdk_weights": (_i, [_dp, _i, _d, _i]),

@@ -312,6 +312,10 @@ int arecon_bp_plan_destroy(arecon_bp_plan* plan) {
     return guarded([&] { bp_plan_destroy(reinterpret_cast<BPPlan*>(plan)); });
 }
 
+int arecon_bp_plan_nz(const arecon_bp_plan* plan) {
+    return bp_plan_nz(reinterpret_cast<const BPPlan*>(plan));
+}
+
 int arecon_bp_plan_execute(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
                            int fp_dtype, void* stream) {
     return guarded([&] {

@@ -88,6 +88,7 @@ void launch_filter(void* out, int out_dtype, const float* in, int YL, int ZL,
 struct BPPlan;
 BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w);
 void bp_plan_destroy(BPPlan* p);
+int bp_plan_nz(const BPPlan* p);
 void bp_plan_launch(const BPPlan* p, float* img, const void* fp, int fp_dtype,
                     hipStream_t s);
 void bp_launch_f64(double* img, const double* fp, const arecon_bp_geom& g,

@@ -63,6 +63,7 @@ struct BPArgs {
     float YLC, ZLC, invdYL, KSD;      // KSD = SO SD / dZL
     float YLm1, ZLm1;                 // YL-1, ZL-1 as floats
     float SO2, hstep, k1, DeltaFai;   // hstep = h DeltaFai / 2pi
+    long long view_stride;            // ZL * YL samples
 };
 
 // Two adjacent detector samples (channels U-1, U of one row) as floats.
@@ -189,6 +190,119 @@ __global__ __launch_bounds__(256) void bp_fdk3dw(float* __restrict__ img,
     }
 }
 
+// Slice-table kernel.  temp2 = z - h V / 2pi depends only on (slice, view), so
+// the row offset temp2 and the slice-dependent part of the 3D weight,
+//   Q = log2(w2/w1) + k1 log2(|temp2| / |temp2 -+ 0.5|)      (-inf when w2 = 0),
+// come from a per-(slice, offset) table built in FP64 and read with scalar
+// loads.  With the column term LR = k1 log2((t1-UU)/(t1+UU)),
+//   weight = w1 Gc^k1 / (w2 G^k1 + w1 Gc^k1) = 1 / (1 + exp2(Q + LR)),
+// i.e. one exp2 and one reciprocal per voxel-update inside the conjugate
+// region and none in the window centre.
+template <int NZ, class T>
+__global__ __launch_bounds__(256) void bp_fdk3dw_st(float* __restrict__ img,
+                                                    const T* __restrict__ fp,
+                                                    const float2* __restrict__ vtab,
+                                                    const float2* __restrict__ stab,
+                                                    const BPZ* __restrict__ ztab, BPArgs A) {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int qx = nb >> 3, rx = nb & 7, xcd = b & 7;
+    const int lb = xcd * qx + min(xcd, rx) + (b >> 3);
+    int zs, tile;
+    if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
+    else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
+    const int tyb = tile / A.tilesx, txb = tile - tyb * A.tilesx;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int xi = txb * 16 + (wave & 1) * 8 + (lane & 7);
+    const int yi = tyb * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const float x = ((float)xi - A.XNC) * A.dx;
+    const float y = ((float)yi - A.YNC) * A.dy;
+    const int z0 = zs * NZ;
+
+    // Slice j's table row, indexed by k = p - s0 in [-W, W]; entries are read as
+    // integer bit patterns so the window and -inf tests stay on the scalar unit.
+    constexpr int kNegInf = (int)0xff800000;
+    int s0[NZ], s1[NZ], s2[NZ];
+    const int2* srow[NZ];
+    float acc[NZ];
+    int pmin = INT_MAX, pmax = INT_MIN;
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+        acc[j] = 0.f;
+        int a = 1, e = 0, c = 0, zrow = 0;
+        if (z0 + j < A.ZN) {
+            const BPZ t = ztab[z0 + j];
+            a = t.s1; e = t.s2; c = t.s0; zrow = z0 + j;
+        }
+        s0[j] = c; s1[j] = a; s2[j] = e;
+        srow[j] = reinterpret_cast<const int2*>(stab) + (long long)zrow * (2 * A.W + 1) + A.W;
+        if (a <= e) { pmin = min(pmin, a); pmax = max(pmax, e); }
+    }
+    const unsigned YLu = (unsigned)A.YL;
+    for (int p = pmin; p <= pmax; ++p) {
+        int2 ent[NZ];
+#pragma unroll
+        for (int j = 0; j < NZ; ++j)   // always in-bounds: k clamped to the row
+            ent[j] = srow[j][min(max(p - s0[j], -A.W), A.W)];
+        bool need_lr = false;
+#pragma unroll
+        for (int j = 0; j < NZ; ++j)
+            need_lr |= (p >= s1[j]) && (p <= s2[j]) && (ent[j].y != kNegInf);
+        const float2 cs = vtab[p];
+        const T* vb = fp + (long long)p * A.view_stride;     // wavefront-uniform base
+        const float UU = -x * cs.x - y * cs.y;
+        const float Yr = -x * cs.y + y * cs.x;
+        const float t1sq = A.SO2 - Yr * Yr;
+        const float temp1 = fast_sqrt(t1sq);
+        const float U1 = Yr * A.invdYL + A.YLC;
+        const bool uok = (U1 > 0.f) && (U1 <= A.YLm1);     // 0 < ceil(U1) < YL
+        const float Uc = ceilf(U1);
+        const float Dey = Uc - U1;
+        const unsigned ucol = uok ? (unsigned)Uc - 1u : 0u;
+        const float K = A.KSD * fast_rcp(fmaf(UU, temp1, t1sq));
+        float LR = 0.f;
+        if (need_lr && !A.k1zero)
+            LR = A.k1 * fast_log2(fabsf((temp1 - UU) * fast_rcp(temp1 + UU)));
+#pragma unroll
+        for (int j = 0; j < NZ; ++j) {
+            if (p >= s1[j] && p <= s2[j]) {
+                const float temp2 = __int_as_float(ent[j].x);
+                const float V1 = fmaf(temp2, K, A.ZLC);
+                if (uok && V1 > 0.f && V1 <= A.ZLm1) {        // 0 < ceil(V1) < ZL
+                    const float Vc = ceilf(V1);
+                    const float Dez = Vc - V1;
+                    const unsigned o = ((unsigned)Vc - 1u) * YLu + ucol;
+                    const float2 r0 = load2<T>(vb + o);          // row V-1: U-1, U
+                    const float2 r1 = load2<T>(vb + (o + YLu));  // row V:   U-1, U
+                    const float top = fmaf(Dey, r0.x - r0.y, r0.y);
+                    const float bot = fmaf(Dey, r1.x - r1.y, r1.y);
+                    const float tou = fmaf(Dez, top - bot, bot);
+                    if (ent[j].y == kNegInf) {
+                        acc[j] += tou;
+                    } else {
+                        const float wgt =
+                            fast_rcp(1.f + fast_exp2(__int_as_float(ent[j].y) + LR));
+                        acc[j] = fmaf(wgt, tou, acc[j]);
+                    }
+                }
+            }
+        }
+    }
+    if (xi < A.XN && yi < A.YN) {
+        float* o = img + ((long long)yi * A.XN + xi) * A.ZN + z0;
+        if (NZ % 4 == 0 && A.vec4 && z0 + NZ <= A.ZN) {
+#pragma unroll
+            for (int j = 0; j < NZ; j += 4)
+                *reinterpret_cast<float4*>(o + j) =
+                    make_float4(acc[j] * A.DeltaFai, acc[(j + 1) % NZ] * A.DeltaFai,
+                                acc[(j + 2) % NZ] * A.DeltaFai, acc[(j + 3) % NZ] * A.DeltaFai);
+        } else {
+#pragma unroll
+            for (int j = 0; j < NZ; ++j)
+                if (z0 + j < A.ZN) o[j] = acc[j] * A.DeltaFai;
+        }
+    }
+}
+
 // FP64 path: the same math as the reference, one thread per voxel, in double.
 __global__ __launch_bounds__(256) void bp_fdk3dw_f64(double* __restrict__ img,
                                                      const double* __restrict__ fp,
@@ -277,15 +391,15 @@ void validate_geom(const arecon_bp_geom& g) {
 }
 
 int auto_nz(const std::vector<BPZ>& z, int W) {
-    // Largest NZ whose union of NZ consecutive windows stays within ~15% of one
+    // Largest NZ whose union of NZ consecutive windows stays within ~20% of one
     // window (slices further apart share too few views to amortise the column
     // work).
     const int n = (int)z.size();
     double ds = 0.0;
     if (n > 1) ds = std::fabs((double)(z[n - 1].s0 - z[0].s0)) / (n - 1);
     const double L = 2.0 * W + 1.0;
-    for (int nz : {8, 4, 2})
-        if ((nz - 1) * ds <= 0.15 * L && n >= nz) return nz;
+    for (int nz : {16, 8, 4, 2})
+        if ((nz - 1) * ds <= 0.2 * L && n >= nz) return nz;
     return 1;
 }
 
@@ -296,8 +410,10 @@ struct BPPlan {
     int device = 0;
     float2* vtab = nullptr;
     float4* ktab = nullptr;
+    float2* stab = nullptr;     // per (slice, offset): temp2, Q
     BPZ* ztab = nullptr;
     BPArgs a{};
+    int variant = 0;
     int nz = 1;
     int grid = 0;
 };
@@ -338,12 +454,38 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         e.dz = (float)(z - g.h * (g.BetaS + s0 * g.DeltaFai) * kInvTwoPi);
         zt[zi] = e;
     }
+    // Per-(slice, offset) table of the slice-table kernel: temp2 = z - h V / 2pi
+    // (:79) and Q = log2(w2/w1) + k1 log2(|temp2| / |temp2 -+ 0.5|) (:95-102),
+    // -inf where w2 = 0 (weight exactly 1), all in FP64.
+    const int L = 2 * W + 1;
+    std::vector<float2> st((size_t)g.ZN * L);
+    for (int zi = 0; zi < g.ZN; ++zi) {
+        const double z = (zi - g.ZNC) * g.dz;
+        const int s0 = zt[zi].s0;
+        for (int k = -W; k <= W; ++k) {
+            const double V = g.BetaS + (s0 + k) * g.DeltaFai;
+            const double temp2 = z - g.h * V * kInvTwoPi;
+            const double w1 = w[weight_index1(g.N_pi, k)], w2 = w[weight_index2(g.N_pi, k)];
+            double Q;
+            if (w2 == 0.0) {
+                Q = -INFINITY;
+            } else {
+                Q = std::log2(w2) - std::log2(w1);
+                if (g.k1 != 0.0)
+                    Q += g.k1 * (std::log2(std::fabs(temp2)) -
+                                 std::log2(std::fabs(temp2 + (k < 0 ? -0.5 : 0.5))));
+            }
+            st[(size_t)zi * L + (k + W)] = make_float2((float)temp2, (float)Q);
+        }
+    }
     BPPlan* P = new BPPlan();
     P->g = g;
+    P->variant = g.variant;
+    ARECON_REQUIRE(g.variant == 0 || g.variant == 1, "bp: variant must be 0 or 1");
     ARECON_HIP(hipGetDevice(&P->device));
     P->nz = g.nz > 0 ? g.nz : auto_nz(zt, W);
-    ARECON_REQUIRE(P->nz == 1 || P->nz == 2 || P->nz == 4 || P->nz == 8,
-                   "bp: nz must be 0, 1, 2, 4 or 8");
+    ARECON_REQUIRE(P->nz == 1 || P->nz == 2 || P->nz == 4 || P->nz == 8 || P->nz == 16,
+                   "bp: nz must be 0, 1, 2, 4, 8 or 16");
     ARECON_REQUIRE(g.order == 0 || g.order == 1, "bp: order must be 0 or 1");
     BPArgs& A = P->a;
     A.XN = g.XN; A.YN = g.YN; A.ZN = g.ZN; A.YL = g.YL; A.ZL = g.ZL; A.PN = g.PN; A.W = W;
@@ -359,6 +501,7 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
     A.YLm1 = (float)(g.YL - 1); A.ZLm1 = (float)(g.ZL - 1);
     A.SO2 = (float)(g.SO * g.SO); A.hstep = (float)(g.h * g.DeltaFai * kInvTwoPi);
     A.k1 = (float)g.k1; A.DeltaFai = (float)g.DeltaFai;
+    A.view_stride = (long long)g.ZL * g.YL;
     const long long grid = (long long)A.ntiles * A.nzsets;
     ARECON_REQUIRE(grid < (1LL << 31), "bp: volume too large for one launch");
     P->grid = (int)grid;
@@ -366,6 +509,8 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         ARECON_HIP(hipMalloc(&P->vtab, sizeof(float2) * vt.size()));
         ARECON_HIP(hipMalloc(&P->ktab, sizeof(float4) * kt.size()));
         ARECON_HIP(hipMalloc(&P->ztab, sizeof(BPZ) * zt.size()));
+        ARECON_HIP(hipMalloc(&P->stab, sizeof(float2) * st.size()));
+        ARECON_HIP(hipMemcpy(P->stab, st.data(), sizeof(float2) * st.size(), hipMemcpyHostToDevice));
         ARECON_HIP(hipMemcpy(P->vtab, vt.data(), sizeof(float2) * vt.size(), hipMemcpyHostToDevice));
         ARECON_HIP(hipMemcpy(P->ktab, kt.data(), sizeof(float4) * kt.size(), hipMemcpyHostToDevice));
         ARECON_HIP(hipMemcpy(P->ztab, zt.data(), sizeof(BPZ) * zt.size(), hipMemcpyHostToDevice));
@@ -376,6 +521,8 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
     return P;
 }
 
+int bp_plan_nz(const BPPlan* P) { return P ? P->nz : 0; }
+
 void bp_plan_destroy(BPPlan* P) {
     if (!P) return;
     int cur = 0;
@@ -384,6 +531,7 @@ void bp_plan_destroy(BPPlan* P) {
     if (P->vtab) (void)hipFree(P->vtab);
     if (P->ktab) (void)hipFree(P->ktab);
     if (P->ztab) (void)hipFree(P->ztab);
+    if (P->stab) (void)hipFree(P->stab);
     (void)hipSetDevice(cur);
     delete P;
 }
@@ -392,8 +540,12 @@ template <int NZ, class T>
 static void launch_nz(const BPPlan* P, float* img, const T* fp, hipStream_t s) {
     BPArgs a = P->a;
     a.vec4 = (a.ZN % 4 == 0) && ((reinterpret_cast<uintptr_t>(img) & 15) == 0);
-    hipLaunchKernelGGL((bp_fdk3dw<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp, P->vtab,
-                       P->ktab, P->ztab, a);
+    if (P->variant == 1)
+        hipLaunchKernelGGL((bp_fdk3dw<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp, P->vtab,
+                           P->ktab, P->ztab, a);
+    else
+        hipLaunchKernelGGL((bp_fdk3dw_st<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp,
+                           P->vtab, P->stab, P->ztab, a);
 }
 
 template <class T>
@@ -402,7 +554,8 @@ static void launch_t(const BPPlan* P, float* img, const T* fp, hipStream_t s) {
     case 1: launch_nz<1, T>(P, img, fp, s); break;
     case 2: launch_nz<2, T>(P, img, fp, s); break;
     case 4: launch_nz<4, T>(P, img, fp, s); break;
-    default: launch_nz<8, T>(P, img, fp, s); break;
+    case 8: launch_nz<8, T>(P, img, fp, s); break;
+    default: launch_nz<16, T>(P, img, fp, s); break;
     }
 }
 

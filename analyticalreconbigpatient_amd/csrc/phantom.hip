@@ -25,11 +25,10 @@ namespace arecon {
 namespace {
 
 struct Ellipsoid {
-    float cx, cy, cz;       // centre, mm
-    float ia, ib, ic;       // reciprocal semi-axes, 1/mm
-    float cphi, sphi;       // rotation about z
-    float mu;               // attenuation added inside, 1/mm
-    float pad[3];
+    double cx, cy, cz;      // centre, mm
+    double ia, ib, ic;      // reciprocal semi-axes, 1/mm
+    double cphi, sphi;      // rotation about z
+    double mu;              // attenuation added inside, 1/mm
 };
 
 __global__ __launch_bounds__(256) void project_ellipsoids(
@@ -50,26 +49,29 @@ __global__ __launch_bounds__(256) void project_ellipsoids(
     const double sx = SO * cos(beta), sy = SO * sin(beta), sz = h * beta / (2.0 * M_PI);
     const double dxr = -cos(theta), dyr = -sin(theta);
     const double slope = (k - ZLC0) * dZL / SD;       // dz per unit in-plane length
-    const float dlen = (float)sqrt(1.0 + slope * slope);
-    float acc = 0.f;
+    const double dlen = sqrt(1.0 + slope * slope);
+    // FP64 throughout: rays start ~SO from the ellipsoid centres, and the
+    // quadratic's discriminant cancels badly in FP32.
+    double acc = 0.0;
     for (int e = 0; e < nE; ++e) {
         const Ellipsoid q = E[e];
         // Ray relative to the ellipsoid centre, rotated into its frame, scaled.
-        const float px = (float)(sx - q.cx), py = (float)(sy - q.cy), pz = (float)(sz - q.cz);
-        const float ux = (float)dxr, uy = (float)dyr, uz = (float)slope;
-        const float ax = (q.cphi * px + q.sphi * py) * q.ia;
-        const float ay = (-q.sphi * px + q.cphi * py) * q.ib;
-        const float az = pz * q.ic;
-        const float bx = (q.cphi * ux + q.sphi * uy) * q.ia;
-        const float by = (-q.sphi * ux + q.cphi * uy) * q.ib;
-        const float bz = uz * q.ic;
-        const float A = bx * bx + by * by + bz * bz;
-        const float B = ax * bx + ay * by + az * bz;
-        const float Cc = ax * ax + ay * ay + az * az - 1.f;
-        const float disc = B * B - A * Cc;
-        if (disc > 0.f) acc += q.mu * 2.f * sqrtf(disc) / A * dlen;
+        const double px = sx - q.cx, py = sy - q.cy, pz = sz - q.cz;
+        const double cp = q.cphi, sp = q.sphi;
+        const double ax = (cp * px + sp * py) * q.ia;
+        const double ay = (-sp * px + cp * py) * q.ib;
+        const double az = pz * q.ic;
+        const double bx = (cp * dxr + sp * dyr) * q.ia;
+        const double by = (-sp * dxr + cp * dyr) * q.ib;
+        const double bz = slope * q.ic;
+        // closest approach in the scaled frame: m2 = |a + l0 b|^2, l0 = -(a.b)/(b.b)
+        const double A = bx * bx + by * by + bz * bz;
+        const double l0 = -(ax * bx + ay * by + az * bz) / A;
+        const double mx = ax + l0 * bx, my = ay + l0 * by, mz = az + l0 * bz;
+        const double m2 = mx * mx + my * my + mz * mz;
+        if (m2 < 1.0) acc += q.mu * 2.0 * sqrt((1.0 - m2) / A) * dlen;
     }
-    out[id] = acc;
+    out[id] = (float)acc;
 }
 
 }  // namespace
@@ -91,8 +93,8 @@ extern "C" int arecon_project_ellipsoids_dev(float* d_out, const float* ellipsoi
         for (int e = 0; e < nE; ++e) {
             const float* r = ellipsoids + 8 * e;
             ARECON_REQUIRE(r[3] > 0.f && r[4] > 0.f && r[5] > 0.f, "project: semi-axes must be positive");
-            E[e] = Ellipsoid{r[0], r[1], r[2], 1.f / r[3], 1.f / r[4], 1.f / r[5],
-                             std::cos(r[6]), std::sin(r[6]), r[7], {0.f, 0.f, 0.f}};
+            E[e] = Ellipsoid{r[0], r[1], r[2], 1.0 / r[3], 1.0 / r[4], 1.0 / r[5],
+                             std::cos((double)r[6]), std::sin((double)r[6]), r[7]};
         }
         const long long n = (long long)ViewN * YL * ZL;
         const long long grid = (n + 255) / 256;

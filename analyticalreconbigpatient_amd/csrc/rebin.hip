@@ -11,8 +11,8 @@
 // Design.  Everything that depends on the channel only (Beta, the channel pair
 // and its weights) is computed once per call on the host, in the reference's
 // own arithmetic (double libm rounded to float), and read from a small table.
-// The kernels do the view-dependent part with round-to-nearest intrinsics and no
-// FMA contraction, so their result is bit-identical to the float CPU path.
+// The kernels do the view-dependent part in the same order with FMA contraction
+// switched off, so their result is bit-identical to the float CPU path.
 //
 //   rebin_vyz  output [ViewN][YL][ZL] (reference layout): one thread per
 //              4 consecutive rows of one (view, channel), float4 loads/stores.
@@ -110,8 +110,8 @@ const ChanCoef* channel_table_dev(int YL, float YLC, float dYA, float PLC, float
     return d;
 }
 
-// View-dependent part of the blend weights (:34,:39,:41,:43,:45-46,:55-69),
-// with explicit round-to-nearest operations so no FMA is formed.
+// View-dependent part of the blend weights (:34,:39,:41,:43,:45-46,:55-69).
+// FMA contraction is off here so every operation rounds as the CPU's does.
 struct Blend {
     float c00, c10, c01, c11;   // (XB,YB) (XU,YB) (XB,YU) (XU,YU)
     int ixb, ixu;
@@ -119,30 +119,30 @@ struct Blend {
 
 __device__ __forceinline__ Blend view_blend(int i, const ChanCoef& cc, int ViewN,
                                             float DeltaTheta, float DeltaFai) {
-    const float theta = __fmul_rn((float)i, DeltaTheta);
-    const float fai = __fadd_rn(theta, cc.beta);
-    const float fidx = __fdiv_rn(fai, DeltaFai);
+#pragma clang fp contract(off)
+    const float theta = (float)i * DeltaTheta;
+    const float fai = theta + cc.beta;
+    const float fidx = fai / DeltaFai;          // IEEE division (correctly rounded)
     const float fic = ceilf(fidx);
     const int FI = (int)fic;
-    const float cxb = __fsub_rn(fic, fidx);
-    const float cxu = __fsub_rn(1.0f, cxb);
+    const float cxb = fic - fidx;
+    const float cxu = 1.0f - cxb;
     Blend b;
     if (FI <= 0) { b.ixb = 0; b.ixu = 0; }
     else if (FI > ViewN - 1) { b.ixb = ViewN - 1; b.ixu = ViewN - 1; }
     else { b.ixb = FI - 1; b.ixu = FI; }
-    b.c00 = __fmul_rn(cxb, cc.cyb);
-    b.c10 = __fmul_rn(cxu, cc.cyb);
-    b.c01 = __fmul_rn(cxb, cc.cyu);
-    b.c11 = __fmul_rn(cxu, cc.cyu);
+    b.c00 = cxb * cc.cyb;
+    b.c10 = cxu * cc.cyb;
+    b.c01 = cxb * cc.cyu;
+    b.c11 = cxu * cc.cyu;
     return b;
 }
 
 // ((c00*p0 + c10*p1) + c01*p2) + c11*p3, each operation rounded (:89-93).
 __device__ __forceinline__ float blend4(const Blend& b, float p0, float p1, float p2,
                                         float p3) {
-    float acc = __fadd_rn(__fmul_rn(b.c00, p0), __fmul_rn(b.c10, p1));
-    acc = __fadd_rn(acc, __fmul_rn(b.c01, p2));
-    return __fadd_rn(acc, __fmul_rn(b.c11, p3));
+#pragma clang fp contract(off)
+    return b.c00 * p0 + b.c10 * p1 + b.c01 * p2 + b.c11 * p3;
 }
 
 // [ViewN][YL][ZL] output, ZL % 4 == 0: one thread per float4 of rows.

@@ -43,15 +43,16 @@ class BPPlan:
     """Owner of an ``arecon_bp_plan`` (tables built in FP64, stored FP32)."""
 
     def __init__(self, params: BPParams, *, dz: float | None = None, nz: int = 0,
-                 order: int = 0, w: np.ndarray | None = None):
+                 order: int = 0, variant: int = 0, w: np.ndarray | None = None):
         self.params = params
-        self.geom = params.to_geom(dz=dz, nz=nz, order=order)
+        self.geom = params.to_geom(dz=dz, nz=nz, order=order, variant=variant)
         self.w = fdk_weights(params.N_2pi, params.HSCoef, params.delta) if w is None else \
             np.ascontiguousarray(w, dtype=np.float64)
         h = C.c_void_p()
         _lib.check(_lib.lib().arecon_bp_plan_create(C.byref(self.geom), _lib.dptr(self.w),
                                                     C.byref(h)), "arecon_bp_plan_create")
         self._h = h
+        self.nz = _lib.lib().arecon_bp_plan_nz(h)
 
     def execute(self, img: torch.Tensor, fp: torch.Tensor, stream=None) -> None:
         g = self.geom
@@ -122,7 +123,7 @@ class HelicalFDK:
 
     def __init__(self, setup: ReconSetup, *, storage: str = "f32", first_view: int = 0,
                  n_views: int | None = None, bp_geom=None, dz: float | None = None,
-                 nz: int = 0, order: int = 0, device: str = "cuda"):
+                 nz: int = 0, order: int = 0, variant: int = 0, device: str = "cuda"):
         self.setup = setup
         self.ra = rebin_args(setup.geom)
         self.ViewN, self.YL, self.ZL = self.ra.ViewN, self.ra.YL, self.ra.ZL
@@ -131,7 +132,7 @@ class HelicalFDK:
         g = bp_scan_geom(setup, geom=bp_geom, n_views=self.n_views)
         self.params = bp_params(g)
         self.dz = dz
-        self.plan = BPPlan(self.params, dz=dz, nz=nz, order=order)
+        self.plan = BPPlan(self.params, dz=dz, nz=nz, order=order, variant=variant)
         self.storage = storage
         code, tdtype = STORAGE[storage]
         dev = torch.device(device)
@@ -142,6 +143,7 @@ class HelicalFDK:
         p = self.params
         self.volume = torch.empty((p.YN, p.XN, p.ZN), dtype=torch.float32, device=dev)
         self.updates = voxel_updates(p, dz)
+        self.plan_nz = self.plan.nz
 
     def rebin(self, raw: torch.Tensor, stream=None) -> torch.Tensor:
         rebin_dev(self.rebinned, raw, self.setup.geom, _lib.ARECON_LAYOUT_VZY, stream)

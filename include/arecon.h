@@ -152,8 +152,9 @@ typedef struct arecon_bp_geom {
     double BetaS, DeltaFai;     /* first view angle, view step */
     double N_pi, HSCoef;        /* N_2pi / 2, half-scan coefficient */
     double SO, SD, k1;
-    int nz;                     /* slices per thread: 0 = automatic, else 1/2/4/8 */
+    int nz;                     /* slices per thread: 0 = automatic, else 1/2/4/8/16 */
     int order;                  /* workgroup order: 0 = slice-major, 1 = column-major */
+    int variant;                /* 0 = slice-table kernel (default), 1 = per-voxel weight */
 } arecon_bp_geom;
 
 typedef struct arecon_bp_plan arecon_bp_plan;
@@ -164,6 +165,9 @@ typedef struct arecon_bp_plan arecon_bp_plan;
 int arecon_bp_plan_create(const arecon_bp_geom* g, const double* w,
                           arecon_bp_plan** out);
 int arecon_bp_plan_destroy(arecon_bp_plan* plan);
+
+/* Slices per thread the plan chose (1, 2, 4, 8 or 16); 0 for a null plan. */
+int arecon_bp_plan_nz(const arecon_bp_plan* plan);
 
 /* Backprojects d_fp ([PN][ZL][YL], element type fp_dtype) into d_img
  * ([YN][XN][ZN], FP32; every voxel is written). */

@@ -126,21 +126,21 @@ class P_:
         self.__dict__.update(g)
         self.YL, self.ZL, self.PN, self.delta = YL, ZL, PN, delta
 
-    def to_geom(self, dz=None, nz=0, order=0):
+    def to_geom(self, dz=None, nz=0, order=0, variant=0):
         return _lib.BPGeom(N_2pi=self.N_2pi, XN=self.XN, YN=self.YN, ZN=self.ZN, YL=self.YL,
                            ZL=self.ZL, PN=self.PN, XNC=self.XNC, YNC=self.YNC, ZNC=self.ZNC,
                            dx=self.dx, dy=self.dy, dz=self.dz if dz is None else dz,
                            YLC=self.YLC, ZLC=self.ZLC, dYL=self.dYL, dZL=self.dZL, h=self.h,
                            BetaS=self.BetaS, DeltaFai=self.DeltaFai, N_pi=self.N_pi,
                            HSCoef=self.HSCoef, SO=self.SO, SD=self.SD, k1=self.k1, nz=nz,
-                           order=order)
+                           order=order, variant=variant)
 
 
-def gpu_bp(fp, g, nz=0, order=0, storage=torch.float32):
+def gpu_bp(fp, g, nz=0, order=0, storage=torch.float32, variant=0):
     PN, ZL, YL = fp.shape
     p = P_(g, YL, ZL, PN)
     w = oracle.fdk_weights(g["N_2pi"], g["HSCoef"], 8)
-    plan = BPPlan(p, nz=nz, order=order, w=w)
+    plan = BPPlan(p, nz=nz, order=order, w=w, variant=variant)
     dfp = torch.from_numpy(np.ascontiguousarray(fp, dtype=np.float32)).cuda().to(storage)
     img = torch.full((g["YN"], g["XN"], g["ZN"]), float("nan"), device="cuda")
     plan.execute(img, dfp)
@@ -162,15 +162,16 @@ def realistic(**kw):
     return g
 
 
-@pytest.mark.parametrize("nz", [1, 2, 4, 8])
+@pytest.mark.parametrize("nz", [1, 2, 4, 8, 16])
 @pytest.mark.parametrize("order", [0, 1])
-def test_bp_matches_fp64_oracle(gpu, nz, order):
-    print(f"bp nz={nz} order={order}", flush=True)
+@pytest.mark.parametrize("variant", [0, 1])
+def test_bp_matches_fp64_oracle(gpu, nz, order, variant):
+    print(f"bp nz={nz} order={order} variant={variant}", flush=True)
     g = realistic()
     PN, ZL, YL = 200, 16, 128
     fp = np.random.default_rng(nz + 10 * order).normal(size=(PN, ZL, YL))
     fp32 = fp.astype(np.float32)
-    got, w = gpu_bp(fp32, g, nz=nz, order=order)
+    got, w = gpu_bp(fp32, g, nz=nz, order=order, variant=variant)
     want = oracle.fdk_core(fp32.astype(np.float64), w,
                            **{k: v for k, v in g.items() if k not in ("YL", "ZL")})
     assert not np.isnan(got).any()
@@ -179,20 +180,22 @@ def test_bp_matches_fp64_oracle(gpu, nz, order):
 
 
 @pytest.mark.parametrize("k1", [0.0, 2.85])
-def test_bp_full_window_random(gpu, k1):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_bp_full_window_random(gpu, k1, variant):
     g = bp_geom_dict(k1=k1, XN=17, YN=21, ZN=9, XNC=8.2, YNC=10.1, dz=0.113, ZNC=-14.77)
     fp = np.random.default_rng(3).normal(size=(192, 16, 64)).astype(np.float32)
-    got, w = gpu_bp(fp, g)
+    got, w = gpu_bp(fp, g, variant=variant)
     want = oracle.fdk_core(fp.astype(np.float64), w,
                            **{k: v for k, v in g.items() if k not in ("YL", "ZL")})
     assert rel_l2(got, want) < 1e-4
 
 
 @pytest.mark.parametrize("k1", [0.0, 5.0])
-def test_bp_constant_projection_gives_pi_c(gpu, k1):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_bp_constant_projection_gives_pi_c(gpu, k1, variant):
     g = bp_geom_dict(h=1.0, k1=k1)
     c = 1.75
-    got, _ = gpu_bp(np.full((192, 16, 64), c, dtype=np.float32), g)
+    got, _ = gpu_bp(np.full((192, 16, 64), c, dtype=np.float32), g, variant=variant)
     np.testing.assert_allclose(got, math.pi * c, rtol=2e-5)
 
 
