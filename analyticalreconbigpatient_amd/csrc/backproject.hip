@@ -43,6 +43,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 namespace arecon {
@@ -64,6 +65,7 @@ struct BPArgs {
     float YLm1, ZLm1;                 // YL-1, ZL-1 as floats
     float SO2, hstep, k1, DeltaFai;   // hstep = h DeltaFai / 2pi
     long long view_stride;            // ZL * YL samples
+    unsigned view_bytes;              // ZL * YL * sizeof(sample)
 };
 
 // Two adjacent detector samples (channels U-1, U of one row) as floats.
@@ -81,6 +83,44 @@ template <> __device__ __forceinline__ float2 load2<__half>(const __half* p) {
 template <> __device__ __forceinline__ float2 load2<__hip_bfloat16>(const __hip_bfloat16* p) {
     const unsigned short* q = reinterpret_cast<const unsigned short*>(p);
     return make_float2(__uint_as_float((unsigned)q[0] << 16), __uint_as_float((unsigned)q[1] << 16));
+}
+
+// Buffer loads: one descriptor per view (wavefront-uniform, in SGPRs) and a
+// 32-bit byte offset per lane; the second detector row comes through the
+// instruction's scalar offset.  Out-of-range buffer reads return 0 instead of
+// faulting.
+typedef float f32x2_v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t view_rsrc(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+// Samples at byte offset `off` and the next element, as floats.
+template <class T> __device__ __forceinline__ float2 bload2(__amdgpu_buffer_rsrc_t r,
+                                                           unsigned off, unsigned soff);
+template <> __device__ __forceinline__ float2 bload2<float>(__amdgpu_buffer_rsrc_t r,
+                                                           unsigned off, unsigned soff) {
+    const f32x2_v v = __builtin_bit_cast(f32x2_v, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+    return make_float2(v.x, v.y);
+}
+// 16-bit samples: one 8-byte load from the 4-byte-aligned address below `off`
+// covers both elements whether `off` is 4-aligned or 2 past it.
+__device__ __forceinline__ unsigned pair16(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+    const unsigned long long v = __builtin_bit_cast(
+        unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, off & ~3u, soff, 0));
+    return (unsigned)(v >> ((off & 2u) * 8u));
+}
+template <> __device__ __forceinline__ float2 bload2<__half>(__amdgpu_buffer_rsrc_t r,
+                                                            unsigned off, unsigned soff) {
+    const unsigned b = pair16(r, off, soff);
+    return make_float2(__half2float(__ushort_as_half((unsigned short)(b & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(b >> 16))));
+}
+template <> __device__ __forceinline__ float2 bload2<__hip_bfloat16>(__amdgpu_buffer_rsrc_t r,
+                                                                    unsigned off, unsigned soff) {
+    const unsigned b = pair16(r, off, soff);
+    return make_float2(__uint_as_float(b << 16), __uint_as_float(b & 0xffff0000u));
 }
 
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -190,20 +230,76 @@ __global__ __launch_bounds__(256) void bp_fdk3dw(float* __restrict__ img,
     }
 }
 
-// Slice-table kernel.  temp2 = z - h V / 2pi depends only on (slice, view), so
-// the row offset temp2 and the slice-dependent part of the 3D weight,
-//   Q = log2(w2/w1) + k1 log2(|temp2| / |temp2 -+ 0.5|)      (-inf when w2 = 0),
-// come from a per-(slice, offset) table built in FP64 and read with scalar
-// loads.  With the column term LR = k1 log2((t1-UU)/(t1+UU)),
-//   weight = w1 Gc^k1 / (w2 G^k1 + w1 Gc^k1) = 1 / (1 + exp2(Q + LR)),
-// i.e. one exp2 and one reciprocal per voxel-update inside the conjugate
-// region and none in the window centre.
+// View-table kernel.  temp2 = z - h V / 2pi depends only on (slice, view), so the
+// row offset and the slice-dependent part of the 3D weight,
+//   a = 2^Q,  Q = log2(w2/w1) + k1 log2(|temp2| / |temp2 -+ 0.5|)   (a = 0 when w2 = 0),
+// come from a table built in FP64 on the host, laid out per (slice set, view) so
+// that one scalar load brings the NZ slices' (temp2, a) for a view.  A slice
+// whose window does not contain the view has temp2 = 1e30 there (its row is then
+// off the detector), so the branch-free update adds nothing and no window tests
+// run in the loop.  With the column term B = ((t1-UU)/(t1+UU))^k1,
+//   weight = w1 Gc^k1 / (w2 G^k1 + w1 Gc^k1) = 1 / (1 + a B),
+// and the views where every slice of the set has a = 0 (no conjugate view in
+// the window) run a second loop body without the weight at all.
+struct ZSet {
+    int pmin, pmax;     // union of the set's windows (pmin > pmax: empty)
+    int clo, chi;       // views with weight 1 for every slice of the set
+    long long off;      // first table entry (view pmin, slice 0)
+    long long pad;
+};
+
+template <int NZ, class T, bool WEIGHTED>
+__device__ __forceinline__ void bp_view(int p, float x, float y, const T* __restrict__ fp,
+                                        const float2* __restrict__ vtab,
+                                        const float2* __restrict__ trow, const BPArgs& A,
+                                        unsigned rowb, float (&acc)[NZ]) {
+    struct Ent { float2 e[NZ]; };
+    const Ent ent = *reinterpret_cast<const Ent*>(trow + (long long)p * NZ);   // scalar load
+    const float2 cs = vtab[p];
+    const __amdgpu_buffer_rsrc_t rs = view_rsrc(fp + (long long)p * A.view_stride, A.view_bytes);
+    const float UU = -x * cs.x - y * cs.y;
+    const float Yr = -x * cs.y + y * cs.x;
+    const float t1sq = A.SO2 - Yr * Yr;
+    const float temp1 = fast_sqrt(t1sq);
+    const float U1 = Yr * A.invdYL + A.YLC;
+    const bool uok = (U1 > 0.f) && (U1 <= A.YLm1);           // 0 < ceil(U1) < YL
+    const float Uc = ceilf(U1);
+    const float Dey = Uc - U1;
+    // byte offset of channel U-1 in row V-1 is (V-1)*rowb + (U-1)*sizeof(T); the
+    // -rowb folds the row's -1 into the per-slice multiply-add below
+    const unsigned ucolb = (uok ? (unsigned)Uc - 1u : 0u) * (unsigned)sizeof(T) - rowb;
+    const float inv = fast_rcp(fmaf(UU, temp1, t1sq));     // 1 / (t1 (t1 + UU))
+    const float K = A.KSD * inv;
+    float B = 1.f;
+    if (WEIGHTED)   // B = ((t1-UU)/(t1+UU))^k1, kept finite and nonzero
+        B = fminf(fmaxf(fast_exp2(A.k1 * fast_log2(fabsf((temp1 - UU) * temp1 * inv))), 1e-30f),
+                  1e30f);
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+        const float V1 = fmaf(ent.e[j].x, K, A.ZLC);     // huge for an inactive slice
+        const bool ok = uok && V1 > 0.f && V1 <= A.ZLm1;  // 0 < ceil(V1) < ZL
+        const float Vc = ceilf(V1);
+        const float Dez = Vc - V1;
+        // off-detector lanes read past the descriptor's range: 0, no access
+        const unsigned ob = ok ? __umul24((unsigned)Vc, rowb) + ucolb : 0xfffffff0u;
+        const float2 r0 = bload2<T>(rs, ob, 0u);          // row V-1: U-1, U
+        const float2 r1 = bload2<T>(rs, ob, rowb);        // row V:   U-1, U
+        const float top = fmaf(Dey, r0.x - r0.y, r0.y);
+        const float bot = fmaf(Dey, r1.x - r1.y, r1.y);
+        const float tou = ok ? fmaf(Dez, top - bot, bot) : 0.f;
+        if (WEIGHTED)   // w1 Gc^k1 / (w2 G^k1 + w1 Gc^k1) = 1 / (1 + 2^Q B)
+            acc[j] = fmaf(fast_rcp(fmaf(ent.e[j].y, B, 1.f)), tou, acc[j]);
+        else
+            acc[j] += tou;
+    }
+}
+
 template <int NZ, class T>
-__global__ __launch_bounds__(256) void bp_fdk3dw_st(float* __restrict__ img,
+__global__ __launch_bounds__(256) void bp_fdk3dw_vt(float* __restrict__ img,
                                                     const T* __restrict__ fp,
                                                     const float2* __restrict__ vtab,
-                                                    const float2* __restrict__ stab,
-                                                    const BPZ* __restrict__ ztab, BPArgs A) {
+                                                    const ZSet* __restrict__ zsets,
+                                                    const float2* __restrict__ table, BPArgs A) {
     const int nb = gridDim.x, b = blockIdx.x;
     const int qx = nb >> 3, rx = nb & 7, xcd = b & 7;
     const int lb = xcd * qx + min(xcd, rx) + (b >> 3);
@@ -217,75 +313,22 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_st(float* __restrict__ img,
     const float x = ((float)xi - A.XNC) * A.dx;
     const float y = ((float)yi - A.YNC) * A.dy;
     const int z0 = zs * NZ;
-
-    // Slice j's table row, indexed by k = p - s0 in [-W, W]; entries are read as
-    // integer bit patterns so the window and -inf tests stay on the scalar unit.
-    constexpr int kNegInf = (int)0xff800000;
-    int s0[NZ], s1[NZ], s2[NZ];
-    const int2* srow[NZ];
+    const ZSet h = zsets[zs];
+    const float2* trow = table + h.off - (long long)h.pmin * NZ;
+    const unsigned rowb = (unsigned)A.YL * (unsigned)sizeof(T);   // bytes per detector row
+    // (V, rowb) < 2^24 for every supported detector, so 24-bit multiplies suffice
     float acc[NZ];
-    int pmin = INT_MAX, pmax = INT_MIN;
 #pragma unroll
-    for (int j = 0; j < NZ; ++j) {
-        acc[j] = 0.f;
-        int a = 1, e = 0, c = 0, zrow = 0;
-        if (z0 + j < A.ZN) {
-            const BPZ t = ztab[z0 + j];
-            a = t.s1; e = t.s2; c = t.s0; zrow = z0 + j;
-        }
-        s0[j] = c; s1[j] = a; s2[j] = e;
-        srow[j] = reinterpret_cast<const int2*>(stab) + (long long)zrow * (2 * A.W + 1) + A.W;
-        if (a <= e) { pmin = min(pmin, a); pmax = max(pmax, e); }
-    }
-    const unsigned YLu = (unsigned)A.YL;
-    for (int p = pmin; p <= pmax; ++p) {
-        int2 ent[NZ];
-#pragma unroll
-        for (int j = 0; j < NZ; ++j)   // always in-bounds: k clamped to the row
-            ent[j] = srow[j][min(max(p - s0[j], -A.W), A.W)];
-        bool need_lr = false;
-#pragma unroll
-        for (int j = 0; j < NZ; ++j)
-            need_lr |= (p >= s1[j]) && (p <= s2[j]) && (ent[j].y != kNegInf);
-        const float2 cs = vtab[p];
-        const T* vb = fp + (long long)p * A.view_stride;     // wavefront-uniform base
-        const float UU = -x * cs.x - y * cs.y;
-        const float Yr = -x * cs.y + y * cs.x;
-        const float t1sq = A.SO2 - Yr * Yr;
-        const float temp1 = fast_sqrt(t1sq);
-        const float U1 = Yr * A.invdYL + A.YLC;
-        const bool uok = (U1 > 0.f) && (U1 <= A.YLm1);     // 0 < ceil(U1) < YL
-        const float Uc = ceilf(U1);
-        const float Dey = Uc - U1;
-        const unsigned ucol = uok ? (unsigned)Uc - 1u : 0u;
-        const float K = A.KSD * fast_rcp(fmaf(UU, temp1, t1sq));
-        float LR = 0.f;
-        if (need_lr && !A.k1zero)
-            LR = A.k1 * fast_log2(fabsf((temp1 - UU) * fast_rcp(temp1 + UU)));
-#pragma unroll
-        for (int j = 0; j < NZ; ++j) {
-            if (p >= s1[j] && p <= s2[j]) {
-                const float temp2 = __int_as_float(ent[j].x);
-                const float V1 = fmaf(temp2, K, A.ZLC);
-                if (uok && V1 > 0.f && V1 <= A.ZLm1) {        // 0 < ceil(V1) < ZL
-                    const float Vc = ceilf(V1);
-                    const float Dez = Vc - V1;
-                    const unsigned o = ((unsigned)Vc - 1u) * YLu + ucol;
-                    const float2 r0 = load2<T>(vb + o);          // row V-1: U-1, U
-                    const float2 r1 = load2<T>(vb + (o + YLu));  // row V:   U-1, U
-                    const float top = fmaf(Dey, r0.x - r0.y, r0.y);
-                    const float bot = fmaf(Dey, r1.x - r1.y, r1.y);
-                    const float tou = fmaf(Dez, top - bot, bot);
-                    if (ent[j].y == kNegInf) {
-                        acc[j] += tou;
-                    } else {
-                        const float wgt =
-                            fast_rcp(1.f + fast_exp2(__int_as_float(ent[j].y) + LR));
-                        acc[j] = fmaf(wgt, tou, acc[j]);
-                    }
-                }
-            }
-        }
+    for (int j = 0; j < NZ; ++j) acc[j] = 0.f;
+    const int c0 = max(h.clo, h.pmin), c1 = min(h.chi, h.pmax);
+    if (c0 <= c1) {
+        for (int p = h.pmin; p < c0; ++p) bp_view<NZ, T, true>(p, x, y, fp, vtab, trow, A, rowb, acc);
+        for (int p = c0; p <= c1; ++p) bp_view<NZ, T, false>(p, x, y, fp, vtab, trow, A, rowb, acc);
+        for (int p = c1 + 1; p <= h.pmax; ++p)
+            bp_view<NZ, T, true>(p, x, y, fp, vtab, trow, A, rowb, acc);
+    } else {
+        for (int p = h.pmin; p <= h.pmax; ++p)
+            bp_view<NZ, T, true>(p, x, y, fp, vtab, trow, A, rowb, acc);
     }
     if (xi < A.XN && yi < A.YN) {
         float* o = img + ((long long)yi * A.XN + xi) * A.ZN + z0;
@@ -391,7 +434,7 @@ void validate_geom(const arecon_bp_geom& g) {
 }
 
 int auto_nz(const std::vector<BPZ>& z, int W) {
-    // Largest NZ whose union of NZ consecutive windows stays within ~20% of one
+    // Largest NZ whose union of NZ consecutive windows stays within ~5% of one
     // window (slices further apart share too few views to amortise the column
     // work).
     const int n = (int)z.size();
@@ -399,7 +442,7 @@ int auto_nz(const std::vector<BPZ>& z, int W) {
     if (n > 1) ds = std::fabs((double)(z[n - 1].s0 - z[0].s0)) / (n - 1);
     const double L = 2.0 * W + 1.0;
     for (int nz : {16, 8, 4, 2})
-        if ((nz - 1) * ds <= 0.2 * L && n >= nz) return nz;
+        if ((nz - 1) * ds <= 0.05 * L && n >= nz) return nz;
     return 1;
 }
 
@@ -410,13 +453,78 @@ struct BPPlan {
     int device = 0;
     float2* vtab = nullptr;
     float4* ktab = nullptr;
-    float2* stab = nullptr;     // per (slice, offset): temp2, Q
+    ZSet* zsets = nullptr;      // per slice set: view ranges and table offset
+    float2* vt_table = nullptr; // per (slice set, view, slice): temp2, Q
     BPZ* ztab = nullptr;
     BPArgs a{};
     int variant = 0;
     int nz = 1;
     int grid = 0;
 };
+
+// Per-(slice set, view, slice) table of the view-table kernel, FP64 on the host:
+// temp2 = z - h V / 2pi (:79) and a = 2^Q, Q = log2(w2/w1) + k1 log2(|temp2|/|temp2 -+ 0.5|)
+// (:95-102); a = 0 where w2 = 0; temp2 = 1e30 where the view is outside the
+// slice's clamped window (:63-71).
+static void build_view_table(BPPlan* P, const double* w, const std::vector<BPZ>& zt) {
+    const arecon_bp_geom& g = P->g;
+    const int NZ = P->nz, nsets = P->a.nzsets;
+    std::vector<ZSet> sets(nsets);
+    std::vector<float2> tab;
+    for (int zs = 0; zs < nsets; ++zs) {
+        ZSet h{};
+        h.pmin = INT_MAX; h.pmax = INT_MIN;
+        for (int j = 0; j < NZ; ++j) {
+            const int zi = zs * NZ + j;
+            if (zi < g.ZN && zt[zi].s1 <= zt[zi].s2) {
+                h.pmin = std::min(h.pmin, zt[zi].s1);
+                h.pmax = std::max(h.pmax, zt[zi].s2);
+            }
+        }
+        h.off = (long long)tab.size();
+        h.clo = 1; h.chi = 0;
+        if (h.pmin > h.pmax) { h.pmin = 1; h.pmax = 0; sets[zs] = h; continue; }
+        int run_lo = -1, best_lo = 1, best_hi = 0;
+        for (int p = h.pmin; p <= h.pmax; ++p) {
+            bool unweighted = true;
+            for (int j = 0; j < NZ; ++j) {
+                const int zi = zs * NZ + j;
+                float2 e = make_float2(1e30f, 0.f);
+                if (zi < g.ZN && p >= zt[zi].s1 && p <= zt[zi].s2) {
+                    const int k = p - zt[zi].s0;
+                    const double z = (zi - g.ZNC) * g.dz;
+                    const double V = g.BetaS + p * g.DeltaFai;
+                    const double temp2 = z - g.h * V * kInvTwoPi;
+                    const double w1 = w[weight_index1(g.N_pi, k)], w2 = w[weight_index2(g.N_pi, k)];
+                    double Q = -INFINITY;
+                    if (w2 != 0.0) {
+                        Q = std::log2(w2) - std::log2(w1);
+                        if (g.k1 != 0.0)
+                            Q += g.k1 * (std::log2(std::fabs(temp2)) -
+                                         std::log2(std::fabs(temp2 + (k < 0 ? -0.5 : 0.5))));
+                        unweighted = false;
+                    }
+                    // 2^Q in FP32: 0 for Q = -inf, +inf for Q = +inf (weight 0)
+                    e = make_float2((float)temp2, (float)std::exp2(Q));
+                }
+                tab.push_back(e);
+            }
+            if (unweighted) {
+                if (run_lo < 0) run_lo = p;
+                if (p - run_lo > best_hi - best_lo) { best_lo = run_lo; best_hi = p; }
+            } else {
+                run_lo = -1;
+            }
+        }
+        h.clo = best_lo; h.chi = best_hi;
+        sets[zs] = h;
+    }
+    if (tab.empty()) tab.push_back(make_float2(1e30f, 0.f));
+    ARECON_HIP(hipMalloc(&P->zsets, sizeof(ZSet) * sets.size()));
+    ARECON_HIP(hipMalloc(&P->vt_table, sizeof(float2) * tab.size()));
+    ARECON_HIP(hipMemcpy(P->zsets, sets.data(), sizeof(ZSet) * sets.size(), hipMemcpyHostToDevice));
+    ARECON_HIP(hipMemcpy(P->vt_table, tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice));
+}
 
 BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
     validate_geom(g);
@@ -454,39 +562,15 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         e.dz = (float)(z - g.h * (g.BetaS + s0 * g.DeltaFai) * kInvTwoPi);
         zt[zi] = e;
     }
-    // Per-(slice, offset) table of the slice-table kernel: temp2 = z - h V / 2pi
-    // (:79) and Q = log2(w2/w1) + k1 log2(|temp2| / |temp2 -+ 0.5|) (:95-102),
-    // -inf where w2 = 0 (weight exactly 1), all in FP64.
-    const int L = 2 * W + 1;
-    std::vector<float2> st((size_t)g.ZN * L);
-    for (int zi = 0; zi < g.ZN; ++zi) {
-        const double z = (zi - g.ZNC) * g.dz;
-        const int s0 = zt[zi].s0;
-        for (int k = -W; k <= W; ++k) {
-            const double V = g.BetaS + (s0 + k) * g.DeltaFai;
-            const double temp2 = z - g.h * V * kInvTwoPi;
-            const double w1 = w[weight_index1(g.N_pi, k)], w2 = w[weight_index2(g.N_pi, k)];
-            double Q;
-            if (w2 == 0.0) {
-                Q = -INFINITY;
-            } else {
-                Q = std::log2(w2) - std::log2(w1);
-                if (g.k1 != 0.0)
-                    Q += g.k1 * (std::log2(std::fabs(temp2)) -
-                                 std::log2(std::fabs(temp2 + (k < 0 ? -0.5 : 0.5))));
-            }
-            st[(size_t)zi * L + (k + W)] = make_float2((float)temp2, (float)Q);
-        }
-    }
+    ARECON_REQUIRE(g.variant == 0 || g.variant == 1, "bp: variant must be 0 or 1");
+    ARECON_REQUIRE(g.order == 0 || g.order == 1, "bp: order must be 0 or 1");
+    const int nz = g.nz > 0 ? g.nz : auto_nz(zt, W);
+    ARECON_REQUIRE(nz == 1 || nz == 2 || nz == 4 || nz == 8 || nz == 16,
+                   "bp: nz must be 0, 1, 2, 4, 8 or 16");
     BPPlan* P = new BPPlan();
     P->g = g;
     P->variant = g.variant;
-    ARECON_REQUIRE(g.variant == 0 || g.variant == 1, "bp: variant must be 0 or 1");
-    ARECON_HIP(hipGetDevice(&P->device));
-    P->nz = g.nz > 0 ? g.nz : auto_nz(zt, W);
-    ARECON_REQUIRE(P->nz == 1 || P->nz == 2 || P->nz == 4 || P->nz == 8 || P->nz == 16,
-                   "bp: nz must be 0, 1, 2, 4, 8 or 16");
-    ARECON_REQUIRE(g.order == 0 || g.order == 1, "bp: order must be 0 or 1");
+    P->nz = nz;
     BPArgs& A = P->a;
     A.XN = g.XN; A.YN = g.YN; A.ZN = g.ZN; A.YL = g.YL; A.ZL = g.ZL; A.PN = g.PN; A.W = W;
     A.tilesx = (g.XN + 15) / 16;
@@ -503,14 +587,17 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
     A.k1 = (float)g.k1; A.DeltaFai = (float)g.DeltaFai;
     A.view_stride = (long long)g.ZL * g.YL;
     const long long grid = (long long)A.ntiles * A.nzsets;
-    ARECON_REQUIRE(grid < (1LL << 31), "bp: volume too large for one launch");
     P->grid = (int)grid;
     try {
+        ARECON_REQUIRE(A.view_stride * 4 < (1LL << 31), "bp: one view exceeds 2 GiB");
+        ARECON_REQUIRE((long long)g.YL * 4 < (1LL << 24) && g.ZL < (1 << 24),
+                       "bp: detector too large for 24-bit row offsets");
+        ARECON_REQUIRE(grid < (1LL << 31), "bp: volume too large for one launch");
+        ARECON_HIP(hipGetDevice(&P->device));
         ARECON_HIP(hipMalloc(&P->vtab, sizeof(float2) * vt.size()));
         ARECON_HIP(hipMalloc(&P->ktab, sizeof(float4) * kt.size()));
         ARECON_HIP(hipMalloc(&P->ztab, sizeof(BPZ) * zt.size()));
-        ARECON_HIP(hipMalloc(&P->stab, sizeof(float2) * st.size()));
-        ARECON_HIP(hipMemcpy(P->stab, st.data(), sizeof(float2) * st.size(), hipMemcpyHostToDevice));
+        if (P->variant == 0) build_view_table(P, w, zt);
         ARECON_HIP(hipMemcpy(P->vtab, vt.data(), sizeof(float2) * vt.size(), hipMemcpyHostToDevice));
         ARECON_HIP(hipMemcpy(P->ktab, kt.data(), sizeof(float4) * kt.size(), hipMemcpyHostToDevice));
         ARECON_HIP(hipMemcpy(P->ztab, zt.data(), sizeof(BPZ) * zt.size(), hipMemcpyHostToDevice));
@@ -531,7 +618,8 @@ void bp_plan_destroy(BPPlan* P) {
     if (P->vtab) (void)hipFree(P->vtab);
     if (P->ktab) (void)hipFree(P->ktab);
     if (P->ztab) (void)hipFree(P->ztab);
-    if (P->stab) (void)hipFree(P->stab);
+    if (P->zsets) (void)hipFree(P->zsets);
+    if (P->vt_table) (void)hipFree(P->vt_table);
     (void)hipSetDevice(cur);
     delete P;
 }
@@ -540,12 +628,13 @@ template <int NZ, class T>
 static void launch_nz(const BPPlan* P, float* img, const T* fp, hipStream_t s) {
     BPArgs a = P->a;
     a.vec4 = (a.ZN % 4 == 0) && ((reinterpret_cast<uintptr_t>(img) & 15) == 0);
+    a.view_bytes = (unsigned)(a.view_stride * (long long)sizeof(T));
     if (P->variant == 1)
         hipLaunchKernelGGL((bp_fdk3dw<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp, P->vtab,
                            P->ktab, P->ztab, a);
     else
-        hipLaunchKernelGGL((bp_fdk3dw_st<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp,
-                           P->vtab, P->stab, P->ztab, a);
+        hipLaunchKernelGGL((bp_fdk3dw_vt<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp,
+                           P->vtab, P->zsets, P->vt_table, a);
 }
 
 template <class T>

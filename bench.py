@@ -146,7 +146,7 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"bp_fdk3dw{'_st' if args.variant == 0 else ''}<{fdk.plan_nz},float>",
+                "kernel": f"bp_fdk3dw{'_vt' if args.variant == 0 else ''}<{fdk.plan_nz},float>",
                 "kernel_ms": round(bp_ms, 3),
                 "bytes_per_update": 16, "note": "gathers are served from L2/MALL; "
                 "compulsory HBM bytes are the filtered sinogram once and the volume once"}
