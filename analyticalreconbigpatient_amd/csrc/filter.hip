@@ -36,6 +36,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 namespace arecon {
@@ -214,6 +215,199 @@ __global__ __launch_bounds__(256) void filter_rows(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wavefront-per-transform kernel (N = 256 .. 2048).  Each thread keeps N/T
+// points in registers (T = 64, or 128 at N = 2048); a pass is a register DFT of
+// radix 16, 8 or 4 (Stockham order), and LDS only carries the data between
+// passes, in place, in a workgroup of one or two wavefronts.  Twiddles of a
+// butterfly come from one table read w^1 and a short power recurrence; the
+// spectrum H is read through L1.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cscale(float2 a, float c, float s) {   // a * (c + i s)
+    return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+template <int R> __device__ __forceinline__ void dft(float2* v);
+template <> __device__ __forceinline__ void dft<4>(float2* v) { bfly4(v[0], v[1], v[2], v[3]); }
+template <> __device__ __forceinline__ void dft<8>(float2* v) {
+    // n = n2 + 2 n1, k = k1 + 4 k2: X[k1 + 4k2] = y0[k1] + (-1)^k2 W8^k1 y1[k1]
+    float2 a0 = v[0], a1 = v[2], a2 = v[4], a3 = v[6];
+    float2 b0 = v[1], b1 = v[3], b2 = v[5], b3 = v[7];
+    bfly4(a0, a1, a2, a3);
+    bfly4(b0, b1, b2, b3);
+    const float h = 0.70710678118654752f;
+    b1 = cscale(b1, h, -h);
+    b2 = make_float2(b2.y, -b2.x);                        // * -i
+    b3 = cscale(b3, -h, -h);
+    v[0] = cadd(a0, b0); v[4] = csub(a0, b0);
+    v[1] = cadd(a1, b1); v[5] = csub(a1, b1);
+    v[2] = cadd(a2, b2); v[6] = csub(a2, b2);
+    v[3] = cadd(a3, b3); v[7] = csub(a3, b3);
+}
+template <> __device__ __forceinline__ void dft<16>(float2* v) {
+    // n = n2 + 4 n1, k = k1 + 4 k2: inner DFT4 over n1, twiddle W16^(n2 k1), outer DFT4 over n2
+    float2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        float2 a0 = v[n2], a1 = v[n2 + 4], a2 = v[n2 + 8], a3 = v[n2 + 12];
+        bfly4(a0, a1, a2, a3);
+        y[n2][0] = a0; y[n2][1] = a1; y[n2][2] = a2; y[n2][3] = a3;
+    }
+    const float c1 = 0.92387953251128674f, s1 = 0.38268343236508977f, h = 0.70710678118654752f;
+    // W16^m = cos(pi m/8) - i sin(pi m/8), m = n2 * k1
+    y[1][1] = cscale(y[1][1], c1, -s1);
+    y[1][2] = cscale(y[1][2], h, -h);
+    y[1][3] = cscale(y[1][3], s1, -c1);
+    y[2][1] = cscale(y[2][1], h, -h);
+    y[2][2] = make_float2(y[2][2].y, -y[2][2].x);
+    y[2][3] = cscale(y[2][3], -h, -h);
+    y[3][1] = cscale(y[3][1], s1, -c1);
+    y[3][2] = cscale(y[3][2], -h, -h);
+    y[3][3] = cscale(y[3][3], -c1, s1);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 b0 = y[0][k1], b1 = y[1][k1], b2 = y[2][k1], b3 = y[3][k1];
+        bfly4(b0, b1, b2, b3);
+        v[k1] = b0; v[k1 + 4] = b1; v[k1 + 8] = b2; v[k1 + 12] = b3;
+    }
+}
+
+// v[r] *= w^r for r = 1 .. R-1 (powers by repeated squaring/products).
+template <int R> __device__ __forceinline__ void twiddle(float2* v, float2 w1) {
+    float2 wp[R];
+    wp[1] = w1;
+#pragma unroll
+    for (int r = 2; r < R; ++r) wp[r] = (r & 1) ? cmul(wp[r - 1], w1) : cmul(wp[r / 2], wp[r / 2]);
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], wp[r]);
+}
+
+// Pass modes: first forward pass reads global rows (preweighted); last forward
+// pass multiplies by H and conjugates; last inverse pass stores rows.
+enum { SRC_LDS = 0, SRC_GLOBAL = 1 };
+enum { DST_LDS = 0, DST_LDS_SPEC = 1, DST_GLOBAL = 2 };
+
+struct RowCtx {
+    const float* ia; const float* ib;      // input rows (ib valid when has_b)
+    float ba, bb;                           // row heights for the preweight
+    bool has_b;
+    int YL;
+    float PLC, dYL, SO;
+    const float* H;
+};
+
+// Threads per transform: one wavefront up to N = 1024, two at N = 2048 (one
+// radix-16 butterfly per thread keeps the register count near 100).
+template <int N> constexpr int tpt() { return N >= 2048 ? N / 16 : 64; }
+
+template <int N, int R, int Ns, int SRC, int DST, class OutT>
+__device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict__ twg,
+                                          const RowCtx& c, OutT* oa, OutT* ob) {
+    constexpr int T = tpt<N>();
+    constexpr int NB = N / R, PER = NB / T;
+    static_assert(PER >= 1 && PER * T == NB, "radix schedule does not fit the thread count");
+    const int lane = threadIdx.x;
+    float2 v[PER][R];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = lane + T * b;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int n = j + r * NB;
+            if (SRC == SRC_LDS) {
+                v[b][r] = lds[n];
+            } else if (r < R / 2) {                          // n < N/2; n >= YL reads zero
+                float2 x = make_float2(0.f, 0.f);
+                if (n < c.YL) {
+                    const float t = ((float)n - c.PLC) * c.dYL;
+                    const float so2 = c.SO * c.SO;
+                    x.x = c.ia[n] * (so2 / sqrtf(so2 * (so2 + c.ba * c.ba) - c.ba * t * c.ba * t));
+                    if (c.has_b)
+                        x.y = c.ib[n] * (so2 / sqrtf(so2 * (so2 + c.bb * c.bb) - c.bb * t * c.bb * t));
+                }
+                v[b][r] = x;
+            } else {
+                v[b][r] = make_float2(0.f, 0.f);
+            }
+        }
+    }
+    if (SRC == SRC_LDS && DST != DST_GLOBAL) __syncthreads();   // all reads before in-place writes
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = lane + T * b;
+        const int jm = j & (Ns - 1);
+        if (Ns > 1) twiddle<R>(v[b], twg[jm * (N / (Ns * R))]);
+        dft<R>(v[b]);
+        const int d = (j - jm) * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k = d + r * Ns;
+            if (DST == DST_LDS) {
+                lds[k] = v[b][r];
+            } else if (DST == DST_LDS_SPEC) {
+                const float hs = c.H[k];
+                lds[k] = make_float2(hs * v[b][r].x, -hs * v[b][r].y);
+            } else if (k < c.YL) {
+                oa[k] = to_out<OutT>(v[b][r].x);
+                if (c.has_b) ob[k] = to_out<OutT>(-v[b][r].y);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// The whole forward (MODE 0: global rows -> spectrum in LDS) or inverse
+// (MODE 1: LDS -> global rows) transform of one pair of rows.
+template <int N, int MODE, class OutT>
+__device__ __forceinline__ void wave_fft(float2* lds, const float2* __restrict__ twg,
+                                         const RowCtx& c, OutT* oa, OutT* ob) {
+    constexpr int S0 = (MODE == 0) ? SRC_GLOBAL : SRC_LDS;
+    constexpr int DL = (MODE == 0) ? DST_LDS_SPEC : DST_GLOBAL;
+    if constexpr (N == 2048) {
+        wave_pass<N, 16, 1, S0, DST_LDS, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 8, 256, SRC_LDS, DL, OutT>(lds, twg, c, oa, ob);
+    } else if constexpr (N == 1024) {
+        wave_pass<N, 16, 1, S0, DST_LDS, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 4, 256, SRC_LDS, DL, OutT>(lds, twg, c, oa, ob);
+    } else if constexpr (N == 512) {
+        wave_pass<N, 8, 1, S0, DST_LDS, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 8, 8, SRC_LDS, DST_LDS, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 8, 64, SRC_LDS, DL, OutT>(lds, twg, c, oa, ob);
+    } else {
+        static_assert(N == 256, "wave FFT supports N = 256 .. 2048");
+        wave_pass<N, 4, 1, S0, DST_LDS, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 4, 4, SRC_LDS, DST_LDS, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 4, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 4, 64, SRC_LDS, DL, OutT>(lds, twg, c, oa, ob);
+    }
+}
+
+template <int N, class OutT>
+__global__ __launch_bounds__(tpt<N>()) void filter_rows_wave(
+    OutT* out, const float* in, const float2* __restrict__ twg, const float* __restrict__ Hg,
+    int YL, int ZL, long long nrows, float PLC, float ZLC, float dYL, float dZL, float SO) {
+    __shared__ float2 lds[N];
+    const long long npairs = (nrows + 1) >> 1;
+    for (long long pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
+        const long long ra = 2 * pair;
+        RowCtx c;
+        c.has_b = (ra + 1) < nrows;
+        c.ia = in + ra * YL;
+        c.ib = c.ia + YL;
+        const int ka = (int)(ra % ZL);
+        const int kb = (ka + 1 == ZL) ? 0 : ka + 1;
+        c.ba = ((float)ka - ZLC) * dZL;
+        c.bb = ((float)kb - ZLC) * dZL;
+        c.YL = YL; c.PLC = PLC; c.dYL = dYL; c.SO = SO; c.H = Hg;
+        OutT* oa = out + ra * YL;
+        OutT* ob = oa + YL;
+        wave_fft<N, 0, OutT>(lds, twg, c, oa, ob);
+        wave_fft<N, 1, OutT>(lds, twg, c, oa, ob);
+    }
+}
+
 struct FilterTables {
     float2* tw = nullptr;
     float* H = nullptr;
@@ -304,6 +498,29 @@ void launch_filter(void* out, int out_dtype, const float* in, int YL, int ZL, in
         (void)hipFuncSetAttribute((const void*)filter_rows<__hip_bfloat16>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
     });
+    if (N >= 256 && N <= 2048) {
+        const long long wgrid = std::min<long long>(npairs, (long long)cus * 16);
+        auto go = [&](auto outp) {
+            using OT = std::remove_pointer_t<decltype(outp)>;
+            switch (N) {
+            case 256: hipLaunchKernelGGL((filter_rows_wave<256, OT>), dim3((unsigned)wgrid), dim3(tpt<256>()), 0, s,
+                                         outp, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+            case 512: hipLaunchKernelGGL((filter_rows_wave<512, OT>), dim3((unsigned)wgrid), dim3(tpt<512>()), 0, s,
+                                         outp, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+            case 1024: hipLaunchKernelGGL((filter_rows_wave<1024, OT>), dim3((unsigned)wgrid), dim3(tpt<1024>()), 0, s,
+                                          outp, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+            default: hipLaunchKernelGGL((filter_rows_wave<2048, OT>), dim3((unsigned)wgrid), dim3(tpt<2048>()), 0, s,
+                                        outp, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+            }
+        };
+        switch (out_dtype) {
+        case ARECON_F32: go(static_cast<float*>(out)); break;
+        case ARECON_F16: go(static_cast<__half*>(out)); break;
+        default: go(static_cast<__hip_bfloat16*>(out)); break;
+        }
+        ARECON_HIP(hipGetLastError());
+        return;
+    }
     switch (out_dtype) {
     case ARECON_F32:
         hipLaunchKernelGGL(filter_rows<float>, dim3((unsigned)grid), dim3(256), lds, s,

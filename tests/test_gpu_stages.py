@@ -89,7 +89,7 @@ def run_filter(P, PLC0, ZLC0, dYL, dZL, dtype=_lib.ARECON_F32, inplace=False):
     return dout.float().cpu().numpy()
 
 
-@pytest.mark.parametrize("ViewN,ZL,YL", [(40, 16, 368), (9, 64, 888), (7, 3, 20), (5, 5, 130),
+@pytest.mark.parametrize("ViewN,ZL,YL", [(40, 16, 368), (9, 64, 888), (7, 3, 20), (5, 5, 130), (6, 3, 100),
                                          (3, 1, 1024), (11, 4, 2048)])
 def test_filter_matches_fp64_oracle(gpu, ViewN, ZL, YL):
     print(f"filter ViewN={ViewN} ZL={ZL} YL={YL}", flush=True)
