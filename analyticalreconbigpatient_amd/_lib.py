@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libarecon.so"
 ARECON_OK = 0
 ARECON_F32, ARECON_F16, ARECON_BF16 = 0, 1, 2
 ARECON_LAYOUT_VYZ, ARECON_LAYOUT_VZY = 0, 1
+ARECON_FP_ROWS, ARECON_FP_ROWPAIRS = 0, 1
 
 _f = C.c_float
 _d = C.c_double
@@ -40,6 +41,7 @@ class BPGeom(C.Structure):
         ("nz", _i),
         ("order", _i),
         ("variant", _i),
+        ("group", _i),
     ]
 
 
@@ -60,11 +62,11 @@ SIGNATURES = {
     "arecon_version": (_i, []),
     "arecon_device_count": (_i, []),
     "arecon_rebin_dev": (_i, [_p, _p, _i, _i, _i, _f, _f, _f, _f, _f, _f, _f, _i, _p]),
-    "arecon_filter_dev": (_i, [_p, _i, _p, _i, _i, _i, _f, _f, _f, _f, _f, _p]),
+    "arecon_filter_dev": (_i, [_p, _i, _i, _p, _i, _i, _i, _f, _f, _f, _f, _f, _p]),
     "arecon_bp_plan_create": (_i, [C.POINTER(BPGeom), _dp, C.POINTER(_p)]),
     "arecon_bp_plan_destroy": (_i, [_p]),
     "arecon_bp_plan_nz": (_i, [_p]),
-    "arecon_bp_plan_execute": (_i, [_p, _p, _p, _i, _p]),
+    "arecon_bp_plan_execute": (_i, [_p, _p, _p, _i, _i, _p]),
     "arecon_bp_dev_f64": (_i, [_p, _p, C.POINTER(BPGeom), _dp, _p]),
     "arecon_fdk_weights": (_i, [_dp, _i, _d, _i]),
     "arecon_project_ellipsoids_dev": (_i, [_p, _fp, _i, _i, _i, _i, _d, _d, _d, _d, _d, _d,

@@ -39,6 +39,16 @@ HIPCC_FLAGS = [
 ]
 
 
+# Per-file device-compiler options.  The backprojectors keep per-slice
+# accumulators in a register array that uniform branches update one element at a
+# time; promoted to a single vector value, every branch join would copy the whole
+# array.  Left as an alloca, the array is split into independent scalars after
+# unrolling.
+FILE_FLAGS = {
+    "backproject.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],
+}
+
+
 def _hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and Path(cand).exists():
@@ -69,7 +79,8 @@ def build_library(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     sources = sorted(CSRC.glob("*.hip"))
     objs = [BUILD / (s.stem + ".o") for s in sources]
-    cmds = [[hipcc, *HIPCC_FLAGS, "-c", str(s), "-o", str(o)] for s, o in zip(sources, objs)]
+    cmds = [[hipcc, *HIPCC_FLAGS, *FILE_FLAGS.get(s.name, []), "-c", str(s), "-o", str(o)]
+            for s, o in zip(sources, objs)]
     if verbose:
         for c in cmds:
             print(" ".join(c))

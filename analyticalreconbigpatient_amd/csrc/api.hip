@@ -169,7 +169,8 @@ void filtering(float* hfpwd, const float* hProj, const int YL, const int ZL, con
         const size_t n = (size_t)YL * ZL * ViewN;
         DevBuf d(sizeof(float) * n, s);
         h2d(d.p, hProj, n, s);
-        launch_filter(d.p, ARECON_F32, d.as<float>(), YL, ZL, ViewN, PLC, ZLC, dYL, dZL, SO, s);
+        launch_filter(d.p, ARECON_F32, ARECON_FP_ROWS, d.as<float>(), YL, ZL, ViewN, PLC, ZLC,
+                      dYL, dZL, SO, s);
         d2h(hfpwd, d.p, n, s);
         ARECON_HIP(hipStreamSynchronize(s));
     });
@@ -198,7 +199,7 @@ void ParallekFDKHelical3DWeighting_GPU_float(
         const size_t nfp = (size_t)YL * ZL * PN, nimg = (size_t)XN * YN * ZN;
         DevBuf dfp(sizeof(float) * nfp, s), dimg(sizeof(float) * nimg, s);
         h2d(dfp.p, hfp, nfp, s);
-        bp_plan_launch(plan, dimg.as<float>(), dfp.p, ARECON_F32, s);
+        bp_plan_launch(plan, dimg.as<float>(), dfp.p, ARECON_F32, ARECON_FP_ROWS, s);
         d2h(hctimage, dimg.p, nimg, s);
         ARECON_HIP(hipStreamSynchronize(s));
     });
@@ -274,7 +275,7 @@ void ParallelFDKHelical3DWeighting(double* ctimage, const double* fp, const doub
         std::vector<float> fimg(nimg);
         DevBuf dfp(sizeof(float) * nfp, s), dimg(sizeof(float) * nimg, s);
         h2d(dfp.p, ffp.data(), nfp, s);
-        bp_plan_launch(plan, dimg.as<float>(), dfp.p, ARECON_F32, s);
+        bp_plan_launch(plan, dimg.as<float>(), dfp.p, ARECON_F32, ARECON_FP_ROWS, s);
         d2h(fimg.data(), dimg.p, nimg, s);
         ARECON_HIP(hipStreamSynchronize(s));
         for (size_t i = 0; i < nimg; ++i) ctimage[i] = fimg[i];
@@ -293,10 +294,11 @@ int arecon_rebin_dev(float* d_out, const float* d_in, int YL, int ZL, int ViewN,
     });
 }
 
-int arecon_filter_dev(void* d_out, int out_dtype, const float* d_in, int YL, int ZL, int ViewN,
-                      float PLC, float ZLC, float dYL, float dZL, float SO, void* stream) {
+int arecon_filter_dev(void* d_out, int out_dtype, int out_layout, const float* d_in, int YL,
+                      int ZL, int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
+                      void* stream) {
     return guarded([&] {
-        launch_filter(d_out, out_dtype, d_in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL, SO,
+        launch_filter(d_out, out_dtype, out_layout, d_in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL, SO,
                       static_cast<hipStream_t>(stream));
     });
 }
@@ -317,9 +319,9 @@ int arecon_bp_plan_nz(const arecon_bp_plan* plan) {
 }
 
 int arecon_bp_plan_execute(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
-                           int fp_dtype, void* stream) {
+                           int fp_dtype, int fp_layout, void* stream) {
     return guarded([&] {
-        bp_plan_launch(reinterpret_cast<const BPPlan*>(plan), d_img, d_fp, fp_dtype,
+        bp_plan_launch(reinterpret_cast<const BPPlan*>(plan), d_img, d_fp, fp_dtype, fp_layout,
                        static_cast<hipStream_t>(stream));
     });
 }

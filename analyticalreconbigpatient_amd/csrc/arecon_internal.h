@@ -79,7 +79,7 @@ void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN,
                   float YLC, float dYA, float DeltaTheta, float PLC, float DeltaT,
                   float DeltaFai, float SO, int out_layout, hipStream_t s);
 
-void launch_filter(void* out, int out_dtype, const float* in, int YL, int ZL,
+void launch_filter(void* out, int out_dtype, int out_layout, const float* in, int YL, int ZL,
                    int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
                    hipStream_t s);
 
@@ -89,7 +89,7 @@ struct BPPlan;
 BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w);
 void bp_plan_destroy(BPPlan* p);
 int bp_plan_nz(const BPPlan* p);
-void bp_plan_launch(const BPPlan* p, float* img, const void* fp, int fp_dtype,
+void bp_plan_launch(const BPPlan* p, float* img, const void* fp, int fp_dtype, int fp_layout,
                     hipStream_t s);
 void bp_launch_f64(double* img, const double* fp, const arecon_bp_geom& g,
                    const double* w, hipStream_t s);

@@ -346,6 +346,194 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_vt(float* __restrict__ img,
     }
 }
 
+// Z-column kernel.  Each thread owns a block of ZB consecutive slices of its
+// column and walks the union of their windows once.  Consecutive slices' central
+// views are s0(z+dz) - s0(z) = N_2pi dz / h apart, so one view lies in the windows
+// of ~L / (N_2pi dz / h) consecutive slices: the column terms (UU, Yr, temp1, the
+// channel, the magnification and B) are computed once per (column, view) and
+// shared by all of them.  Per (block, view) the table holds two wavefront-uniform
+// bitmasks -- the slices whose window contains the view, and those of them with
+// a nonzero conjugate weight (a != 0) -- and each slice's (temp2, a) as in the
+// view-table kernel.  Inactive slices are skipped with scalar branches, so every
+// voxel still sees exactly its own window and no work is spent outside it.
+struct ZBlock {
+    int pmin, pmax;     // union of the block's windows (pmin > pmax: empty)
+    long long off;      // first (temp2, a) entry of view pmin; masks at off / ZB
+};
+
+// Bilinear sample of the z-column kernel at row offset fr (= 1 - Dez) and channel
+// blend weight Dey (on channel U-1), from the byte offset `ob` of sample
+// (row V-1, channel U-1).  Rows layout: two gathers, rows V-1 and V (the second
+// through the instruction's scalar offset).  Row-pair layout: one gather of the
+// four samples (V-1, U-1), (V, U-1), (V-1, U), (V, U).
+template <class T, int LAYOUT> struct Gather;
+
+template <class T> struct Gather<T, ARECON_FP_ROWS> {
+    static constexpr unsigned kElem = sizeof(T);            // bytes per channel step
+    __device__ static __forceinline__ float sample(__amdgpu_buffer_rsrc_t rs, unsigned ob,
+                                                   unsigned rowb, float Dey, float fr) {
+        const float2 r0 = bload2<T>(rs, ob, 0u);          // row V-1: U-1, U
+        const float2 r1 = bload2<T>(rs, ob, rowb);        // row V:   U-1, U
+        const float top = fmaf(Dey, r0.x - r0.y, r0.y);
+        const float bot = fmaf(Dey, r1.x - r1.y, r1.y);
+        return fmaf(fr, bot - top, top);
+    }
+};
+
+typedef float f32x4_v __attribute__((ext_vector_type(4)));
+
+// Four samples as floats: (V-1, U-1), (V, U-1), (V-1, U), (V, U).
+template <class T> __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ float4 bload4<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    const f32x4_v v = __builtin_bit_cast(f32x4_v, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0u, 0));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+template <> __device__ __forceinline__ float4 bload4<__half>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    const unsigned long long v =
+        __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0u, 0));
+    const unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+    return make_float4(__half2float(__ushort_as_half((unsigned short)(lo & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(lo >> 16))),
+                       __half2float(__ushort_as_half((unsigned short)(hi & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(hi >> 16))));
+}
+template <> __device__ __forceinline__ float4 bload4<__hip_bfloat16>(__amdgpu_buffer_rsrc_t r,
+                                                                    unsigned off) {
+    const unsigned long long v =
+        __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0u, 0));
+    const unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+    return make_float4(__uint_as_float(lo << 16), __uint_as_float(lo & 0xffff0000u),
+                       __uint_as_float(hi << 16), __uint_as_float(hi & 0xffff0000u));
+}
+
+template <class T> struct Gather<T, ARECON_FP_ROWPAIRS> {
+    static constexpr unsigned kElem = 2 * sizeof(T);
+    __device__ static __forceinline__ float sample(__amdgpu_buffer_rsrc_t rs, unsigned ob,
+                                                   unsigned, float Dey, float fr) {
+        const float4 q = bload4<T>(rs, ob);
+        const float c0 = fmaf(fr, q.y - q.x, q.x);        // channel U-1, rows blended
+        const float c1 = fmaf(fr, q.w - q.z, q.z);        // channel U
+        return fmaf(Dey, c0 - c1, c1);
+    }
+};
+
+// One group of G consecutive slices of the block, branch-free: the slices of the
+// group outside the view's window carry temp2 = 1e30 in the table, so their row
+// is off the detector and their gathers read 0.  The row index is floor(V1)
+// (= ceil(V1) - 1 of the reference for every non-integer V1; for an integer V1
+// both pick the same sample) and the row blend weight is fract(V1) = 1 - Dez.
+template <int G, class T, int LAYOUT, bool WEIGHTED>
+__device__ __forceinline__ void zc_group(const float2* __restrict__ e, float K, float zlc,
+                                         float B, unsigned ucolb, unsigned rowb,
+                                         unsigned zlm1, __amdgpu_buffer_rsrc_t rs, float Dey,
+                                         float* acc) {
+    float tou[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const float V1 = fmaf(e[i].x, K, zlc);
+        const float Vf = floorf(V1);
+        const unsigned vi = (unsigned)(int)Vf;                  // row V-1 of the reference
+        const unsigned ob = vi < zlm1 ? __umul24(vi, rowb) + ucolb : 0x80000000u;
+        tou[i] = Gather<T, LAYOUT>::sample(rs, ob, rowb, Dey, V1 - Vf);
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        if (WEIGHTED)   // 1 / (1 + 2^Q B); exactly 1 where a = 0
+            acc[i] = fmaf(fast_rcp(fmaf(e[i].y, B, 1.f)), tou[i], acc[i]);
+        else
+            acc[i] += tou[i];
+    }
+}
+
+template <int ZB, int G, class T, int LAYOUT>
+__global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
+                                                    const T* __restrict__ fp,
+                                                    const float2* __restrict__ vtab,
+                                                    const ZBlock* __restrict__ blocks,
+                                                    const uint2* __restrict__ masks,
+                                                    const float2* __restrict__ table, BPArgs A) {
+    static_assert(ZB % G == 0 && ZB <= 32 && (G == 1 || G == 2 || G == 4), "bad block shape");
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int qx = nb >> 3, rx = nb & 7, xcd = b & 7;
+    const int lb = xcd * qx + min(xcd, rx) + (b >> 3);
+    int zs, tile;
+    if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
+    else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
+    const int tyb = tile / A.tilesx, txb = tile - tyb * A.tilesx;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int xi = txb * 16 + (wave & 1) * 8 + (lane & 7);
+    const int yi = tyb * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const float x = ((float)xi - A.XNC) * A.dx;
+    const float y = ((float)yi - A.YNC) * A.dy;
+    const int z0 = zs * ZB;
+    const ZBlock hb = blocks[zs];
+    const uint2* mrow = masks + hb.off / ZB - hb.pmin;
+    const float2* trow = table + hb.off - (long long)hb.pmin * ZB;
+    constexpr unsigned kElem = Gather<T, LAYOUT>::kElem;
+    const unsigned rowb = (unsigned)A.YL * kElem;           // bytes per detector row
+    const unsigned zlm1 = (unsigned)A.ZL - 1u;
+    // ZLC as a per-lane value, so the per-slice fma takes the table's temp2 as
+    // its one scalar operand.
+    float zlc;
+    asm("v_mov_b32 %0, %1" : "=v"(zlc) : "s"(A.ZLC));
+    constexpr unsigned gmask = (1u << G) - 1u;
+    float acc[ZB];
+#pragma unroll
+    for (int j = 0; j < ZB; ++j) acc[j] = 0.f;
+
+    for (int p = hb.pmin; p <= hb.pmax; ++p) {
+        const uint2 m = mrow[p];                       // scalar: active, weighted
+        const float2 cs = vtab[p];
+        const __amdgpu_buffer_rsrc_t rs =
+            view_rsrc(fp + (long long)p * A.view_stride, A.view_bytes);
+        const float UU = -x * cs.x - y * cs.y;
+        const float Yr = -x * cs.y + y * cs.x;
+        const float t1sq = A.SO2 - Yr * Yr;
+        const float temp1 = fast_sqrt(t1sq);
+        const float U1 = Yr * A.invdYL + A.YLC;
+        const float den = fmaf(UU, temp1, t1sq);              // t1 (t1 + UU)
+        // 0 < ceil(U1) < YL, and a finite magnification: the reference's row test
+        // rejects the NaN / infinite Zr of a voxel outside the source circle
+        const bool uok = (U1 > 0.f) && (U1 <= A.YLm1) && (t1sq >= 0.f) && (den != 0.f);
+        const float Uc = ceilf(U1);
+        const float Dey = Uc - U1;
+        // byte offset of channel U-1; off the detector, past any view (< 2 GiB) so
+        // both gathers read 0 whatever the row
+        const unsigned ucolb = uok ? ((unsigned)Uc - 1u) * kElem : 0x80000000u;
+        const float inv = fast_rcp(den);
+        const float K = uok ? A.KSD * inv : 0.f;
+        float B = 1.f;
+        if (m.y != 0u)   // B = ((t1-UU)/(t1+UU))^k1, kept finite and nonzero
+            B = fminf(fmaxf(fast_exp2(A.k1 * fast_log2(fabsf((temp1 - UU) * temp1 * inv))),
+                            1e-30f), 1e30f);
+        const float2* erow = trow + (long long)p * ZB;
+#pragma unroll
+        for (int g = 0; g < ZB; g += G) {
+            if (((m.x >> g) & gmask) == 0u) continue;
+            if ((m.y >> g) & gmask)
+                zc_group<G, T, LAYOUT, true>(erow + g, K, zlc, B, ucolb, rowb, zlm1, rs, Dey,
+                                             acc + g);
+            else
+                zc_group<G, T, LAYOUT, false>(erow + g, K, zlc, B, ucolb, rowb, zlm1, rs, Dey,
+                                              acc + g);
+        }
+    }
+    if (xi < A.XN && yi < A.YN) {
+        float* o = img + ((long long)yi * A.XN + xi) * A.ZN + z0;
+        if (A.vec4 && z0 + ZB <= A.ZN) {
+#pragma unroll
+            for (int j = 0; j < ZB; j += 4)
+                *reinterpret_cast<float4*>(o + j) =
+                    make_float4(acc[j] * A.DeltaFai, acc[j + 1] * A.DeltaFai,
+                                acc[j + 2] * A.DeltaFai, acc[j + 3] * A.DeltaFai);
+        } else {
+#pragma unroll
+            for (int j = 0; j < ZB; ++j)
+                if (z0 + j < A.ZN) o[j] = acc[j] * A.DeltaFai;
+        }
+    }
+}
+
 // FP64 path: the same math as the reference, one thread per voxel, in double.
 __global__ __launch_bounds__(256) void bp_fdk3dw_f64(double* __restrict__ img,
                                                      const double* __restrict__ fp,
@@ -454,18 +642,47 @@ struct BPPlan {
     float2* vtab = nullptr;
     float4* ktab = nullptr;
     ZSet* zsets = nullptr;      // per slice set: view ranges and table offset
-    float2* vt_table = nullptr; // per (slice set, view, slice): temp2, Q
+    float2* vt_table = nullptr; // per (slice set or block, view, slice): temp2, a
+    ZBlock* zblocks = nullptr;  // z-column kernel: per block view range and offset
+    uint2* zmasks = nullptr;    // z-column kernel: per (block, view) slice masks
     BPZ* ztab = nullptr;
     BPArgs a{};
     int variant = 0;
     int nz = 1;
+    int group = 1;      // z-column kernel: slices per branch-free group
     int grid = 0;
 };
 
-// Per-(slice set, view, slice) table of the view-table kernel, FP64 on the host:
-// temp2 = z - h V / 2pi (:79) and a = 2^Q, Q = log2(w2/w1) + k1 log2(|temp2|/|temp2 -+ 0.5|)
-// (:95-102); a = 0 where w2 = 0; temp2 = 1e30 where the view is outside the
-// slice's clamped window (:63-71).
+// One slice's entry for one view, FP64 on the host: temp2 = z - h V / 2pi (:79) and
+// a = 2^Q, Q = log2(w2/w1) + k1 log2(|temp2|/|temp2 -+ 0.5|) (:95-102); a = 0 where
+// w2 = 0.  Returns false (entry (1e30, 0)) when the view is outside the slice's
+// clamped window (:63-71); `weighted` tells whether w2 != 0.
+static bool slice_view_entry(const arecon_bp_geom& g, const double* w, const std::vector<BPZ>& zt,
+                             int zi, int p, float2& e, bool& weighted) {
+    e = make_float2(1e30f, 0.f);
+    weighted = false;
+    if (zi >= g.ZN || p < zt[zi].s1 || p > zt[zi].s2) return false;
+    const int k = p - zt[zi].s0;
+    const double z = (zi - g.ZNC) * g.dz;
+    const double V = g.BetaS + p * g.DeltaFai;
+    const double temp2 = z - g.h * V * kInvTwoPi;
+    const double w1 = w[weight_index1(g.N_pi, k)], w2 = w[weight_index2(g.N_pi, k)];
+    double Q = -INFINITY;
+    if (w2 != 0.0) {
+        Q = std::log2(w2) - std::log2(w1);
+        if (g.k1 != 0.0)
+            Q += g.k1 * (std::log2(std::fabs(temp2)) -
+                         std::log2(std::fabs(temp2 + (k < 0 ? -0.5 : 0.5))));
+        weighted = true;
+    }
+    // 2^Q in FP32: 0 for Q = -inf, +inf for Q = +inf (weight 0)
+    e = make_float2((float)temp2, (float)std::exp2(Q));
+    return true;
+}
+
+// Per-(slice set, view, slice) table of the view-table kernel.  A slice whose
+// window does not contain the view gets temp2 = 1e30 (its row is off the
+// detector).
 static void build_view_table(BPPlan* P, const double* w, const std::vector<BPZ>& zt) {
     const arecon_bp_geom& g = P->g;
     const int NZ = P->nz, nsets = P->a.nzsets;
@@ -488,25 +705,10 @@ static void build_view_table(BPPlan* P, const double* w, const std::vector<BPZ>&
         for (int p = h.pmin; p <= h.pmax; ++p) {
             bool unweighted = true;
             for (int j = 0; j < NZ; ++j) {
-                const int zi = zs * NZ + j;
-                float2 e = make_float2(1e30f, 0.f);
-                if (zi < g.ZN && p >= zt[zi].s1 && p <= zt[zi].s2) {
-                    const int k = p - zt[zi].s0;
-                    const double z = (zi - g.ZNC) * g.dz;
-                    const double V = g.BetaS + p * g.DeltaFai;
-                    const double temp2 = z - g.h * V * kInvTwoPi;
-                    const double w1 = w[weight_index1(g.N_pi, k)], w2 = w[weight_index2(g.N_pi, k)];
-                    double Q = -INFINITY;
-                    if (w2 != 0.0) {
-                        Q = std::log2(w2) - std::log2(w1);
-                        if (g.k1 != 0.0)
-                            Q += g.k1 * (std::log2(std::fabs(temp2)) -
-                                         std::log2(std::fabs(temp2 + (k < 0 ? -0.5 : 0.5))));
-                        unweighted = false;
-                    }
-                    // 2^Q in FP32: 0 for Q = -inf, +inf for Q = +inf (weight 0)
-                    e = make_float2((float)temp2, (float)std::exp2(Q));
-                }
+                float2 e;
+                bool wt;
+                slice_view_entry(g, w, zt, zs * NZ + j, p, e, wt);
+                if (wt) unweighted = false;
                 tab.push_back(e);
             }
             if (unweighted) {
@@ -523,6 +725,54 @@ static void build_view_table(BPPlan* P, const double* w, const std::vector<BPZ>&
     ARECON_HIP(hipMalloc(&P->zsets, sizeof(ZSet) * sets.size()));
     ARECON_HIP(hipMalloc(&P->vt_table, sizeof(float2) * tab.size()));
     ARECON_HIP(hipMemcpy(P->zsets, sets.data(), sizeof(ZSet) * sets.size(), hipMemcpyHostToDevice));
+    ARECON_HIP(hipMemcpy(P->vt_table, tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice));
+}
+
+// Per-(block, view) masks and per-(block, view, slice) entries of the z-column
+// kernel.  Bit j of masks.x: slice j's window contains the view; bit j of
+// masks.y: that slice's weight has a conjugate term (a != 0).
+static void build_block_table(BPPlan* P, const double* w, const std::vector<BPZ>& zt) {
+    const arecon_bp_geom& g = P->g;
+    const int ZB = P->nz, nblk = P->a.nzsets;
+    std::vector<ZBlock> blks(nblk);
+    std::vector<float2> tab;
+    std::vector<uint2> msk;
+    for (int zb = 0; zb < nblk; ++zb) {
+        ZBlock h{};
+        h.pmin = INT_MAX; h.pmax = INT_MIN;
+        for (int j = 0; j < ZB; ++j) {
+            const int zi = zb * ZB + j;
+            if (zi < g.ZN && zt[zi].s1 <= zt[zi].s2) {
+                h.pmin = std::min(h.pmin, zt[zi].s1);
+                h.pmax = std::max(h.pmax, zt[zi].s2);
+            }
+        }
+        h.off = (long long)tab.size();
+        if (h.pmin > h.pmax) { h.pmin = 1; h.pmax = 0; blks[zb] = h; continue; }
+        for (int p = h.pmin; p <= h.pmax; ++p) {
+            uint2 m = make_uint2(0u, 0u);
+            for (int j = 0; j < ZB; ++j) {
+                float2 e;
+                bool wt;
+                if (slice_view_entry(g, w, zt, zb * ZB + j, p, e, wt)) {
+                    m.x |= 1u << j;
+                    if (wt) m.y |= 1u << j;
+                }
+                tab.push_back(e);
+            }
+            msk.push_back(m);
+        }
+        blks[zb] = h;
+    }
+    if (tab.empty()) {
+        tab.assign(ZB, make_float2(1e30f, 0.f));
+        msk.push_back(make_uint2(0u, 0u));
+    }
+    ARECON_HIP(hipMalloc(&P->zblocks, sizeof(ZBlock) * blks.size()));
+    ARECON_HIP(hipMalloc(&P->zmasks, sizeof(uint2) * msk.size()));
+    ARECON_HIP(hipMalloc(&P->vt_table, sizeof(float2) * tab.size()));
+    ARECON_HIP(hipMemcpy(P->zblocks, blks.data(), sizeof(ZBlock) * blks.size(), hipMemcpyHostToDevice));
+    ARECON_HIP(hipMemcpy(P->zmasks, msk.data(), sizeof(uint2) * msk.size(), hipMemcpyHostToDevice));
     ARECON_HIP(hipMemcpy(P->vt_table, tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice));
 }
 
@@ -562,15 +812,27 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         e.dz = (float)(z - g.h * (g.BetaS + s0 * g.DeltaFai) * kInvTwoPi);
         zt[zi] = e;
     }
-    ARECON_REQUIRE(g.variant == 0 || g.variant == 1, "bp: variant must be 0 or 1");
+    ARECON_REQUIRE(g.variant >= 0 && g.variant <= 2, "bp: variant must be 0, 1 or 2");
     ARECON_REQUIRE(g.order == 0 || g.order == 1, "bp: order must be 0 or 1");
-    const int nz = g.nz > 0 ? g.nz : auto_nz(zt, W);
-    ARECON_REQUIRE(nz == 1 || nz == 2 || nz == 4 || nz == 8 || nz == 16,
-                   "bp: nz must be 0, 1, 2, 4, 8 or 16");
+    int nz;
+    int group = 1;
+    if (g.variant == 2) {
+        nz = g.nz > 0 ? g.nz : 32;
+        ARECON_REQUIRE(nz == 8 || nz == 16 || nz == 32,
+                       "bp: nz of the z-column kernel must be 0, 8, 16 or 32");
+        group = g.group > 0 ? g.group : 2;
+        ARECON_REQUIRE(group == 1 || group == 2 || group == 4,
+                       "bp: group of the z-column kernel must be 0, 1, 2 or 4");
+    } else {
+        nz = g.nz > 0 ? g.nz : auto_nz(zt, W);
+        ARECON_REQUIRE(nz == 1 || nz == 2 || nz == 4 || nz == 8 || nz == 16,
+                       "bp: nz must be 0, 1, 2, 4, 8 or 16");
+    }
     BPPlan* P = new BPPlan();
     P->g = g;
     P->variant = g.variant;
     P->nz = nz;
+    P->group = group;
     BPArgs& A = P->a;
     A.XN = g.XN; A.YN = g.YN; A.ZN = g.ZN; A.YL = g.YL; A.ZL = g.ZL; A.PN = g.PN; A.W = W;
     A.tilesx = (g.XN + 15) / 16;
@@ -589,8 +851,10 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
     const long long grid = (long long)A.ntiles * A.nzsets;
     P->grid = (int)grid;
     try {
-        ARECON_REQUIRE(A.view_stride * 4 < (1LL << 31), "bp: one view exceeds 2 GiB");
-        ARECON_REQUIRE((long long)g.YL * 4 < (1LL << 24) && g.ZL < (1 << 24),
+        // one view of either layout (rows, or row pairs at twice the size) stays
+        // below 2 GiB, and rows and row strides fit the kernels' 24-bit multiplies
+        ARECON_REQUIRE(A.view_stride * 8 < (1LL << 31), "bp: one view exceeds 1 GiB");
+        ARECON_REQUIRE((long long)g.YL * 8 < (1LL << 24) && g.ZL < (1 << 24),
                        "bp: detector too large for 24-bit row offsets");
         ARECON_REQUIRE(grid < (1LL << 31), "bp: volume too large for one launch");
         ARECON_HIP(hipGetDevice(&P->device));
@@ -598,6 +862,7 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         ARECON_HIP(hipMalloc(&P->ktab, sizeof(float4) * kt.size()));
         ARECON_HIP(hipMalloc(&P->ztab, sizeof(BPZ) * zt.size()));
         if (P->variant == 0) build_view_table(P, w, zt);
+        if (P->variant == 2) build_block_table(P, w, zt);
         ARECON_HIP(hipMemcpy(P->vtab, vt.data(), sizeof(float2) * vt.size(), hipMemcpyHostToDevice));
         ARECON_HIP(hipMemcpy(P->ktab, kt.data(), sizeof(float4) * kt.size(), hipMemcpyHostToDevice));
         ARECON_HIP(hipMemcpy(P->ztab, zt.data(), sizeof(BPZ) * zt.size(), hipMemcpyHostToDevice));
@@ -620,42 +885,78 @@ void bp_plan_destroy(BPPlan* P) {
     if (P->ztab) (void)hipFree(P->ztab);
     if (P->zsets) (void)hipFree(P->zsets);
     if (P->vt_table) (void)hipFree(P->vt_table);
+    if (P->zblocks) (void)hipFree(P->zblocks);
+    if (P->zmasks) (void)hipFree(P->zmasks);
     (void)hipSetDevice(cur);
     delete P;
 }
 
 template <int NZ, class T>
-static void launch_nz(const BPPlan* P, float* img, const T* fp, hipStream_t s) {
+static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipStream_t s) {
     BPArgs a = P->a;
     a.vec4 = (a.ZN % 4 == 0) && ((reinterpret_cast<uintptr_t>(img) & 15) == 0);
+    if (layout == ARECON_FP_ROWPAIRS) a.view_stride = 2LL * (a.ZL - 1) * a.YL;
     a.view_bytes = (unsigned)(a.view_stride * (long long)sizeof(T));
-    if (P->variant == 1)
-        hipLaunchKernelGGL((bp_fdk3dw<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp, P->vtab,
-                           P->ktab, P->ztab, a);
-    else
-        hipLaunchKernelGGL((bp_fdk3dw_vt<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp,
-                           P->vtab, P->zsets, P->vt_table, a);
+    ARECON_REQUIRE(layout == ARECON_FP_ROWS || P->variant == 2,
+                   "bp: the row-pair layout needs the z-column kernel (variant 2)");
+    if constexpr (NZ <= 16) {
+        if (P->variant == 1) {
+            hipLaunchKernelGGL((bp_fdk3dw<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp,
+                               P->vtab, P->ktab, P->ztab, a);
+            return;
+        }
+        if (P->variant == 0) {
+            hipLaunchKernelGGL((bp_fdk3dw_vt<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp,
+                               P->vtab, P->zsets, P->vt_table, a);
+            return;
+        }
+    }
+    if constexpr (NZ % 8 == 0 && NZ <= 32) {
+        if (P->variant == 2) {
+#define ARECON_ZC(G, LY)                                                                      \
+    hipLaunchKernelGGL((bp_fdk3dw_zc<NZ, G, T, LY>), dim3(P->grid), dim3(256), 0, s, img, fp, \
+                       P->vtab, P->zblocks, P->zmasks, P->vt_table, a)
+            if (layout == ARECON_FP_ROWS) {
+                if (P->group == 1) ARECON_ZC(1, ARECON_FP_ROWS);
+                else if (P->group == 2) ARECON_ZC(2, ARECON_FP_ROWS);
+                else ARECON_ZC(4, ARECON_FP_ROWS);
+            } else {
+                if (P->group == 1) ARECON_ZC(1, ARECON_FP_ROWPAIRS);
+                else if (P->group == 2) ARECON_ZC(2, ARECON_FP_ROWPAIRS);
+                else ARECON_ZC(4, ARECON_FP_ROWPAIRS);
+            }
+#undef ARECON_ZC
+            return;
+        }
+    }
+    fail(ARECON_ERR_INTERNAL, "bp: no kernel for this variant and nz");
 }
 
 template <class T>
-static void launch_t(const BPPlan* P, float* img, const T* fp, hipStream_t s) {
+static void launch_t(const BPPlan* P, float* img, const T* fp, int layout, hipStream_t s) {
     switch (P->nz) {
-    case 1: launch_nz<1, T>(P, img, fp, s); break;
-    case 2: launch_nz<2, T>(P, img, fp, s); break;
-    case 4: launch_nz<4, T>(P, img, fp, s); break;
-    case 8: launch_nz<8, T>(P, img, fp, s); break;
-    default: launch_nz<16, T>(P, img, fp, s); break;
+    case 1: launch_nz<1, T>(P, img, fp, layout, s); break;
+    case 2: launch_nz<2, T>(P, img, fp, layout, s); break;
+    case 4: launch_nz<4, T>(P, img, fp, layout, s); break;
+    case 8: launch_nz<8, T>(P, img, fp, layout, s); break;
+    case 16: launch_nz<16, T>(P, img, fp, layout, s); break;
+    default: launch_nz<32, T>(P, img, fp, layout, s); break;
     }
 }
 
-void bp_plan_launch(const BPPlan* P, float* img, const void* fp, int fp_dtype, hipStream_t s) {
+void bp_plan_launch(const BPPlan* P, float* img, const void* fp, int fp_dtype, int fp_layout,
+                    hipStream_t s) {
     ARECON_REQUIRE(P != nullptr, "bp: null plan");
     ARECON_REQUIRE(img != nullptr && fp != nullptr, "bp: null buffer");
+    ARECON_REQUIRE(fp_layout == ARECON_FP_ROWS || fp_layout == ARECON_FP_ROWPAIRS,
+                   "bp: unknown fp layout");
     switch (fp_dtype) {
-    case ARECON_F32: launch_t<float>(P, img, static_cast<const float*>(fp), s); break;
-    case ARECON_F16: launch_t<__half>(P, img, static_cast<const __half*>(fp), s); break;
+    case ARECON_F32: launch_t<float>(P, img, static_cast<const float*>(fp), fp_layout, s); break;
+    case ARECON_F16:
+        launch_t<__half>(P, img, static_cast<const __half*>(fp), fp_layout, s);
+        break;
     case ARECON_BF16:
-        launch_t<__hip_bfloat16>(P, img, static_cast<const __hip_bfloat16*>(fp), s);
+        launch_t<__hip_bfloat16>(P, img, static_cast<const __hip_bfloat16*>(fp), fp_layout, s);
         break;
     default: fail(ARECON_ERR_ARG, "bp: unknown fp dtype");
     }

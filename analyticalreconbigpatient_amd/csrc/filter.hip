@@ -66,28 +66,68 @@ template <> __device__ __forceinline__ __hip_bfloat16 to_out<__hip_bfloat16>(flo
     return __float2bfloat16(v);
 }
 
+// Destinations of the two filtered rows a, b of a transform.  Rows layout: row
+// r of view v at ((v ZL + r) YL + c).  Row-pair layout ([ViewN][ZL-1][YL][2]):
+// element (v, r, c) holds rows r and r+1 of channel c, so row k goes to slot 0
+// of pair k (k < ZL-1) and to slot 1 of pair k-1 (k > 0).  Null pointers are
+// uniform across the workgroup.
+template <class OutT> struct RowOut {
+    OutT* a0; OutT* a1; OutT* b0; OutT* b1;
+    int stride;
+    __device__ __forceinline__ void put(int d, float va, float vb) const {
+        const int i = d * stride;
+        if (a0) a0[i] = to_out<OutT>(va);
+        if (a1) a1[i] = to_out<OutT>(va);
+        if (b0) b0[i] = to_out<OutT>(vb);
+        if (b1) b1[i] = to_out<OutT>(vb);
+    }
+};
+
+template <class OutT>
+__device__ __forceinline__ void row_dest(OutT* out, int layout, long long row, int YL, int ZL,
+                                         OutT*& p0, OutT*& p1) {
+    if (layout == ARECON_FP_ROWS) {
+        p0 = out + row * YL;
+        p1 = nullptr;
+        return;
+    }
+    const long long v = row / ZL;
+    const int k = (int)(row - v * ZL);
+    OutT* base = out + v * (long long)(ZL - 1) * YL * 2;
+    p0 = (k < ZL - 1) ? base + (long long)k * YL * 2 : nullptr;
+    p1 = (k > 0) ? base + (long long)(k - 1) * YL * 2 + 1 : nullptr;
+}
+
+template <class OutT>
+__device__ __forceinline__ RowOut<OutT> row_out(OutT* out, int layout, long long ra, bool has_b,
+                                                int YL, int ZL) {
+    RowOut<OutT> o;
+    row_dest(out, layout, ra, YL, ZL, o.a0, o.a1);
+    o.b0 = o.b1 = nullptr;
+    if (has_b) row_dest(out, layout, ra + 1, YL, ZL, o.b0, o.b1);
+    o.stride = (layout == ARECON_FP_ROWS) ? 1 : 2;
+    return o;
+}
+
 // Epilogue of a pass: 0 = LDS store, 1 = LDS store of H[d]*conj(v),
 // 2 = store of the first YL outputs of both rows to global memory.
 template <int EPI, class OutT>
 __device__ __forceinline__ void emit(float2* dst, int d, float2 v, const float* H,
-                                     OutT* oa, OutT* ob, int YL, bool has_b) {
+                                     const RowOut<OutT>& o, int YL) {
     if constexpr (EPI == 0) {
         dst[d] = v;
     } else if constexpr (EPI == 1) {
         const float hs = H[d];
         dst[d] = make_float2(hs * v.x, -hs * v.y);
     } else {
-        if (d < YL) {
-            oa[d] = to_out<OutT>(v.x);
-            if (has_b) ob[d] = to_out<OutT>(-v.y);
-        }
+        if (d < YL) o.put(d, v.x, -v.y);
     }
 }
 
 template <int EPI, class OutT>
 __device__ __forceinline__ void pass4(const float2* src, float2* dst, int N, int Ns,
-                                      const float2* tw, const float* H, OutT* oa,
-                                      OutT* ob, int YL, bool has_b) {
+                                      const float2* tw, const float* H,
+                                      const RowOut<OutT>& o, int YL) {
     const int q = N >> 2;
     const int tstride = N / (Ns * 4);
     for (int j = threadIdx.x; j < q; j += blockDim.x) {
@@ -99,24 +139,24 @@ __device__ __forceinline__ void pass4(const float2* src, float2* dst, int N, int
         v3 = cmul(v3, tw[3 * t]);
         bfly4(v0, v1, v2, v3);
         const int d = (j - jm) * 4 + jm;
-        emit<EPI, OutT>(dst, d, v0, H, oa, ob, YL, has_b);
-        emit<EPI, OutT>(dst, d + Ns, v1, H, oa, ob, YL, has_b);
-        emit<EPI, OutT>(dst, d + 2 * Ns, v2, H, oa, ob, YL, has_b);
-        emit<EPI, OutT>(dst, d + 3 * Ns, v3, H, oa, ob, YL, has_b);
+        emit<EPI, OutT>(dst, d, v0, H, o, YL);
+        emit<EPI, OutT>(dst, d + Ns, v1, H, o, YL);
+        emit<EPI, OutT>(dst, d + 2 * Ns, v2, H, o, YL);
+        emit<EPI, OutT>(dst, d + 3 * Ns, v3, H, o, YL);
     }
 }
 
 // Final radix-2 pass (Ns = N/2).
 template <int EPI, class OutT>
 __device__ __forceinline__ void pass2_last(const float2* src, float2* dst, int N,
-                                           const float2* tw, const float* H, OutT* oa,
-                                           OutT* ob, int YL, bool has_b) {
+                                           const float2* tw, const float* H,
+                                           const RowOut<OutT>& o, int YL) {
     const int hN = N >> 1;
     for (int j = threadIdx.x; j < hN; j += blockDim.x) {
         const float2 v0 = src[j];
         const float2 v1 = cmul(src[j + hN], tw[j]);
-        emit<EPI, OutT>(dst, j, cadd(v0, v1), H, oa, ob, YL, has_b);
-        emit<EPI, OutT>(dst, j + hN, csub(v0, v1), H, oa, ob, YL, has_b);
+        emit<EPI, OutT>(dst, j, cadd(v0, v1), H, o, YL);
+        emit<EPI, OutT>(dst, j + hN, csub(v0, v1), H, o, YL);
     }
 }
 
@@ -125,22 +165,21 @@ __device__ __forceinline__ void pass2_last(const float2* src, float2* dst, int N
 template <int EPI_LAST, class OutT>
 __device__ __forceinline__ float2* passes_after_first(float2* cur, float2* nxt, int N,
                                                       const float2* tw, const float* H,
-                                                      OutT* oa, OutT* ob, int YL,
-                                                      bool has_b) {
+                                                      const RowOut<OutT>& o, int YL) {
     int Ns = 4;
     while (Ns * 4 <= N) {
         const bool last = (Ns * 4 == N);
         __syncthreads();
         if (last)
-            pass4<EPI_LAST, OutT>(cur, nxt, N, Ns, tw, H, oa, ob, YL, has_b);
+            pass4<EPI_LAST, OutT>(cur, nxt, N, Ns, tw, H, o, YL);
         else
-            pass4<0, OutT>(cur, nxt, N, Ns, tw, H, oa, ob, YL, has_b);
+            pass4<0, OutT>(cur, nxt, N, Ns, tw, H, o, YL);
         float2* t = cur; cur = nxt; nxt = t;
         Ns *= 4;
     }
     if (Ns < N) {   // N = 2 * 4^k
         __syncthreads();
-        pass2_last<EPI_LAST, OutT>(cur, nxt, N, tw, H, oa, ob, YL, has_b);
+        pass2_last<EPI_LAST, OutT>(cur, nxt, N, tw, H, o, YL);
         float2* t = cur; cur = nxt; nxt = t;
     }
     return cur;
@@ -154,7 +193,7 @@ __device__ __forceinline__ float preweight(int j, float b, float PLC, float dYL,
 
 template <class OutT>
 __global__ __launch_bounds__(256) void filter_rows(
-    OutT* out, const float* in, const float2* __restrict__ twg,
+    OutT* out, int layout, const float* in, const float2* __restrict__ twg,
     const float* __restrict__ Hg, int YL, int ZL, long long nrows, int N, float PLC,
     float ZLC, float dYL, float dZL, float SO) {
     extern __shared__ float2 smem[];
@@ -197,9 +236,8 @@ __global__ __launch_bounds__(256) void filter_rows(
             bufA[4 * j + 2] = csub(v0, v1);
             bufA[4 * j + 3] = csub(v0, nv1);
         }
-        OutT* oa = out + ra * YL;
-        OutT* ob = oa + YL;
-        float2* spec = passes_after_first<1, OutT>(bufA, bufB, N, tw, H, oa, ob, YL, has_b);
+        const RowOut<OutT> o = row_out(out, layout, ra, has_b, YL, ZL);
+        float2* spec = passes_after_first<1, OutT>(bufA, bufB, N, tw, H, o, YL);
         float2* other = (spec == bufA) ? bufB : bufA;
         // Inverse: forward passes over conj(H X / N); first pass from LDS.
         __syncthreads();
@@ -211,7 +249,7 @@ __global__ __launch_bounds__(256) void filter_rows(
             other[4 * j + 2] = v2;
             other[4 * j + 3] = v3;
         }
-        passes_after_first<2, OutT>(other, spec, N, tw, H, oa, ob, YL, has_b);
+        passes_after_first<2, OutT>(other, spec, N, tw, H, o, YL);
     }
 }
 
@@ -302,7 +340,7 @@ template <int N> constexpr int tpt() { return N >= 2048 ? N / 16 : 64; }
 
 template <int N, int R, int Ns, int SRC, int DST, class OutT>
 __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict__ twg,
-                                          const RowCtx& c, OutT* oa, OutT* ob) {
+                                          const RowCtx& c, const RowOut<OutT>& o) {
     constexpr int T = tpt<N>();
     constexpr int NB = N / R, PER = NB / T;
     static_assert(PER >= 1 && PER * T == NB, "radix schedule does not fit the thread count");
@@ -348,8 +386,7 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
                 const float hs = c.H[k];
                 lds[k] = make_float2(hs * v[b][r].x, -hs * v[b][r].y);
             } else if (k < c.YL) {
-                oa[k] = to_out<OutT>(v[b][r].x);
-                if (c.has_b) ob[k] = to_out<OutT>(-v[b][r].y);
+                o.put(k, v[b][r].x, -v[b][r].y);
             }
         }
     }
@@ -360,34 +397,35 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
 // (MODE 1: LDS -> global rows) transform of one pair of rows.
 template <int N, int MODE, class OutT>
 __device__ __forceinline__ void wave_fft(float2* lds, const float2* __restrict__ twg,
-                                         const RowCtx& c, OutT* oa, OutT* ob) {
+                                         const RowCtx& c, const RowOut<OutT>& o) {
     constexpr int S0 = (MODE == 0) ? SRC_GLOBAL : SRC_LDS;
     constexpr int DL = (MODE == 0) ? DST_LDS_SPEC : DST_GLOBAL;
     if constexpr (N == 2048) {
-        wave_pass<N, 16, 1, S0, DST_LDS, OutT>(lds, twg, c, oa, ob);
-        wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, oa, ob);
-        wave_pass<N, 8, 256, SRC_LDS, DL, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 16, 1, S0, DST_LDS, OutT>(lds, twg, c, o);
+        wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, o);
+        wave_pass<N, 8, 256, SRC_LDS, DL, OutT>(lds, twg, c, o);
     } else if constexpr (N == 1024) {
-        wave_pass<N, 16, 1, S0, DST_LDS, OutT>(lds, twg, c, oa, ob);
-        wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, oa, ob);
-        wave_pass<N, 4, 256, SRC_LDS, DL, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 16, 1, S0, DST_LDS, OutT>(lds, twg, c, o);
+        wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, o);
+        wave_pass<N, 4, 256, SRC_LDS, DL, OutT>(lds, twg, c, o);
     } else if constexpr (N == 512) {
-        wave_pass<N, 8, 1, S0, DST_LDS, OutT>(lds, twg, c, oa, ob);
-        wave_pass<N, 8, 8, SRC_LDS, DST_LDS, OutT>(lds, twg, c, oa, ob);
-        wave_pass<N, 8, 64, SRC_LDS, DL, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 8, 1, S0, DST_LDS, OutT>(lds, twg, c, o);
+        wave_pass<N, 8, 8, SRC_LDS, DST_LDS, OutT>(lds, twg, c, o);
+        wave_pass<N, 8, 64, SRC_LDS, DL, OutT>(lds, twg, c, o);
     } else {
         static_assert(N == 256, "wave FFT supports N = 256 .. 2048");
-        wave_pass<N, 4, 1, S0, DST_LDS, OutT>(lds, twg, c, oa, ob);
-        wave_pass<N, 4, 4, SRC_LDS, DST_LDS, OutT>(lds, twg, c, oa, ob);
-        wave_pass<N, 4, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, oa, ob);
-        wave_pass<N, 4, 64, SRC_LDS, DL, OutT>(lds, twg, c, oa, ob);
+        wave_pass<N, 4, 1, S0, DST_LDS, OutT>(lds, twg, c, o);
+        wave_pass<N, 4, 4, SRC_LDS, DST_LDS, OutT>(lds, twg, c, o);
+        wave_pass<N, 4, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, o);
+        wave_pass<N, 4, 64, SRC_LDS, DL, OutT>(lds, twg, c, o);
     }
 }
 
 template <int N, class OutT>
 __global__ __launch_bounds__(tpt<N>()) void filter_rows_wave(
-    OutT* out, const float* in, const float2* __restrict__ twg, const float* __restrict__ Hg,
-    int YL, int ZL, long long nrows, float PLC, float ZLC, float dYL, float dZL, float SO) {
+    OutT* out, int layout, const float* in, const float2* __restrict__ twg,
+    const float* __restrict__ Hg, int YL, int ZL, long long nrows, float PLC, float ZLC,
+    float dYL, float dZL, float SO) {
     __shared__ float2 lds[N];
     const long long npairs = (nrows + 1) >> 1;
     for (long long pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
@@ -401,10 +439,9 @@ __global__ __launch_bounds__(tpt<N>()) void filter_rows_wave(
         c.ba = ((float)ka - ZLC) * dZL;
         c.bb = ((float)kb - ZLC) * dZL;
         c.YL = YL; c.PLC = PLC; c.dYL = dYL; c.SO = SO; c.H = Hg;
-        OutT* oa = out + ra * YL;
-        OutT* ob = oa + YL;
-        wave_fft<N, 0, OutT>(lds, twg, c, oa, ob);
-        wave_fft<N, 1, OutT>(lds, twg, c, oa, ob);
+        const RowOut<OutT> o = row_out(out, layout, ra, c.has_b, YL, ZL);
+        wave_fft<N, 0, OutT>(lds, twg, c, o);
+        wave_fft<N, 1, OutT>(lds, twg, c, o);
     }
 }
 
@@ -466,15 +503,21 @@ const FilterTables& filter_tables(int device, int YL, float dYL_f) {
 
 }  // namespace
 
-void launch_filter(void* out, int out_dtype, const float* in, int YL, int ZL, int ViewN,
-                   float PLC, float ZLC, float dYL, float dZL, float SO, hipStream_t s) {
+void launch_filter(void* out, int out_dtype, int out_layout, const float* in, int YL, int ZL,
+                   int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
+                   hipStream_t s) {
     ARECON_REQUIRE(YL > 0 && ZL > 0 && ViewN > 0, "filter: YL, ZL and ViewN must be positive");
     ARECON_REQUIRE(YL <= 2048, "filter: YL above 2048 is not supported");
     ARECON_REQUIRE(out != nullptr && in != nullptr, "filter: null buffer");
     ARECON_REQUIRE(out_dtype == ARECON_F32 || out_dtype == ARECON_F16 || out_dtype == ARECON_BF16,
                    "filter: unknown output dtype");
-    ARECON_REQUIRE(out_dtype == ARECON_F32 || out != (const void*)in,
-                   "filter: a 16-bit output cannot alias the FP32 input");
+    ARECON_REQUIRE(out_layout == ARECON_FP_ROWS || out_layout == ARECON_FP_ROWPAIRS,
+                   "filter: unknown output layout");
+    ARECON_REQUIRE((out_dtype == ARECON_F32 && out_layout == ARECON_FP_ROWS) ||
+                       out != (const void*)in,
+                   "filter: only the FP32 rows layout can be written in place");
+    ARECON_REQUIRE(out_layout == ARECON_FP_ROWS || ZL >= 2,
+                   "filter: the row-pair layout needs ZL >= 2");
     ARECON_REQUIRE(dYL != 0.f && SO > 0.f, "filter: dYL must be nonzero and SO positive");
     int dev = 0;
     ARECON_HIP(hipGetDevice(&dev));
@@ -504,13 +547,13 @@ void launch_filter(void* out, int out_dtype, const float* in, int YL, int ZL, in
             using OT = std::remove_pointer_t<decltype(outp)>;
             switch (N) {
             case 256: hipLaunchKernelGGL((filter_rows_wave<256, OT>), dim3((unsigned)wgrid), dim3(tpt<256>()), 0, s,
-                                         outp, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+                                         outp, out_layout, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
             case 512: hipLaunchKernelGGL((filter_rows_wave<512, OT>), dim3((unsigned)wgrid), dim3(tpt<512>()), 0, s,
-                                         outp, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+                                         outp, out_layout, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
             case 1024: hipLaunchKernelGGL((filter_rows_wave<1024, OT>), dim3((unsigned)wgrid), dim3(tpt<1024>()), 0, s,
-                                          outp, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+                                          outp, out_layout, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
             default: hipLaunchKernelGGL((filter_rows_wave<2048, OT>), dim3((unsigned)wgrid), dim3(tpt<2048>()), 0, s,
-                                        outp, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+                                        outp, out_layout, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
             }
         };
         switch (out_dtype) {
@@ -524,18 +567,18 @@ void launch_filter(void* out, int out_dtype, const float* in, int YL, int ZL, in
     switch (out_dtype) {
     case ARECON_F32:
         hipLaunchKernelGGL(filter_rows<float>, dim3((unsigned)grid), dim3(256), lds, s,
-                           static_cast<float*>(out), in, t.tw, t.H, YL, ZL, nrows, N, PLC,
+                           static_cast<float*>(out), out_layout, in, t.tw, t.H, YL, ZL, nrows, N, PLC,
                            ZLC, dYL, dZL, SO);
         break;
     case ARECON_F16:
         hipLaunchKernelGGL(filter_rows<__half>, dim3((unsigned)grid), dim3(256), lds, s,
-                           static_cast<__half*>(out), in, t.tw, t.H, YL, ZL, nrows, N, PLC,
+                           static_cast<__half*>(out), out_layout, in, t.tw, t.H, YL, ZL, nrows, N, PLC,
                            ZLC, dYL, dZL, SO);
         break;
     default:
         hipLaunchKernelGGL(filter_rows<__hip_bfloat16>, dim3((unsigned)grid), dim3(256), lds,
-                           s, static_cast<__hip_bfloat16*>(out), in, t.tw, t.H, YL, ZL,
-                           nrows, N, PLC, ZLC, dYL, dZL, SO);
+                           s, static_cast<__hip_bfloat16*>(out), out_layout, in, t.tw, t.H,
+                           YL, ZL, nrows, N, PLC, ZLC, dYL, dZL, SO);
         break;
     }
     ARECON_HIP(hipGetLastError());

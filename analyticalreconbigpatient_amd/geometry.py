@@ -264,7 +264,7 @@ class BPParams:
         return self.SO + self.DO
 
     def to_geom(self, *, dz: float | None = None, nz: int = 0, order: int = 0,
-                variant: int = 0) -> BPGeom:
+                variant: int = 0, group: int = 0) -> BPGeom:
         """FP64 ``arecon_bp_geom``; ``dz`` overrides the 2 ObjR / ZN coupling."""
         return BPGeom(
             N_2pi=self.N_2pi, XN=self.XN, YN=self.YN, ZN=self.ZN, YL=self.YL, ZL=self.ZL,
@@ -272,7 +272,7 @@ class BPParams:
             dz=self.dz if dz is None else dz, YLC=self.YLC, ZLC=self.ZLC, dYL=self.dYL,
             dZL=self.dZL, h=self.h, BetaS=self.BetaS, DeltaFai=self.DeltaFai,
             N_pi=self.N_pi, HSCoef=self.HSCoef, SO=self.SO, SD=self.SD, k1=self.k1,
-            nz=nz, order=order, variant=variant)
+            nz=nz, order=order, variant=variant, group=group)
 
 
 def bp_params(g: ScanGeom) -> BPParams:

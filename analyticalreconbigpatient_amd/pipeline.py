@@ -12,6 +12,7 @@ libarecon.so.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -23,6 +24,13 @@ from .geometry import (BPParams, ReconSetup, bp_params, bp_scan_geom, filter_arg
 
 STORAGE = {"f32": (_lib.ARECON_F32, torch.float32), "f16": (_lib.ARECON_F16, torch.float16),
            "bf16": (_lib.ARECON_BF16, torch.bfloat16)}
+# Filtered-sinogram layouts: the reference's rows [ViewN][ZL][YL], or row pairs
+# [ViewN][ZL-1][YL][2] (rows r and r+1 of each channel side by side).
+LAYOUT = {"rows": _lib.ARECON_FP_ROWS, "rowpairs": _lib.ARECON_FP_ROWPAIRS}
+
+
+def filtered_shape(ViewN: int, ZL: int, YL: int, layout: str) -> tuple:
+    return (ViewN, ZL, YL) if layout == "rows" else (ViewN, ZL - 1, YL, 2)
 
 
 def _stream_ptr(stream: torch.cuda.Stream | None) -> C.c_void_p:
@@ -43,9 +51,10 @@ class BPPlan:
     """Owner of an ``arecon_bp_plan`` (tables built in FP64, stored FP32)."""
 
     def __init__(self, params: BPParams, *, dz: float | None = None, nz: int = 0,
-                 order: int = 0, variant: int = 0, w: np.ndarray | None = None):
+                 order: int = 0, variant: int = 0, group: int = 0,
+                 w: np.ndarray | None = None):
         self.params = params
-        self.geom = params.to_geom(dz=dz, nz=nz, order=order, variant=variant)
+        self.geom = params.to_geom(dz=dz, nz=nz, order=order, variant=variant, group=group)
         self.w = fdk_weights(params.N_2pi, params.HSCoef, params.delta) if w is None else \
             np.ascontiguousarray(w, dtype=np.float64)
         h = C.c_void_p()
@@ -54,17 +63,19 @@ class BPPlan:
         self._h = h
         self.nz = _lib.lib().arecon_bp_plan_nz(h)
 
-    def execute(self, img: torch.Tensor, fp: torch.Tensor, stream=None) -> None:
+    def execute(self, img: torch.Tensor, fp: torch.Tensor, stream=None,
+                layout: str = "rows") -> None:
         g = self.geom
         dtype = {torch.float32: _lib.ARECON_F32, torch.float16: _lib.ARECON_F16,
                  torch.bfloat16: _lib.ARECON_BF16}[fp.dtype]
         assert img.is_cuda and fp.is_cuda and img.dtype == torch.float32
         assert img.is_contiguous() and fp.is_contiguous()
         assert img.numel() == g.XN * g.YN * g.ZN, "volume size does not match the plan"
-        assert fp.numel() >= g.PN * g.ZL * g.YL, "fp holds fewer than PN views"
+        per_view = math.prod(filtered_shape(1, g.ZL, g.YL, layout))
+        assert fp.numel() >= g.PN * per_view, "fp holds fewer than PN views"
         _lib.check(_lib.lib().arecon_bp_plan_execute(self._h, C.c_void_p(img.data_ptr()),
                                                      C.c_void_p(fp.data_ptr()), dtype,
-                                                     _stream_ptr(stream)),
+                                                     LAYOUT[layout], _stream_ptr(stream)),
                    "arecon_bp_plan_execute")
 
     def close(self) -> None:
@@ -89,16 +100,18 @@ def rebin_dev(out: torch.Tensor, raw: torch.Tensor, setup_geom, layout: int, str
         "arecon_rebin_dev")
 
 
-def filter_dev(out: torch.Tensor, inp: torch.Tensor, setup: ReconSetup, stream=None):
+def filter_dev(out: torch.Tensor, inp: torch.Tensor, setup: ReconSetup, stream=None,
+               layout: str = "rows"):
     fa = filter_args(setup)
     ViewN, ZL, YL = inp.shape
     dtype = {torch.float32: _lib.ARECON_F32, torch.float16: _lib.ARECON_F16,
              torch.bfloat16: _lib.ARECON_BF16}[out.dtype]
     assert inp.dtype == torch.float32 and inp.is_contiguous() and out.is_contiguous()
-    assert out.numel() == inp.numel()
+    assert out.numel() == math.prod(filtered_shape(ViewN, ZL, YL, layout))
     _lib.check(_lib.lib().arecon_filter_dev(
-        C.c_void_p(out.data_ptr()), dtype, C.c_void_p(inp.data_ptr()), YL, ZL, ViewN, fa.PLC0,
-        fa.ZLC0, fa.dYL, fa.dZL, fa.SO, _stream_ptr(stream)), "arecon_filter_dev")
+        C.c_void_p(out.data_ptr()), dtype, LAYOUT[layout], C.c_void_p(inp.data_ptr()), YL, ZL,
+        ViewN, fa.PLC0, fa.ZLC0, fa.dYL, fa.dZL, fa.SO, _stream_ptr(stream)),
+        "arecon_filter_dev")
 
 
 @dataclass
@@ -118,12 +131,15 @@ class HelicalFDK:
     ``first_view``/``n_views`` select the backprojected views of the rebinned
     sinogram (the segment trimming of src/analyticalReconBigPatient.m:80-93);
     ``bp_geom`` is the ScanGeom after trimming (BetaS shifted accordingly).
-    ``storage`` is the element type of the filtered sinogram.
+    ``storage`` is the element type of the filtered sinogram and ``layout`` its
+    layout ("rows", or "rowpairs" for the z-column backprojector, variant 2).
     """
 
-    def __init__(self, setup: ReconSetup, *, storage: str = "f32", first_view: int = 0,
+    def __init__(self, setup: ReconSetup, *, storage: str = "f32", layout: str = "rows",
+                 first_view: int = 0,
                  n_views: int | None = None, bp_geom=None, dz: float | None = None,
-                 nz: int = 0, order: int = 0, variant: int = 0, device: str = "cuda"):
+                 nz: int = 0, order: int = 0, variant: int = 0, group: int = 0,
+                 device: str = "cuda"):
         self.setup = setup
         self.ra = rebin_args(setup.geom)
         self.ViewN, self.YL, self.ZL = self.ra.ViewN, self.ra.YL, self.ra.ZL
@@ -132,14 +148,18 @@ class HelicalFDK:
         g = bp_scan_geom(setup, geom=bp_geom, n_views=self.n_views)
         self.params = bp_params(g)
         self.dz = dz
-        self.plan = BPPlan(self.params, dz=dz, nz=nz, order=order, variant=variant)
+        self.plan = BPPlan(self.params, dz=dz, nz=nz, order=order, variant=variant, group=group)
         self.storage = storage
+        self.layout = layout
         code, tdtype = STORAGE[storage]
         dev = torch.device(device)
         shape = (self.ViewN, self.ZL, self.YL)
         self.rebinned = torch.empty(shape, dtype=torch.float32, device=dev)
-        self.filtered = self.rebinned if storage == "f32" else torch.empty(shape, dtype=tdtype,
-                                                                           device=dev)
+        if storage == "f32" and layout == "rows":
+            self.filtered = self.rebinned          # in place
+        else:
+            self.filtered = torch.empty(filtered_shape(self.ViewN, self.ZL, self.YL, layout),
+                                        dtype=tdtype, device=dev)
         p = self.params
         self.volume = torch.empty((p.YN, p.XN, p.ZN), dtype=torch.float32, device=dev)
         self.updates = voxel_updates(p, dz)
@@ -150,12 +170,12 @@ class HelicalFDK:
         return self.rebinned
 
     def filter(self, stream=None) -> torch.Tensor:
-        filter_dev(self.filtered, self.rebinned, self.setup, stream)
+        filter_dev(self.filtered, self.rebinned, self.setup, stream, self.layout)
         return self.filtered
 
     def backproject(self, stream=None) -> torch.Tensor:
         fp = self.filtered[self.first_view:self.first_view + self.n_views]
-        self.plan.execute(self.volume, fp, stream)
+        self.plan.execute(self.volume, fp, stream, self.layout)
         return self.volume
 
     def run(self, raw: torch.Tensor, stream=None) -> torch.Tensor:

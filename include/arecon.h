@@ -45,6 +45,12 @@ extern "C" {
 #define ARECON_LAYOUT_VYZ 0   /* [ViewN][YL][ZL], the reference's rebin layout */
 #define ARECON_LAYOUT_VZY 1   /* [ViewN][ZL][YL], the filter's input layout */
 
+/* Layouts of the filtered sinogram (filter output, backprojection input). */
+#define ARECON_FP_ROWS 0      /* [ViewN][ZL][YL], the reference's layout */
+#define ARECON_FP_ROWPAIRS 1  /* [ViewN][ZL-1][YL][2]: element (v, r, c) holds rows r
+                                 and r+1 of channel c, so one 4-sample load fetches
+                                 the 2x2 bilinear neighbourhood of the backprojector */
+
 /* ========================================================================== */
 /* 1. Reference drop-ins                                                      */
 /* ========================================================================== */
@@ -134,10 +140,11 @@ int arecon_rebin_dev(float* d_out, const float* d_in, int YL, int ZL, int ViewN,
                      float YLC, float dYA, float DeltaTheta, float PLC, float DeltaT,
                      float DeltaFai, float SO, int out_layout, void* stream);
 
-/* Preweight + ramp filter on device buffers, layout [ViewN][ZL][YL].
- * d_out may equal d_in when out_dtype is ARECON_F32. */
-int arecon_filter_dev(void* d_out, int out_dtype, const float* d_in, int YL, int ZL,
-                      int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
+/* Preweight + ramp filter on device buffers; input layout [ViewN][ZL][YL],
+ * output layout out_layout (ARECON_FP_ROWS or ARECON_FP_ROWPAIRS).  d_out may
+ * equal d_in only for ARECON_F32 rows. */
+int arecon_filter_dev(void* d_out, int out_dtype, int out_layout, const float* d_in, int YL,
+                      int ZL, int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
                       void* stream);
 
 /* Backprojection geometry: the derived quantities the reference passes to its
@@ -152,9 +159,13 @@ typedef struct arecon_bp_geom {
     double BetaS, DeltaFai;     /* first view angle, view step */
     double N_pi, HSCoef;        /* N_2pi / 2, half-scan coefficient */
     double SO, SD, k1;
-    int nz;                     /* slices per thread: 0 = automatic, else 1/2/4/8/16 */
+    int nz;                     /* slices per thread: 0 = automatic; variants 0/1:
+                                   1/2/4/8/16; variant 2: 8/16/32 */
     int order;                  /* workgroup order: 0 = slice-major, 1 = column-major */
-    int variant;                /* 0 = slice-table kernel (default), 1 = per-voxel weight */
+    int variant;                /* 0 = view-table kernel (default), 1 = per-voxel weight,
+                                   2 = z-column kernel (slice blocks with view masks) */
+    int group;                  /* z-column kernel: slices per branch-free group,
+                                   0 = automatic, else 1/2/4 */
 } arecon_bp_geom;
 
 typedef struct arecon_bp_plan arecon_bp_plan;
@@ -166,13 +177,14 @@ int arecon_bp_plan_create(const arecon_bp_geom* g, const double* w,
                           arecon_bp_plan** out);
 int arecon_bp_plan_destroy(arecon_bp_plan* plan);
 
-/* Slices per thread the plan chose (1, 2, 4, 8 or 16); 0 for a null plan. */
+/* Slices per thread the plan chose (1 .. 32); 0 for a null plan. */
 int arecon_bp_plan_nz(const arecon_bp_plan* plan);
 
-/* Backprojects d_fp ([PN][ZL][YL], element type fp_dtype) into d_img
- * ([YN][XN][ZN], FP32; every voxel is written). */
+/* Backprojects d_fp (PN views in layout fp_layout, element type fp_dtype) into
+ * d_img ([YN][XN][ZN], FP32; every voxel is written).  ARECON_FP_ROWPAIRS needs
+ * the z-column kernel (variant 2). */
 int arecon_bp_plan_execute(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
-                           int fp_dtype, void* stream);
+                           int fp_dtype, int fp_layout, void* stream);
 
 /* FP64 backprojection on device buffers (same math in double throughout).
  * Synchronises `stream` before returning (w is a host buffer). */

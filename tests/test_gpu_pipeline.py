@@ -43,10 +43,12 @@ def small_workload():
 
 
 @pytest.mark.parametrize("storage,tol", [("f32", 1e-4), ("f16", 2e-3), ("bf16", 1e-2)])
-def test_pipeline_matches_oracle_chain(gpu, nthreads, storage, tol):
+@pytest.mark.parametrize("variant,layout", [(0, "rows"), (2, "rows"), (2, "rowpairs")])
+def test_pipeline_matches_oracle_chain(gpu, nthreads, storage, tol, variant, layout):
+    print(f"pipeline storage={storage} variant={variant} layout={layout}", flush=True)
     wl, s, dz, E = small_workload()
     raw = phantom.project(E, s, wl.hdr)
-    fdk = HelicalFDK(s, dz=dz, storage=storage)
+    fdk = HelicalFDK(s, dz=dz, storage=storage, variant=variant, layout=layout)
     vol = fdk.run(raw)
     torch.cuda.synchronize()
     want = oracle_chain(raw.cpu().numpy(), s, fdk.params, dz=dz, nthreads=nthreads)
@@ -56,13 +58,15 @@ def test_pipeline_matches_oracle_chain(gpu, nthreads, storage, tol):
     assert err < tol
 
 
-def test_pipeline_trimmed_segment(gpu, nthreads):
+@pytest.mark.parametrize("variant,layout", [(0, "rows"), (2, "rowpairs")])
+def test_pipeline_trimmed_segment(gpu, nthreads, variant, layout):
     """Backprojecting a trimmed view range (segment 2 of 3) by pointer offset."""
     wl, s, dz, E = small_workload()
     raw = phantom.project(E, s, wl.hdr)
     ov = 7
     first, n, g = G.trim_segment(s.geom, 2, 3, ov, wl.hdr.NumberOfViewPerRotation)
-    fdk = HelicalFDK(s, dz=dz, first_view=first, n_views=n, bp_geom=g)
+    fdk = HelicalFDK(s, dz=dz, first_view=first, n_views=n, bp_geom=g, variant=variant,
+                     layout=layout)
     vol = fdk.run(raw)
     torch.cuda.synchronize()
     want = oracle_chain(raw.cpu().numpy(), s, fdk.params, first=first, n=n, dz=dz,

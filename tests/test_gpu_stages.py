@@ -76,14 +76,18 @@ def test_rebin_drop_in_gateway(gpu):
 
 
 # --------------------------------------------------------------------------- filter
-def run_filter(P, PLC0, ZLC0, dYL, dZL, dtype=_lib.ARECON_F32, inplace=False):
+def run_filter(P, PLC0, ZLC0, dYL, dZL, dtype=_lib.ARECON_F32, inplace=False,
+               layout=_lib.ARECON_FP_ROWS):
+    """Filtered rows [ViewN][ZL][YL]; the row-pair layout is returned as stored,
+    [ViewN][ZL-1][YL][2]."""
     ViewN, ZL, YL = P.shape
     din = torch.from_numpy(np.ascontiguousarray(P, dtype=np.float32)).cuda()
     tdt = {_lib.ARECON_F32: torch.float32, _lib.ARECON_F16: torch.float16,
            _lib.ARECON_BF16: torch.bfloat16}[dtype]
-    dout = din if inplace else torch.empty(din.shape, dtype=tdt, device=din.device)
+    shape = din.shape if layout == _lib.ARECON_FP_ROWS else (ViewN, ZL - 1, YL, 2)
+    dout = din if inplace else torch.full(shape, float("nan"), dtype=tdt, device=din.device)
     _lib.check(_lib.lib().arecon_filter_dev(
-        C.c_void_p(dout.data_ptr()), dtype, C.c_void_p(din.data_ptr()), YL, ZL, ViewN,
+        C.c_void_p(dout.data_ptr()), dtype, layout, C.c_void_p(din.data_ptr()), YL, ZL, ViewN,
         np.float32(PLC0), np.float32(ZLC0), np.float32(dYL), np.float32(dZL), np.float32(SO),
         stream()), "filter")
     return dout.float().cpu().numpy()
@@ -98,6 +102,20 @@ def test_filter_matches_fp64_oracle(gpu, ViewN, ZL, YL):
     want = oracle.filter_f64(P.astype(np.float64), *[float(np.float32(v)) for v in args], SO)
     got = run_filter(P, *args)
     assert rel_l2(got, want) < 1e-5
+
+
+@pytest.mark.parametrize("ViewN,ZL,YL", [(9, 64, 888), (7, 3, 20), (5, 5, 130), (4, 2, 2048)])
+@pytest.mark.parametrize("dtype", [_lib.ARECON_F32, _lib.ARECON_F16])
+def test_filter_row_pair_layout(gpu, ViewN, ZL, YL, dtype):
+    """Row-pair output holds rows r and r+1 of every channel, bit-identical to the
+    rows layout of the same filter (odd ZL puts row pairs across transform pairs)."""
+    print(f"filter row pairs ViewN={ViewN} ZL={ZL} YL={YL} dtype={dtype}", flush=True)
+    P = np.random.default_rng(YL + ZL).normal(size=(ViewN, ZL, YL)).astype(np.float32)
+    args = ((YL - 1) / 2.0, (ZL - 1) / 2.0, 0.9493, 1.9231)
+    rows = run_filter(P, *args, dtype=dtype)
+    pairs = run_filter(P, *args, dtype=dtype, layout=_lib.ARECON_FP_ROWPAIRS)
+    assert np.array_equal(pairs[..., 0], rows[:, :-1, :])
+    assert np.array_equal(pairs[..., 1], rows[:, 1:, :])
 
 
 def test_filter_in_place_and_16_bit_outputs(gpu):
@@ -126,24 +144,30 @@ class P_:
         self.__dict__.update(g)
         self.YL, self.ZL, self.PN, self.delta = YL, ZL, PN, delta
 
-    def to_geom(self, dz=None, nz=0, order=0, variant=0):
+    def to_geom(self, dz=None, nz=0, order=0, variant=0, group=0):
         return _lib.BPGeom(N_2pi=self.N_2pi, XN=self.XN, YN=self.YN, ZN=self.ZN, YL=self.YL,
                            ZL=self.ZL, PN=self.PN, XNC=self.XNC, YNC=self.YNC, ZNC=self.ZNC,
                            dx=self.dx, dy=self.dy, dz=self.dz if dz is None else dz,
                            YLC=self.YLC, ZLC=self.ZLC, dYL=self.dYL, dZL=self.dZL, h=self.h,
                            BetaS=self.BetaS, DeltaFai=self.DeltaFai, N_pi=self.N_pi,
                            HSCoef=self.HSCoef, SO=self.SO, SD=self.SD, k1=self.k1, nz=nz,
-                           order=order, variant=variant)
+                           order=order, variant=variant, group=group)
 
 
-def gpu_bp(fp, g, nz=0, order=0, storage=torch.float32, variant=0):
+def row_pairs(fp):
+    """[PN][ZL][YL] -> the row-pair layout [PN][ZL-1][YL][2]."""
+    return np.stack([fp[:, :-1, :], fp[:, 1:, :]], axis=-1)
+
+
+def gpu_bp(fp, g, nz=0, order=0, storage=torch.float32, variant=0, group=0, layout="rows"):
     PN, ZL, YL = fp.shape
     p = P_(g, YL, ZL, PN)
     w = oracle.fdk_weights(g["N_2pi"], g["HSCoef"], 8)
-    plan = BPPlan(p, nz=nz, order=order, w=w, variant=variant)
-    dfp = torch.from_numpy(np.ascontiguousarray(fp, dtype=np.float32)).cuda().to(storage)
+    plan = BPPlan(p, nz=nz, order=order, w=w, variant=variant, group=group)
+    host = fp if layout == "rows" else row_pairs(fp)
+    dfp = torch.from_numpy(np.ascontiguousarray(host, dtype=np.float32)).cuda().to(storage)
     img = torch.full((g["YN"], g["XN"], g["ZN"]), float("nan"), device="cuda")
-    plan.execute(img, dfp)
+    plan.execute(img, dfp, layout=layout)
     torch.cuda.synchronize()
     out = img.cpu().numpy()
     plan.close()
@@ -179,8 +203,28 @@ def test_bp_matches_fp64_oracle(gpu, nz, order, variant):
     assert rel_l2(got, want) < 1e-4
 
 
+@pytest.mark.parametrize("nz", [8, 16, 32])
+@pytest.mark.parametrize("group", [1, 2, 4])
+@pytest.mark.parametrize("ZNC", [-15.37, -68.41])
+@pytest.mark.parametrize("layout", ["rows", "rowpairs"])
+def test_bp_z_column_matches_fp64_oracle(gpu, nz, group, ZNC, layout):
+    """Z-column kernel: slice blocks whose windows overlap heavily (thin slices),
+    a ragged last block, windows cut by the first (ZNC=-15.37) or the last
+    (ZNC=-68.41) view of the helix."""
+    print(f"bp z-column nz={nz} group={group} ZNC={ZNC} layout={layout}", flush=True)
+    g = realistic(ZN=37, dz=0.45, ZNC=ZNC)
+    PN, ZL, YL = 200, 16, 128
+    fp = np.random.default_rng(nz + group).normal(size=(PN, ZL, YL)).astype(np.float32)
+    got, w = gpu_bp(fp, g, nz=nz, variant=2, group=group, layout=layout)
+    want = oracle.fdk_core(fp.astype(np.float64), w,
+                           **{k: v for k, v in g.items() if k not in ("YL", "ZL")})
+    assert not np.isnan(got).any()
+    assert np.abs(want).max() > 0
+    assert rel_l2(got, want) < 1e-4
+
+
 @pytest.mark.parametrize("k1", [0.0, 2.85])
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_bp_full_window_random(gpu, k1, variant):
     g = bp_geom_dict(k1=k1, XN=17, YN=21, ZN=9, XNC=8.2, YNC=10.1, dz=0.113, ZNC=-14.77)
     fp = np.random.default_rng(3).normal(size=(192, 16, 64)).astype(np.float32)
@@ -191,7 +235,7 @@ def test_bp_full_window_random(gpu, k1, variant):
 
 
 @pytest.mark.parametrize("k1", [0.0, 5.0])
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_bp_constant_projection_gives_pi_c(gpu, k1, variant):
     g = bp_geom_dict(h=1.0, k1=k1)
     c = 1.75
@@ -200,13 +244,14 @@ def test_bp_constant_projection_gives_pi_c(gpu, k1, variant):
 
 
 @pytest.mark.parametrize("storage,tol", [(torch.float16, 1e-4), (torch.bfloat16, 1e-4)])
-def test_bp_16_bit_storage(gpu, storage, tol):
+@pytest.mark.parametrize("variant,layout", [(0, "rows"), (2, "rows"), (2, "rowpairs")])
+def test_bp_16_bit_storage(gpu, storage, tol, variant, layout):
     """The kernel reads 16-bit samples exactly: compare with the oracle fed the
     same rounded samples."""
     g = realistic()
     fp = np.random.default_rng(9).normal(size=(200, 16, 128)).astype(np.float32)
     rounded = torch.from_numpy(fp).to(storage).float().numpy().astype(np.float64)
-    got, w = gpu_bp(fp, g, storage=storage)
+    got, w = gpu_bp(fp, g, storage=storage, variant=variant, layout=layout)
     want = oracle.fdk_core(rounded, w, **{k: v for k, v in g.items() if k not in ("YL", "ZL")})
     assert rel_l2(got, want) < tol
 
@@ -233,6 +278,13 @@ def test_bp_drop_in_float_and_double(gpu):
                                                *args)
     _lib.check(None, "float drop-in")
     assert rel_l2(img32.reshape(want.shape), want) < 1e-4
+
+
+def test_bp_row_pairs_need_the_z_column_kernel(gpu):
+    g = realistic()
+    fp = np.zeros((200, 16, 128), dtype=np.float32)
+    with pytest.raises(_lib.AreconError, match="z-column"):
+        gpu_bp(fp, g, variant=0, layout="rowpairs")
 
 
 def test_bp_rejects_oversized_window(gpu):
