@@ -363,85 +363,165 @@ struct ZBlock {
 
 // Bilinear sample of the z-column kernel at row offset fr (= 1 - Dez) and channel
 // blend weight Dey (on channel U-1), from the byte offset `ob` of sample
-// (row V-1, channel U-1).  Rows layout: two gathers, rows V-1 and V (the second
-// through the instruction's scalar offset).  Row-pair layout: one gather of the
-// four samples (V-1, U-1), (V, U-1), (V-1, U), (V, U).
+// (row V-1, channel U-1).  load() issues the gathers and lerp() blends them, so
+// a view's gathers can all be in flight before the first blend.  Rows layout: two
+// gathers, rows V-1 and V (the second through the instruction's scalar offset).
+// Row-pair layout: one gather of the four samples (V-1, U-1), (V, U-1), (V-1, U),
+// (V, U).
 template <class T, int LAYOUT> struct Gather;
 
 template <class T> struct Gather<T, ARECON_FP_ROWS> {
     static constexpr unsigned kElem = sizeof(T);            // bytes per channel step
-    __device__ static __forceinline__ float sample(__amdgpu_buffer_rsrc_t rs, unsigned ob,
-                                                   unsigned rowb, float Dey, float fr) {
-        const float2 r0 = bload2<T>(rs, ob, 0u);          // row V-1: U-1, U
-        const float2 r1 = bload2<T>(rs, ob, rowb);        // row V:   U-1, U
-        const float top = fmaf(Dey, r0.x - r0.y, r0.y);
-        const float bot = fmaf(Dey, r1.x - r1.y, r1.y);
+    struct Raw { float2 r0, r1; };
+    __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t rs, unsigned ob,
+                                               unsigned rowb) {
+        return Raw{bload2<T>(rs, ob, 0u), bload2<T>(rs, ob, rowb)};   // rows V-1, V
+    }
+    __device__ static __forceinline__ float lerp(const Raw& q, float fr, float Dey, float) {
+        const float top = fmaf(Dey, q.r0.x - q.r0.y, q.r0.y);
+        const float bot = fmaf(Dey, q.r1.x - q.r1.y, q.r1.y);
         return fmaf(fr, bot - top, top);
     }
 };
 
 typedef float f32x4_v __attribute__((ext_vector_type(4)));
 
-// Four samples as floats: (V-1, U-1), (V, U-1), (V-1, U), (V, U).
-template <class T> __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off);
-template <> __device__ __forceinline__ float4 bload4<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    const f32x4_v v = __builtin_bit_cast(f32x4_v, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0u, 0));
-    return make_float4(v.x, v.y, v.z, v.w);
-}
-template <> __device__ __forceinline__ float4 bload4<__half>(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    const unsigned long long v =
-        __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0u, 0));
-    const unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
-    return make_float4(__half2float(__ushort_as_half((unsigned short)(lo & 0xffffu))),
-                       __half2float(__ushort_as_half((unsigned short)(lo >> 16))),
-                       __half2float(__ushort_as_half((unsigned short)(hi & 0xffffu))),
-                       __half2float(__ushort_as_half((unsigned short)(hi >> 16))));
-}
-template <> __device__ __forceinline__ float4 bload4<__hip_bfloat16>(__amdgpu_buffer_rsrc_t r,
-                                                                    unsigned off) {
-    const unsigned long long v =
-        __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0u, 0));
-    const unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
-    return make_float4(__uint_as_float(lo << 16), __uint_as_float(lo & 0xffff0000u),
-                       __uint_as_float(hi << 16), __uint_as_float(hi & 0xffff0000u));
-}
+template <class T> struct Quad;           // four samples as loaded
+template <> struct Quad<float> {
+    typedef f32x4_v Raw;
+    __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(f32x4_v, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0u, 0));
+    }
+    __device__ static __forceinline__ float4 get(const Raw& v) {
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
+};
+template <> struct Quad<__half> {
+    typedef unsigned long long Raw;
+    __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(unsigned long long,
+                                  __builtin_amdgcn_raw_buffer_load_b64(r, off, 0u, 0));
+    }
+    __device__ static __forceinline__ float4 get(const Raw& v) {
+        const unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+        return make_float4(__half2float(__ushort_as_half((unsigned short)(lo & 0xffffu))),
+                           __half2float(__ushort_as_half((unsigned short)(lo >> 16))),
+                           __half2float(__ushort_as_half((unsigned short)(hi & 0xffffu))),
+                           __half2float(__ushort_as_half((unsigned short)(hi >> 16))));
+    }
+};
+template <> struct Quad<__hip_bfloat16> {
+    typedef unsigned long long Raw;
+    __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(unsigned long long,
+                                  __builtin_amdgcn_raw_buffer_load_b64(r, off, 0u, 0));
+    }
+    __device__ static __forceinline__ float4 get(const Raw& v) {
+        const unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+        return make_float4(__uint_as_float(lo << 16), __uint_as_float(lo & 0xffff0000u),
+                           __uint_as_float(hi << 16), __uint_as_float(hi & 0xffff0000u));
+    }
+};
 
 template <class T> struct Gather<T, ARECON_FP_ROWPAIRS> {
     static constexpr unsigned kElem = 2 * sizeof(T);
-    __device__ static __forceinline__ float sample(__amdgpu_buffer_rsrc_t rs, unsigned ob,
-                                                   unsigned, float Dey, float fr) {
-        const float4 q = bload4<T>(rs, ob);
+    typedef typename Quad<T>::Raw Raw;
+    __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t rs, unsigned ob, unsigned) {
+        return Quad<T>::load(rs, ob);
+    }
+    __device__ static __forceinline__ float lerp(const Raw& raw, float fr, float Dey, float) {
+        const float4 q = Quad<T>::get(raw);
         const float c0 = fmaf(fr, q.y - q.x, q.x);        // channel U-1, rows blended
         const float c1 = fmaf(fr, q.w - q.z, q.z);        // channel U
         return fmaf(Dey, c0 - c1, c1);
     }
 };
 
+// Difference quads: (a, dr, dc, drc) with the sample at (row V-1 + fr, channel
+// U-1 + wu) equal to a + fr dr + wu (dc + fr drc), wu = 1 - Dey.
+template <class T> struct Gather<T, ARECON_FP_QUADS> {
+    static constexpr unsigned kElem = 4 * sizeof(T);
+    typedef typename Quad<T>::Raw Raw;
+    __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t rs, unsigned ob, unsigned) {
+        return Quad<T>::load(rs, ob);
+    }
+    __device__ static __forceinline__ float lerp(const Raw& raw, float fr, float, float wu) {
+        const float4 q = Quad<T>::get(raw);
+        return fmaf(wu, fmaf(fr, q.w, q.z), fmaf(fr, q.y, q.x));
+    }
+};
+
+// Row index and blend weight of one slice: vi = floor(V1) (= ceil(V1) - 1 of the
+// reference for every non-integer V1; for an integer V1 both pick the same
+// sample) and fr = fract(V1) = 1 - Dez.  Returns the gather offset, or one past
+// any view when the row is off the detector (0 < ceil(V1) < ZL fails): with the
+// row-pair and quad layouts the view's buffer ends after row ZL-2, so clamping
+// vi to ZL-1 is enough; with rows, the second row must be tested explicitly.
+template <int LAYOUT>
+__device__ __forceinline__ unsigned zc_offset(float V1, unsigned rowb, unsigned ucolb,
+                                              unsigned zlm1, float& fr) {
+    // floor and convert in one instruction.  V1 comes from an FMA (no
+    // transcendental-result hazard for the asm to miss).
+    int vf;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(vf) : "v"(V1));
+    fr = __builtin_amdgcn_fractf(V1);
+    if (LAYOUT != ARECON_FP_ROWS)
+        return __umul24(min((unsigned)vf, zlm1), rowb) + ucolb;
+    return (unsigned)vf < zlm1 ? __umul24((unsigned)vf, rowb) + ucolb : 0x80000000u;
+}
+
 // One group of G consecutive slices of the block, branch-free: the slices of the
 // group outside the view's window carry temp2 = 1e30 in the table, so their row
-// is off the detector and their gathers read 0.  The row index is floor(V1)
-// (= ceil(V1) - 1 of the reference for every non-integer V1; for an integer V1
-// both pick the same sample) and the row blend weight is fract(V1) = 1 - Dez.
+// is off the detector and their gathers read 0.
 template <int G, class T, int LAYOUT, bool WEIGHTED>
 __device__ __forceinline__ void zc_group(const float2* __restrict__ e, float K, float zlc,
                                          float B, unsigned ucolb, unsigned rowb,
                                          unsigned zlm1, __amdgpu_buffer_rsrc_t rs, float Dey,
-                                         float* acc) {
-    float tou[G];
+                                         float wu, float* acc) {
+    typedef Gather<T, LAYOUT> Gt;
+    typename Gt::Raw q[G];
+    float fr[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+        q[i] = Gt::load(rs, zc_offset<LAYOUT>(fmaf(e[i].x, K, zlc), rowb, ucolb, zlm1, fr[i]),
+                        rowb);
 #pragma unroll
     for (int i = 0; i < G; ++i) {
-        const float V1 = fmaf(e[i].x, K, zlc);
-        const float Vf = floorf(V1);
-        const unsigned vi = (unsigned)(int)Vf;                  // row V-1 of the reference
-        const unsigned ob = vi < zlm1 ? __umul24(vi, rowb) + ucolb : 0x80000000u;
-        tou[i] = Gather<T, LAYOUT>::sample(rs, ob, rowb, Dey, V1 - Vf);
-    }
-#pragma unroll
-    for (int i = 0; i < G; ++i) {
+        const float tou = Gt::lerp(q[i], fr[i], Dey, wu);
         if (WEIGHTED)   // 1 / (1 + 2^Q B); exactly 1 where a = 0
-            acc[i] = fmaf(fast_rcp(fmaf(e[i].y, B, 1.f)), tou[i], acc[i]);
+            acc[i] = fmaf(fast_rcp(fmaf(e[i].y, B, 1.f)), tou, acc[i]);
         else
-            acc[i] += tou[i];
+            acc[i] += tou;
+    }
+}
+
+// All ZB slices of the block for one view, each behind its own uniform window
+// test: first every active slice's gather, then every blend.
+template <int ZB, class T, int LAYOUT>
+__device__ __forceinline__ void zc_split(const float2* __restrict__ e, uint2 m, float K,
+                                         float zlc, float B, unsigned ucolb, unsigned rowb,
+                                         unsigned zlm1, __amdgpu_buffer_rsrc_t rs, float Dey,
+                                         float wu, float* acc) {
+    typedef Gather<T, LAYOUT> Gt;
+    struct EntZ { float2 e[ZB]; };
+    const EntZ ent = *reinterpret_cast<const EntZ*>(e);          // scalar loads
+    typename Gt::Raw q[ZB];
+    float fr[ZB];
+#pragma unroll
+    for (int j = 0; j < ZB; ++j)
+        if ((m.x >> j) & 1u)
+            q[j] = Gt::load(rs, zc_offset<LAYOUT>(fmaf(ent.e[j].x, K, zlc), rowb, ucolb, zlm1,
+                                                  fr[j]), rowb);
+    if (m.y != 0u) {
+#pragma unroll
+        for (int j = 0; j < ZB; ++j)
+            if ((m.x >> j) & 1u)   // 1 / (1 + 2^Q B); exactly 1 where a = 0
+                acc[j] = fmaf(fast_rcp(fmaf(ent.e[j].y, B, 1.f)),
+                              Gt::lerp(q[j], fr[j], Dey, wu), acc[j]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < ZB; ++j)
+            if ((m.x >> j) & 1u) acc[j] += Gt::lerp(q[j], fr[j], Dey, wu);
     }
 }
 
@@ -452,7 +532,8 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
                                                     const ZBlock* __restrict__ blocks,
                                                     const uint2* __restrict__ masks,
                                                     const float2* __restrict__ table, BPArgs A) {
-    static_assert(ZB % G == 0 && ZB <= 32 && (G == 1 || G == 2 || G == 4), "bad block shape");
+    static_assert(G == 0 ? ZB <= 16 : (ZB % G == 0 && ZB <= 32 && (G == 1 || G == 2 || G == 4)),
+                  "bad block shape");
     const int nb = gridDim.x, b = blockIdx.x;
     const int qx = nb >> 3, rx = nb & 7, xcd = b & 7;
     const int lb = xcd * qx + min(xcd, rx) + (b >> 3);
@@ -470,13 +551,14 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
     const uint2* mrow = masks + hb.off / ZB - hb.pmin;
     const float2* trow = table + hb.off - (long long)hb.pmin * ZB;
     constexpr unsigned kElem = Gather<T, LAYOUT>::kElem;
-    const unsigned rowb = (unsigned)A.YL * kElem;           // bytes per detector row
+    // bytes per detector row: YL elements, or YL-1 quads (one per channel pair)
+    const unsigned rowb = (unsigned)(LAYOUT == ARECON_FP_QUADS ? A.YL - 1 : A.YL) * kElem;
     const unsigned zlm1 = (unsigned)A.ZL - 1u;
     // ZLC as a per-lane value, so the per-slice fma takes the table's temp2 as
     // its one scalar operand.
     float zlc;
     asm("v_mov_b32 %0, %1" : "=v"(zlc) : "s"(A.ZLC));
-    constexpr unsigned gmask = (1u << G) - 1u;
+    constexpr unsigned gmask = (1u << (G == 0 ? 1 : G)) - 1u;
     float acc[ZB];
 #pragma unroll
     for (int j = 0; j < ZB; ++j) acc[j] = 0.f;
@@ -497,6 +579,7 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
         const bool uok = (U1 > 0.f) && (U1 <= A.YLm1) && (t1sq >= 0.f) && (den != 0.f);
         const float Uc = ceilf(U1);
         const float Dey = Uc - U1;
+        const float wu = 1.f - Dey;
         // byte offset of channel U-1; off the detector, past any view (< 2 GiB) so
         // both gathers read 0 whatever the row
         const unsigned ucolb = uok ? ((unsigned)Uc - 1u) * kElem : 0x80000000u;
@@ -507,15 +590,20 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
             B = fminf(fmaxf(fast_exp2(A.k1 * fast_log2(fabsf((temp1 - UU) * temp1 * inv))),
                             1e-30f), 1e30f);
         const float2* erow = trow + (long long)p * ZB;
+        if constexpr (G == 0) {
+            zc_split<ZB, T, LAYOUT>(erow, m, K, zlc, B, ucolb, rowb, zlm1, rs, Dey, wu, acc);
+            continue;
+        }
 #pragma unroll
-        for (int g = 0; g < ZB; g += G) {
+        for (int g = 0; g < (G == 0 ? 0 : ZB); g += (G == 0 ? 1 : G)) {
             if (((m.x >> g) & gmask) == 0u) continue;
+            constexpr int GG = G == 0 ? 1 : G;
             if ((m.y >> g) & gmask)
-                zc_group<G, T, LAYOUT, true>(erow + g, K, zlc, B, ucolb, rowb, zlm1, rs, Dey,
-                                             acc + g);
+                zc_group<GG, T, LAYOUT, true>(erow + g, K, zlc, B, ucolb, rowb, zlm1, rs, Dey,
+                                              wu, acc + g);
             else
-                zc_group<G, T, LAYOUT, false>(erow + g, K, zlc, B, ucolb, rowb, zlm1, rs, Dey,
-                                              acc + g);
+                zc_group<GG, T, LAYOUT, false>(erow + g, K, zlc, B, ucolb, rowb, zlm1, rs, Dey,
+                                               wu, acc + g);
         }
     }
     if (xi < A.XN && yi < A.YN) {
@@ -821,8 +909,9 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         ARECON_REQUIRE(nz == 8 || nz == 16 || nz == 32,
                        "bp: nz of the z-column kernel must be 0, 8, 16 or 32");
         group = g.group > 0 ? g.group : 2;
-        ARECON_REQUIRE(group == 1 || group == 2 || group == 4,
-                       "bp: group of the z-column kernel must be 0, 1, 2 or 4");
+        ARECON_REQUIRE(group == 1 || group == 2 || group == 4 || (group == 8 && nz <= 16),
+                       "bp: group of the z-column kernel must be 0, 1, 2, 4, or 8 (split "
+                       "gathers and blends, nz <= 16)");
     } else {
         nz = g.nz > 0 ? g.nz : auto_nz(zt, W);
         ARECON_REQUIRE(nz == 1 || nz == 2 || nz == 4 || nz == 8 || nz == 16,
@@ -851,10 +940,11 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
     const long long grid = (long long)A.ntiles * A.nzsets;
     P->grid = (int)grid;
     try {
-        // one view of either layout (rows, or row pairs at twice the size) stays
-        // below 2 GiB, and rows and row strides fit the kernels' 24-bit multiplies
-        ARECON_REQUIRE(A.view_stride * 8 < (1LL << 31), "bp: one view exceeds 1 GiB");
-        ARECON_REQUIRE((long long)g.YL * 8 < (1LL << 24) && g.ZL < (1 << 24),
+        // one view in any layout (rows, row pairs at twice and quads at four times
+        // the size) stays below 2 GiB, and rows and row strides fit the kernels'
+        // 24-bit multiplies
+        ARECON_REQUIRE(A.view_stride * 16 < (1LL << 31), "bp: one view exceeds 128 MiB");
+        ARECON_REQUIRE((long long)g.YL * 16 < (1LL << 24) && g.ZL < (1 << 24),
                        "bp: detector too large for 24-bit row offsets");
         ARECON_REQUIRE(grid < (1LL << 31), "bp: volume too large for one launch");
         ARECON_HIP(hipGetDevice(&P->device));
@@ -896,9 +986,10 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
     BPArgs a = P->a;
     a.vec4 = (a.ZN % 4 == 0) && ((reinterpret_cast<uintptr_t>(img) & 15) == 0);
     if (layout == ARECON_FP_ROWPAIRS) a.view_stride = 2LL * (a.ZL - 1) * a.YL;
+    if (layout == ARECON_FP_QUADS) a.view_stride = 4LL * (a.ZL - 1) * (a.YL - 1);
     a.view_bytes = (unsigned)(a.view_stride * (long long)sizeof(T));
     ARECON_REQUIRE(layout == ARECON_FP_ROWS || P->variant == 2,
-                   "bp: the row-pair layout needs the z-column kernel (variant 2)");
+                   "bp: the row-pair and quad layouts need the z-column kernel (variant 2)");
     if constexpr (NZ <= 16) {
         if (P->variant == 1) {
             hipLaunchKernelGGL((bp_fdk3dw<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp,
@@ -919,11 +1010,18 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
             if (layout == ARECON_FP_ROWS) {
                 if (P->group == 1) ARECON_ZC(1, ARECON_FP_ROWS);
                 else if (P->group == 2) ARECON_ZC(2, ARECON_FP_ROWS);
-                else ARECON_ZC(4, ARECON_FP_ROWS);
-            } else {
+                else if (P->group == 4) ARECON_ZC(4, ARECON_FP_ROWS);
+                else if constexpr (NZ <= 16) ARECON_ZC(0, ARECON_FP_ROWS);
+            } else if (layout == ARECON_FP_ROWPAIRS) {
                 if (P->group == 1) ARECON_ZC(1, ARECON_FP_ROWPAIRS);
                 else if (P->group == 2) ARECON_ZC(2, ARECON_FP_ROWPAIRS);
-                else ARECON_ZC(4, ARECON_FP_ROWPAIRS);
+                else if (P->group == 4) ARECON_ZC(4, ARECON_FP_ROWPAIRS);
+                else if constexpr (NZ <= 16) ARECON_ZC(0, ARECON_FP_ROWPAIRS);
+            } else {
+                if (P->group == 1) ARECON_ZC(1, ARECON_FP_QUADS);
+                else if (P->group == 2) ARECON_ZC(2, ARECON_FP_QUADS);
+                else if (P->group == 4) ARECON_ZC(4, ARECON_FP_QUADS);
+                else if constexpr (NZ <= 16) ARECON_ZC(0, ARECON_FP_QUADS);
             }
 #undef ARECON_ZC
             return;
@@ -948,7 +1046,8 @@ void bp_plan_launch(const BPPlan* P, float* img, const void* fp, int fp_dtype, i
                     hipStream_t s) {
     ARECON_REQUIRE(P != nullptr, "bp: null plan");
     ARECON_REQUIRE(img != nullptr && fp != nullptr, "bp: null buffer");
-    ARECON_REQUIRE(fp_layout == ARECON_FP_ROWS || fp_layout == ARECON_FP_ROWPAIRS,
+    ARECON_REQUIRE(fp_layout == ARECON_FP_ROWS || fp_layout == ARECON_FP_ROWPAIRS ||
+                       fp_layout == ARECON_FP_QUADS,
                    "bp: unknown fp layout");
     switch (fp_dtype) {
     case ARECON_F32: launch_t<float>(P, img, static_cast<const float*>(fp), fp_layout, s); break;

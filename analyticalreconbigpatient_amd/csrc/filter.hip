@@ -503,9 +503,97 @@ const FilterTables& filter_tables(int device, int YL, float dYL_f) {
 
 }  // namespace
 
+// Difference quads from filtered FP32 rows: element (v, r, c) of
+// [ViewN][ZL-1][YL-1][4] holds a = f(r,c), f(r+1,c)-a, f(r,c+1)-a and
+// (f(r+1,c+1)-f(r,c+1)) - (f(r+1,c)-a).  One thread per element; consecutive
+// threads walk the channels, so reads and writes are coalesced.
+template <class OutT> struct Quad4;
+template <> struct Quad4<float> {
+    __device__ static __forceinline__ void put(float* o, float a, float b, float c, float d) {
+        *reinterpret_cast<float4*>(o) = make_float4(a, b, c, d);
+    }
+};
+template <> struct Quad4<__half> {
+    __device__ static __forceinline__ void put(__half* o, float a, float b, float c, float d) {
+        const unsigned lo = (unsigned)__half_as_ushort(__float2half(a)) |
+                            ((unsigned)__half_as_ushort(__float2half(b)) << 16);
+        const unsigned hi = (unsigned)__half_as_ushort(__float2half(c)) |
+                            ((unsigned)__half_as_ushort(__float2half(d)) << 16);
+        *reinterpret_cast<uint2*>(o) = make_uint2(lo, hi);
+    }
+};
+template <> struct Quad4<__hip_bfloat16> {
+    __device__ static __forceinline__ void put(__hip_bfloat16* o, float a, float b, float c,
+                                               float d) {
+        auto bits = [](float v) { return (unsigned)__bfloat16_as_ushort(__float2bfloat16(v)); };
+        *reinterpret_cast<uint2*>(o) =
+            make_uint2(bits(a) | (bits(b) << 16), bits(c) | (bits(d) << 16));
+    }
+};
+
+template <class OutT>
+__global__ __launch_bounds__(256) void expand_quads(OutT* __restrict__ out,
+                                                    const float* __restrict__ rows, int YL,
+                                                    int ZL, long long n) {
+    const int nc = YL - 1, nr = ZL - 1;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long t = i / nc;
+        const int c = (int)(i - t * nc);
+        const long long v = t / nr;
+        const int r = (int)(t - v * nr);
+        const float* p = rows + (v * ZL + r) * YL + c;
+        const float a = p[0], ac = p[1], b = p[YL], bc = p[YL + 1];
+        const float dr = b - a, dc = ac - a;
+        Quad4<OutT>::put(out + 4 * i, a, dr, dc, (bc - ac) - dr);
+    }
+}
+
+static void launch_filter_rows(void* out, int out_dtype, int out_layout, const float* in,
+                               int YL, int ZL, int ViewN, float PLC, float ZLC, float dYL,
+                               float dZL, float SO, hipStream_t s);
+
 void launch_filter(void* out, int out_dtype, int out_layout, const float* in, int YL, int ZL,
                    int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
                    hipStream_t s) {
+    if (out_layout != ARECON_FP_QUADS) {
+        launch_filter_rows(out, out_dtype, out_layout, in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL,
+                           SO, s);
+        return;
+    }
+    ARECON_REQUIRE(YL >= 2 && ZL >= 2, "filter: the quad layout needs YL >= 2 and ZL >= 2");
+    ARECON_REQUIRE(out != nullptr && out != (const void*)in,
+                   "filter: the quad output cannot alias the input");
+    float* rows = const_cast<float*>(in);
+    launch_filter_rows(rows, ARECON_F32, ARECON_FP_ROWS, in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL,
+                       SO, s);
+    const long long n = (long long)ViewN * (ZL - 1) * (YL - 1);
+    int dev = 0, cus = 0;
+    ARECON_HIP(hipGetDevice(&dev));
+    ARECON_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const long long grid = std::max<long long>(1, std::min<long long>((n + 255) / 256,
+                                                                      (long long)cus * 64));
+    switch (out_dtype) {
+    case ARECON_F32:
+        hipLaunchKernelGGL(expand_quads<float>, dim3((unsigned)grid), dim3(256), 0, s,
+                           static_cast<float*>(out), rows, YL, ZL, n);
+        break;
+    case ARECON_F16:
+        hipLaunchKernelGGL(expand_quads<__half>, dim3((unsigned)grid), dim3(256), 0, s,
+                           static_cast<__half*>(out), rows, YL, ZL, n);
+        break;
+    case ARECON_BF16:
+        hipLaunchKernelGGL(expand_quads<__hip_bfloat16>, dim3((unsigned)grid), dim3(256), 0, s,
+                           static_cast<__hip_bfloat16*>(out), rows, YL, ZL, n);
+        break;
+    default: fail(ARECON_ERR_ARG, "filter: unknown output dtype");
+    }
+    ARECON_HIP(hipGetLastError());
+}
+
+static void launch_filter_rows(void* out, int out_dtype, int out_layout, const float* in,
+                               int YL, int ZL, int ViewN, float PLC, float ZLC, float dYL,
+                               float dZL, float SO, hipStream_t s) {
     ARECON_REQUIRE(YL > 0 && ZL > 0 && ViewN > 0, "filter: YL, ZL and ViewN must be positive");
     ARECON_REQUIRE(YL <= 2048, "filter: YL above 2048 is not supported");
     ARECON_REQUIRE(out != nullptr && in != nullptr, "filter: null buffer");

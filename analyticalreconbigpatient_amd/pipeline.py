@@ -20,17 +20,23 @@ import torch
 
 from . import _lib
 from .geometry import (BPParams, ReconSetup, bp_params, bp_scan_geom, filter_args, rebin_args,
-                       voxel_updates)
+                       voxel_updates, window_half_width)
 
 STORAGE = {"f32": (_lib.ARECON_F32, torch.float32), "f16": (_lib.ARECON_F16, torch.float16),
            "bf16": (_lib.ARECON_BF16, torch.bfloat16)}
-# Filtered-sinogram layouts: the reference's rows [ViewN][ZL][YL], or row pairs
-# [ViewN][ZL-1][YL][2] (rows r and r+1 of each channel side by side).
-LAYOUT = {"rows": _lib.ARECON_FP_ROWS, "rowpairs": _lib.ARECON_FP_ROWPAIRS}
+# Filtered-sinogram layouts: the reference's rows [ViewN][ZL][YL]; row pairs
+# [ViewN][ZL-1][YL][2] (rows r and r+1 of each channel side by side); difference
+# quads [ViewN][ZL-1][YL-1][4] (sample, row, channel and cross differences).
+LAYOUT = {"rows": _lib.ARECON_FP_ROWS, "rowpairs": _lib.ARECON_FP_ROWPAIRS,
+          "quads": _lib.ARECON_FP_QUADS}
 
 
 def filtered_shape(ViewN: int, ZL: int, YL: int, layout: str) -> tuple:
-    return (ViewN, ZL, YL) if layout == "rows" else (ViewN, ZL - 1, YL, 2)
+    if layout == "rows":
+        return (ViewN, ZL, YL)
+    if layout == "rowpairs":
+        return (ViewN, ZL - 1, YL, 2)
+    return (ViewN, ZL - 1, YL - 1, 4)
 
 
 def _stream_ptr(stream: torch.cuda.Stream | None) -> C.c_void_p:
@@ -45,6 +51,22 @@ def fdk_weights(N_2pi: int, HSCoef: float, delta: int) -> np.ndarray:
     _lib.check(_lib.lib().arecon_fdk_weights(_lib.dptr(w), N_2pi, HSCoef, delta),
                "arecon_fdk_weights")
     return w
+
+
+def choose_bp_kernel(params: BPParams, dz: float | None = None, storage: str = "f32") -> dict:
+    """Backprojector launch shape for a geometry.  Consecutive slices' windows
+    are N_2pi dz / h views apart; when a view lies in the windows of at least
+    four consecutive slices, the z-column kernel over blocks of 8 slices shares
+    the per-(column, view) work among them, with one gather per voxel-update:
+    row pairs for FP32 storage, difference quads (same bytes, three FMAs per
+    blend) for 16-bit storage.  Otherwise (thick slices, few shared views) the
+    view-table kernel runs on the rows layout."""
+    dzv = params.dz if dz is None else dz
+    L = 2 * window_half_width(params.N_2pi, params.HSCoef) + 1
+    shift = abs(params.N_2pi * dzv / params.h)
+    if shift * 4 <= L:
+        return dict(variant=2, nz=8, group=8, layout="rowpairs" if storage == "f32" else "quads")
+    return dict(variant=0, nz=0, group=0, layout="rows")
 
 
 class BPPlan:
@@ -133,12 +155,14 @@ class HelicalFDK:
     ``bp_geom`` is the ScanGeom after trimming (BetaS shifted accordingly).
     ``storage`` is the element type of the filtered sinogram and ``layout`` its
     layout ("rows", or "rowpairs" for the z-column backprojector, variant 2).
+    With ``variant=None`` the backprojector's launch shape and the layout come
+    from ``choose_bp_kernel``.
     """
 
-    def __init__(self, setup: ReconSetup, *, storage: str = "f32", layout: str = "rows",
+    def __init__(self, setup: ReconSetup, *, storage: str = "f32", layout: str | None = None,
                  first_view: int = 0,
                  n_views: int | None = None, bp_geom=None, dz: float | None = None,
-                 nz: int = 0, order: int = 0, variant: int = 0, group: int = 0,
+                 nz: int = 0, order: int = 0, variant: int | None = None, group: int = 0,
                  device: str = "cuda"):
         self.setup = setup
         self.ra = rebin_args(setup.geom)
@@ -148,7 +172,13 @@ class HelicalFDK:
         g = bp_scan_geom(setup, geom=bp_geom, n_views=self.n_views)
         self.params = bp_params(g)
         self.dz = dz
+        if variant is None:       # launch shape chosen from the geometry
+            k = choose_bp_kernel(self.params, dz, storage)
+            variant, nz, group = k["variant"], nz or k["nz"], group or k["group"]
+            layout = layout or k["layout"]
+        layout = layout or "rows"
         self.plan = BPPlan(self.params, dz=dz, nz=nz, order=order, variant=variant, group=group)
+        self.variant, self.group = variant, group
         self.storage = storage
         self.layout = layout
         code, tdtype = STORAGE[storage]

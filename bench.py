@@ -25,7 +25,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "GVoxel-updates/s end-to-end at 512²×400, chosen vs plain; HU RMSE vs FP64"
-HBM_PEAK_GBS = 8000.0
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+N_CU, CLOCK_HZ = 256, 2.4e9  # MI355X compute units and peak engine clock
 
 
 def parse():
@@ -35,14 +36,26 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and HU RMSE")
     ap.add_argument("--cpu-slices", type=int, default=2, help="slices in the CPU sample")
-    ap.add_argument("--nz", type=int, default=0)
+    ap.add_argument("--nz", type=int, default=0, help="slices per thread (0: chosen)")
     ap.add_argument("--order", type=int, default=0)
-    ap.add_argument("--variant", type=int, default=0, help="backprojector kernel variant")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="backprojector kernel variant (default: chosen from the geometry)")
+    ap.add_argument("--group", type=int, default=0, help="z-column kernel slice grouping")
+    ap.add_argument("--layout", default=None, help="filtered layout: rows or rowpairs")
     ap.add_argument("--storages", default="f32,f16",
                     help="filtered-sinogram storages to time (plain f32 first)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 2 GiB device copy (calibrates HBM byte counters)")
     return ap.parse_args()
+
+
+def kernel_name(fdk) -> str:
+    """The backprojector's kernel as rocprofv3 names it."""
+    if fdk.variant == 2:
+        lay = 1 if fdk.layout == "rowpairs" else 0
+        g = 0 if fdk.group == 8 else fdk.group
+        return f"bp_fdk3dw_zc<{fdk.plan_nz}, {g}, float, {lay}>"
+    return f"bp_fdk3dw{'_vt' if fdk.variant == 0 else ''}<{fdk.plan_nz}, float>"
 
 
 def log(msg):
@@ -97,7 +110,7 @@ def main():
     assert storages and storages[0] == "f32", "the plain FP32 path is timed first"
     for storage in storages:
         fdk = HelicalFDK(s, storage=storage, nz=args.nz, order=args.order,
-                         variant=args.variant)
+                         variant=args.variant, group=args.group, layout=args.layout)
         for _ in range(args.warmup):
             fdk.run(raw)
         torch.cuda.synchronize()
@@ -137,19 +150,31 @@ def main():
     nvox = p.XN * p.YN * p.ZN
     alg_bytes = 16.0 * updates + 4.0 * nvox
     achieved = alg_bytes / (bp_ms * 1e-3) / 1e9
+    kname = kernel_name(fdk)
     traffic = None
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get("bp_fdk3dw_f32_clinical_bytes")
-        except (ValueError, OSError):
+            entry = json.loads(tfile.read_text()).get("kernels", {}).get(kname)
+            if entry and entry.get("workload") == wl.name:
+                traffic = entry["fetch_bytes"] + entry["write_bytes"]
+        except (ValueError, OSError, KeyError, TypeError):
             traffic = None
+    # Issue bound of the gathers: the texture addresser takes four lane addresses
+    # per clock, so one 64-lane gather instruction occupies it for 16 clocks; the
+    # row-pair layout needs one gather per voxel-update, the rows layout two.
+    gathers = 1 if fdk.layout == "rowpairs" else 2
+    gather_peak = N_CU * CLOCK_HZ / 16.0 * 64 / gathers / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"bp_fdk3dw{'_vt' if args.variant == 0 else ''}<{fdk.plan_nz},float>",
-                "kernel_ms": round(bp_ms, 3),
-                "bytes_per_update": 16, "note": "gathers are served from L2/MALL; "
-                "compulsory HBM bytes are the filtered sinogram once and the volume once"}
+                "kernel": kname, "kernel_ms": round(bp_ms, 3), "bytes_per_update": 16,
+                "note": "gathers are served from L1/L2; compulsory HBM bytes are the filtered "
+                        "sinogram once and the volume once, so the gather issue rate, not HBM, "
+                        "bounds this kernel",
+                "gather_issue": {"achieved": round(updates / (bp_ms * 1e-3) / 1e9, 1),
+                                 "peak": round(gather_peak, 1), "unit": "GVoxel-updates/s",
+                                 "frac": round(updates / (bp_ms * 1e-3) / 1e9 / gather_peak, 4),
+                                 "gathers_per_update": gathers}}
 
     stage_roof = {
         "rebin_GBs": round(8.0 * p.YL * p.ZL * fdk.ViewN / (plain["stages"][0] * 1e-3) / 1e9, 1),

@@ -50,6 +50,10 @@ extern "C" {
 #define ARECON_FP_ROWPAIRS 1  /* [ViewN][ZL-1][YL][2]: element (v, r, c) holds rows r
                                  and r+1 of channel c, so one 4-sample load fetches
                                  the 2x2 bilinear neighbourhood of the backprojector */
+#define ARECON_FP_QUADS 2     /* [ViewN][ZL-1][YL-1][4]: element (v, r, c) holds
+                                 f(r,c), f(r+1,c)-f(r,c), f(r,c+1)-f(r,c) and
+                                 f(r+1,c+1)-f(r+1,c)-f(r,c+1)+f(r,c): the bilinear
+                                 sample at (r+fr, c+fc) is then three FMAs */
 
 /* ========================================================================== */
 /* 1. Reference drop-ins                                                      */
@@ -141,8 +145,10 @@ int arecon_rebin_dev(float* d_out, const float* d_in, int YL, int ZL, int ViewN,
                      float DeltaFai, float SO, int out_layout, void* stream);
 
 /* Preweight + ramp filter on device buffers; input layout [ViewN][ZL][YL],
- * output layout out_layout (ARECON_FP_ROWS or ARECON_FP_ROWPAIRS).  d_out may
- * equal d_in only for ARECON_F32 rows. */
+ * output layout out_layout (ARECON_FP_ROWS, ARECON_FP_ROWPAIRS or
+ * ARECON_FP_QUADS).  d_out may equal d_in only for ARECON_F32 rows.  For
+ * ARECON_FP_QUADS the filter runs in place in d_in (which then holds the FP32
+ * filtered rows) and the quads are derived from it. */
 int arecon_filter_dev(void* d_out, int out_dtype, int out_layout, const float* d_in, int YL,
                       int ZL, int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
                       void* stream);
@@ -164,8 +170,10 @@ typedef struct arecon_bp_geom {
     int order;                  /* workgroup order: 0 = slice-major, 1 = column-major */
     int variant;                /* 0 = view-table kernel (default), 1 = per-voxel weight,
                                    2 = z-column kernel (slice blocks with view masks) */
-    int group;                  /* z-column kernel: slices per branch-free group,
-                                   0 = automatic, else 1/2/4 */
+    int group;                  /* z-column kernel: 1/2/4 = slices per branch-free
+                                   group; 8 = each slice behind its own window test,
+                                   all gathers of a view issued before any blend
+                                   (nz <= 16); 0 = automatic */
 } arecon_bp_geom;
 
 typedef struct arecon_bp_plan arecon_bp_plan;

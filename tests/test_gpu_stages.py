@@ -84,7 +84,8 @@ def run_filter(P, PLC0, ZLC0, dYL, dZL, dtype=_lib.ARECON_F32, inplace=False,
     din = torch.from_numpy(np.ascontiguousarray(P, dtype=np.float32)).cuda()
     tdt = {_lib.ARECON_F32: torch.float32, _lib.ARECON_F16: torch.float16,
            _lib.ARECON_BF16: torch.bfloat16}[dtype]
-    shape = din.shape if layout == _lib.ARECON_FP_ROWS else (ViewN, ZL - 1, YL, 2)
+    shape = {_lib.ARECON_FP_ROWS: din.shape, _lib.ARECON_FP_ROWPAIRS: (ViewN, ZL - 1, YL, 2),
+             _lib.ARECON_FP_QUADS: (ViewN, ZL - 1, YL - 1, 4)}[layout]
     dout = din if inplace else torch.full(shape, float("nan"), dtype=tdt, device=din.device)
     _lib.check(_lib.lib().arecon_filter_dev(
         C.c_void_p(dout.data_ptr()), dtype, layout, C.c_void_p(din.data_ptr()), YL, ZL, ViewN,
@@ -116,6 +117,23 @@ def test_filter_row_pair_layout(gpu, ViewN, ZL, YL, dtype):
     pairs = run_filter(P, *args, dtype=dtype, layout=_lib.ARECON_FP_ROWPAIRS)
     assert np.array_equal(pairs[..., 0], rows[:, :-1, :])
     assert np.array_equal(pairs[..., 1], rows[:, 1:, :])
+
+
+@pytest.mark.parametrize("ViewN,ZL,YL", [(9, 64, 888), (7, 3, 20), (4, 2, 2048)])
+@pytest.mark.parametrize("dtype", [_lib.ARECON_F32, _lib.ARECON_F16])
+def test_filter_quad_layout(gpu, ViewN, ZL, YL, dtype):
+    """Quad output = the differences of the filtered rows, rounded once to the
+    output type (the filter runs in place in its FP32 input)."""
+    print(f"filter quads ViewN={ViewN} ZL={ZL} YL={YL} dtype={dtype}", flush=True)
+    P = np.random.default_rng(YL + 7 * ZL).normal(size=(ViewN, ZL, YL)).astype(np.float32)
+    args = ((YL - 1) / 2.0, (ZL - 1) / 2.0, 0.9493, 1.9231)
+    rows = run_filter(P, *args)
+    got = run_filter(P, *args, dtype=dtype, layout=_lib.ARECON_FP_QUADS)
+    want = quads(rows)
+    if dtype == _lib.ARECON_F32:
+        assert np.array_equal(got, want)
+    else:
+        assert np.array_equal(got, torch.from_numpy(want).half().float().numpy())
 
 
 def test_filter_in_place_and_16_bit_outputs(gpu):
@@ -159,12 +177,20 @@ def row_pairs(fp):
     return np.stack([fp[:, :-1, :], fp[:, 1:, :]], axis=-1)
 
 
+def quads(fp):
+    """[PN][ZL][YL] -> the difference-quad layout [PN][ZL-1][YL-1][4], in FP32."""
+    f = np.asarray(fp, np.float32)
+    a, ac, b, bc = f[:, :-1, :-1], f[:, :-1, 1:], f[:, 1:, :-1], f[:, 1:, 1:]
+    dr = b - a
+    return np.stack([a, dr, ac - a, (bc - ac) - dr], axis=-1)
+
+
 def gpu_bp(fp, g, nz=0, order=0, storage=torch.float32, variant=0, group=0, layout="rows"):
     PN, ZL, YL = fp.shape
     p = P_(g, YL, ZL, PN)
     w = oracle.fdk_weights(g["N_2pi"], g["HSCoef"], 8)
     plan = BPPlan(p, nz=nz, order=order, w=w, variant=variant, group=group)
-    host = fp if layout == "rows" else row_pairs(fp)
+    host = {"rows": lambda f: f, "rowpairs": row_pairs, "quads": quads}[layout](fp)
     dfp = torch.from_numpy(np.ascontiguousarray(host, dtype=np.float32)).cuda().to(storage)
     img = torch.full((g["YN"], g["XN"], g["ZN"]), float("nan"), device="cuda")
     plan.execute(img, dfp, layout=layout)
@@ -203,10 +229,10 @@ def test_bp_matches_fp64_oracle(gpu, nz, order, variant):
     assert rel_l2(got, want) < 1e-4
 
 
-@pytest.mark.parametrize("nz", [8, 16, 32])
-@pytest.mark.parametrize("group", [1, 2, 4])
+@pytest.mark.parametrize("nz,group", [(8, 1), (8, 2), (8, 4), (8, 8), (16, 2), (16, 4), (16, 8),
+                                      (32, 1), (32, 2), (32, 4)])
 @pytest.mark.parametrize("ZNC", [-15.37, -68.41])
-@pytest.mark.parametrize("layout", ["rows", "rowpairs"])
+@pytest.mark.parametrize("layout", ["rows", "rowpairs", "quads"])
 def test_bp_z_column_matches_fp64_oracle(gpu, nz, group, ZNC, layout):
     """Z-column kernel: slice blocks whose windows overlap heavily (thin slices),
     a ragged last block, windows cut by the first (ZNC=-15.37) or the last
@@ -254,6 +280,20 @@ def test_bp_16_bit_storage(gpu, storage, tol, variant, layout):
     got, w = gpu_bp(fp, g, storage=storage, variant=variant, layout=layout)
     want = oracle.fdk_core(rounded, w, **{k: v for k, v in g.items() if k not in ("YL", "ZL")})
     assert rel_l2(got, want) < tol
+
+
+@pytest.mark.parametrize("storage,tol", [(torch.float16, 1e-3), (torch.bfloat16, 8e-3)])
+def test_bp_16_bit_quads(gpu, storage, tol):
+    """16-bit difference quads: each of the four terms is rounded once, so the
+    result stays within the storage precision of the FP64 oracle."""
+    g = realistic(ZN=37, dz=0.45, ZNC=-15.37)
+    fp = np.random.default_rng(19).normal(size=(200, 16, 128)).astype(np.float32)
+    got, w = gpu_bp(fp, g, storage=storage, variant=2, nz=8, group=8, layout="quads")
+    want = oracle.fdk_core(fp.astype(np.float64), w,
+                           **{k: v for k, v in g.items() if k not in ("YL", "ZL")})
+    err = rel_l2(got, want)
+    print(f"16-bit quads {storage}: rel L2 {err:.2e}")
+    assert err < tol
 
 
 def test_bp_drop_in_float_and_double(gpu):
