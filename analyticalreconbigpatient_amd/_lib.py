@@ -67,6 +67,10 @@ SIGNATURES = {
     "arecon_bp_plan_destroy": (_i, [_p]),
     "arecon_bp_plan_nz": (_i, [_p]),
     "arecon_bp_plan_execute": (_i, [_p, _p, _p, _i, _i, _p]),
+    "arecon_bp_batch_create": (_i, [C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), _i, _i, _i,
+                                    C.POINTER(_p)]),
+    "arecon_bp_batch_destroy": (_i, [_p]),
+    "arecon_bp_batch_execute": (_i, [_p, _p]),
     "arecon_bp_dev_f64": (_i, [_p, _p, C.POINTER(BPGeom), _dp, _p]),
     "arecon_fdk_weights": (_i, [_dp, _i, _d, _i]),
     "arecon_project_ellipsoids_dev": (_i, [_p, _fp, _i, _i, _i, _i, _d, _d, _d, _d, _d, _d,

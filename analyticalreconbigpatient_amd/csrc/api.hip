@@ -326,6 +326,27 @@ int arecon_bp_plan_execute(const arecon_bp_plan* plan, float* d_img, const void*
     });
 }
 
+int arecon_bp_batch_create(const arecon_bp_plan* const* plans, float* const* d_imgs,
+                           const void* const* d_fps, int n, int fp_dtype, int fp_layout,
+                           arecon_bp_batch** out) {
+    return guarded([&] {
+        ARECON_REQUIRE(out != nullptr, "arecon_bp_batch_create: null output");
+        *out = reinterpret_cast<arecon_bp_batch*>(
+            bp_batch_create(reinterpret_cast<const BPPlan* const*>(plans), d_imgs, d_fps, n,
+                            fp_dtype, fp_layout));
+    });
+}
+
+int arecon_bp_batch_destroy(arecon_bp_batch* batch) {
+    return guarded([&] { bp_batch_destroy(reinterpret_cast<BPBatch*>(batch)); });
+}
+
+int arecon_bp_batch_execute(const arecon_bp_batch* batch, void* stream) {
+    return guarded([&] {
+        bp_batch_launch(reinterpret_cast<const BPBatch*>(batch), static_cast<hipStream_t>(stream));
+    });
+}
+
 int arecon_bp_dev_f64(double* d_img, const double* d_fp, const arecon_bp_geom* g,
                       const double* w, void* stream) {
     return guarded([&] {

@@ -91,6 +91,11 @@ void bp_plan_destroy(BPPlan* p);
 int bp_plan_nz(const BPPlan* p);
 void bp_plan_launch(const BPPlan* p, float* img, const void* fp, int fp_dtype, int fp_layout,
                     hipStream_t s);
+struct BPBatch;
+BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const void* const* fps,
+                         int n, int fp_dtype, int fp_layout);
+void bp_batch_destroy(BPBatch* b);
+void bp_batch_launch(const BPBatch* b, hipStream_t s);
 void bp_launch_f64(double* img, const double* fp, const arecon_bp_geom& g,
                    const double* w, hipStream_t s);
 

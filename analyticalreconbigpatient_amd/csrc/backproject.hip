@@ -128,6 +128,23 @@ __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sq
 __device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 
+// Read-only tables reached through the constant address space: loads with a
+// uniform address become scalar loads even when the pointer itself came from
+// memory (the batched launch's slab descriptors), where the compiler cannot
+// otherwise prove that the kernel's stores leave them unchanged.
+template <class P> using cptr = const __attribute__((address_space(4))) P*;
+template <class P> __device__ __forceinline__ cptr<P> const_ptr(const P* p) {
+    return (cptr<P>)p;
+}
+// Element i of a float2 / uint2 table read through the constant address space.
+__device__ __forceinline__ float2 cld_f2(cptr<float> p, long long i) {
+    return make_float2(p[2 * i], p[2 * i + 1]);
+}
+__device__ __forceinline__ uint2 cld_u2(cptr<unsigned> p, long long i) {
+    return make_uint2(p[2 * i], p[2 * i + 1]);
+}
+
+
 template <int NZ, class T>
 __global__ __launch_bounds__(256) void bp_fdk3dw(float* __restrict__ img,
                                                  const T* __restrict__ fp,
@@ -474,7 +491,7 @@ __device__ __forceinline__ unsigned zc_offset(float V1, unsigned rowb, unsigned 
 // group outside the view's window carry temp2 = 1e30 in the table, so their row
 // is off the detector and their gathers read 0.
 template <int G, class T, int LAYOUT, bool WEIGHTED>
-__device__ __forceinline__ void zc_group(const float2* __restrict__ e, float K, float zlc,
+__device__ __forceinline__ void zc_group(cptr<float> e, float K, float zlc,
                                          float B, unsigned ucolb, unsigned rowb,
                                          unsigned zlm1, __amdgpu_buffer_rsrc_t rs, float Dey,
                                          float wu, float* acc) {
@@ -483,13 +500,13 @@ __device__ __forceinline__ void zc_group(const float2* __restrict__ e, float K, 
     float fr[G];
 #pragma unroll
     for (int i = 0; i < G; ++i)
-        q[i] = Gt::load(rs, zc_offset<LAYOUT>(fmaf(e[i].x, K, zlc), rowb, ucolb, zlm1, fr[i]),
+        q[i] = Gt::load(rs, zc_offset<LAYOUT>(fmaf(e[2 * i], K, zlc), rowb, ucolb, zlm1, fr[i]),
                         rowb);
 #pragma unroll
     for (int i = 0; i < G; ++i) {
         const float tou = Gt::lerp(q[i], fr[i], Dey, wu);
         if (WEIGHTED)   // 1 / (1 + 2^Q B); exactly 1 where a = 0
-            acc[i] = fmaf(fast_rcp(fmaf(e[i].y, B, 1.f)), tou, acc[i]);
+            acc[i] = fmaf(fast_rcp(fmaf(e[2 * i + 1], B, 1.f)), tou, acc[i]);
         else
             acc[i] += tou;
     }
@@ -498,25 +515,26 @@ __device__ __forceinline__ void zc_group(const float2* __restrict__ e, float K, 
 // All ZB slices of the block for one view, each behind its own uniform window
 // test: first every active slice's gather, then every blend.
 template <int ZB, class T, int LAYOUT>
-__device__ __forceinline__ void zc_split(const float2* __restrict__ e, uint2 m, float K,
+__device__ __forceinline__ void zc_split(cptr<float> e, uint2 m, float K,
                                          float zlc, float B, unsigned ucolb, unsigned rowb,
                                          unsigned zlm1, __amdgpu_buffer_rsrc_t rs, float Dey,
                                          float wu, float* acc) {
     typedef Gather<T, LAYOUT> Gt;
-    struct EntZ { float2 e[ZB]; };
-    const EntZ ent = *reinterpret_cast<const EntZ*>(e);          // scalar loads
+    float2 ent[ZB];                                              // scalar loads
+#pragma unroll
+    for (int j = 0; j < ZB; ++j) ent[j] = cld_f2(e, j);
     typename Gt::Raw q[ZB];
     float fr[ZB];
 #pragma unroll
     for (int j = 0; j < ZB; ++j)
         if ((m.x >> j) & 1u)
-            q[j] = Gt::load(rs, zc_offset<LAYOUT>(fmaf(ent.e[j].x, K, zlc), rowb, ucolb, zlm1,
+            q[j] = Gt::load(rs, zc_offset<LAYOUT>(fmaf(ent[j].x, K, zlc), rowb, ucolb, zlm1,
                                                   fr[j]), rowb);
     if (m.y != 0u) {
 #pragma unroll
         for (int j = 0; j < ZB; ++j)
             if ((m.x >> j) & 1u)   // 1 / (1 + 2^Q B); exactly 1 where a = 0
-                acc[j] = fmaf(fast_rcp(fmaf(ent.e[j].y, B, 1.f)),
+                acc[j] = fmaf(fast_rcp(fmaf(ent[j].y, B, 1.f)),
                               Gt::lerp(q[j], fr[j], Dey, wu), acc[j]);
     } else {
 #pragma unroll
@@ -525,18 +543,47 @@ __device__ __forceinline__ void zc_split(const float2* __restrict__ e, uint2 m, 
     }
 }
 
-template <int ZB, int G, class T, int LAYOUT>
-__global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
-                                                    const T* __restrict__ fp,
-                                                    const float2* __restrict__ vtab,
-                                                    const ZBlock* __restrict__ blocks,
-                                                    const uint2* __restrict__ masks,
-                                                    const float2* __restrict__ table, BPArgs A) {
-    static_assert(G == 0 ? ZB <= 16 : (ZB % G == 0 && ZB <= 32 && (G == 1 || G == 2 || G == 4)),
-                  "bad block shape");
+// XCD-aware remap: hardware workgroup b runs on XCD b % 8; give each XCD a
+// contiguous run of logical workgroups.
+__device__ __forceinline__ int xcd_logical_block() {
     const int nb = gridDim.x, b = blockIdx.x;
     const int qx = nb >> 3, rx = nb & 7, xcd = b & 7;
-    const int lb = xcd * qx + min(xcd, rx) + (b >> 3);
+    return xcd * qx + min(xcd, rx) + (b >> 3);
+}
+
+// Values the whole workgroup shares, re-marked as such: loaded from a slab
+// descriptor in global memory they may reach the loop in VGPRs, and a buffer
+// descriptor built from VGPRs would make the compiler wrap every gather in a
+// readfirstlane loop.
+template <class P> __device__ __forceinline__ P* uniform_ptr(P* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<P*>(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ long long uniform_i64(long long x) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)x);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)x >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Body of the z-column kernel for logical workgroup lb of one plan.
+template <int ZB, int G, class T, int LAYOUT>
+__device__ __forceinline__ void zc_run(int lb, const BPArgs& A, float* __restrict__ img,
+                                       const T* __restrict__ fp,
+                                       const float2* __restrict__ vtab,
+                                       const ZBlock* __restrict__ blocks,
+                                       const uint2* __restrict__ masks,
+                                       const float2* __restrict__ table) {
+    static_assert(G == 0 ? ZB <= 16 : (ZB % G == 0 && ZB <= 32 && (G == 1 || G == 2 || G == 4)),
+                  "bad block shape");
+    fp = uniform_ptr(fp);
+    vtab = uniform_ptr(vtab);
+    blocks = uniform_ptr(blocks);
+    masks = uniform_ptr(masks);
+    table = uniform_ptr(table);
+    const long long view_stride = uniform_i64(A.view_stride);
+    const unsigned view_bytes = __builtin_amdgcn_readfirstlane(A.view_bytes);
     int zs, tile;
     if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
     else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
@@ -547,9 +594,14 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
     const float x = ((float)xi - A.XNC) * A.dx;
     const float y = ((float)yi - A.YNC) * A.dy;
     const int z0 = zs * ZB;
-    const ZBlock hb = blocks[zs];
-    const uint2* mrow = masks + hb.off / ZB - hb.pmin;
-    const float2* trow = table + hb.off - (long long)hb.pmin * ZB;
+    const cptr<int> hbp = (cptr<int>)const_ptr(blocks + zs);
+    ZBlock hb;
+    hb.pmin = hbp[0];
+    hb.pmax = hbp[1];
+    hb.off = *(cptr<long long>)(hbp + 2);
+    const cptr<unsigned> mrow = (cptr<unsigned>)const_ptr(masks + (hb.off / ZB - hb.pmin));
+    const cptr<float> trow = (cptr<float>)const_ptr(table + (hb.off - (long long)hb.pmin * ZB));
+    const cptr<float> vt = (cptr<float>)const_ptr(vtab);
     constexpr unsigned kElem = Gather<T, LAYOUT>::kElem;
     // bytes per detector row: YL elements, or YL-1 quads (one per channel pair)
     const unsigned rowb = (unsigned)(LAYOUT == ARECON_FP_QUADS ? A.YL - 1 : A.YL) * kElem;
@@ -564,10 +616,9 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
     for (int j = 0; j < ZB; ++j) acc[j] = 0.f;
 
     for (int p = hb.pmin; p <= hb.pmax; ++p) {
-        const uint2 m = mrow[p];                       // scalar: active, weighted
-        const float2 cs = vtab[p];
-        const __amdgpu_buffer_rsrc_t rs =
-            view_rsrc(fp + (long long)p * A.view_stride, A.view_bytes);
+        const uint2 m = cld_u2(mrow, p);               // scalar: active, weighted
+        const float2 cs = cld_f2(vt, p);
+        const __amdgpu_buffer_rsrc_t rs = view_rsrc(fp + (long long)p * view_stride, view_bytes);
         const float UU = -x * cs.x - y * cs.y;
         const float Yr = -x * cs.y + y * cs.x;
         const float t1sq = A.SO2 - Yr * Yr;
@@ -589,7 +640,7 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
         if (m.y != 0u)   // B = ((t1-UU)/(t1+UU))^k1, kept finite and nonzero
             B = fminf(fmaxf(fast_exp2(A.k1 * fast_log2(fabsf((temp1 - UU) * temp1 * inv))),
                             1e-30f), 1e30f);
-        const float2* erow = trow + (long long)p * ZB;
+        const cptr<float> erow = trow + 2LL * p * ZB;
         if constexpr (G == 0) {
             zc_split<ZB, T, LAYOUT>(erow, m, K, zlc, B, ucolb, rowb, zlm1, rs, Dey, wu, acc);
             continue;
@@ -599,10 +650,10 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
             if (((m.x >> g) & gmask) == 0u) continue;
             constexpr int GG = G == 0 ? 1 : G;
             if ((m.y >> g) & gmask)
-                zc_group<GG, T, LAYOUT, true>(erow + g, K, zlc, B, ucolb, rowb, zlm1, rs, Dey,
+                zc_group<GG, T, LAYOUT, true>(erow + 2 * g, K, zlc, B, ucolb, rowb, zlm1, rs, Dey,
                                               wu, acc + g);
             else
-                zc_group<GG, T, LAYOUT, false>(erow + g, K, zlc, B, ucolb, rowb, zlm1, rs, Dey,
+                zc_group<GG, T, LAYOUT, false>(erow + 2 * g, K, zlc, B, ucolb, rowb, zlm1, rs, Dey,
                                                wu, acc + g);
         }
     }
@@ -620,6 +671,43 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
                 if (z0 + j < A.ZN) o[j] = acc[j] * A.DeltaFai;
         }
     }
+}
+
+template <int ZB, int G, class T, int LAYOUT>
+__global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
+                                                    const T* __restrict__ fp,
+                                                    const float2* __restrict__ vtab,
+                                                    const ZBlock* __restrict__ blocks,
+                                                    const uint2* __restrict__ masks,
+                                                    const float2* __restrict__ table, BPArgs A) {
+    zc_run<ZB, G, T, LAYOUT>(xcd_logical_block(), A, img, fp, vtab, blocks, masks, table);
+}
+
+// Batched z-column launch: several plans (z-slabs of one or more scans, each
+// with its own geometry, volume and filtered sinogram) in one grid.  Slab s owns
+// logical workgroups [wg_begin, wg_begin + its grid); the XCD remap runs over
+// the whole grid, so each XCD walks a contiguous run across slab boundaries.
+struct SlabDesc {
+    BPArgs A;
+    float* img;
+    const void* fp;
+    const float2* vtab;
+    const ZBlock* blocks;
+    const uint2* masks;
+    const float2* table;
+    int wg_begin;
+    int pad;
+};
+
+template <int ZB, int G, class T, int LAYOUT>
+__global__ __launch_bounds__(256) void bp_fdk3dw_zc_batch(const SlabDesc* __restrict__ slabs,
+                                                          int nslabs) {
+    const int lb = xcd_logical_block();
+    int s = 0;
+    while (s + 1 < nslabs && lb >= slabs[s + 1].wg_begin) ++s;
+    const SlabDesc& d = slabs[s];
+    zc_run<ZB, G, T, LAYOUT>(lb - d.wg_begin, d.A, d.img, static_cast<const T*>(d.fp), d.vtab,
+                             d.blocks, d.masks, d.table);
 }
 
 // FP64 path: the same math as the reference, one thread per voxel, in double.
@@ -981,15 +1069,22 @@ void bp_plan_destroy(BPPlan* P) {
     delete P;
 }
 
-template <int NZ, class T>
-static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipStream_t s) {
+// Kernel arguments of one launch of plan P: the view stride of the filtered
+// layout and whether the volume takes float4 stores.
+static BPArgs launch_args(const BPPlan* P, const float* img, int layout, size_t elem) {
+    ARECON_REQUIRE(layout == ARECON_FP_ROWS || P->variant == 2,
+                   "bp: the row-pair and quad layouts need the z-column kernel (variant 2)");
     BPArgs a = P->a;
     a.vec4 = (a.ZN % 4 == 0) && ((reinterpret_cast<uintptr_t>(img) & 15) == 0);
     if (layout == ARECON_FP_ROWPAIRS) a.view_stride = 2LL * (a.ZL - 1) * a.YL;
     if (layout == ARECON_FP_QUADS) a.view_stride = 4LL * (a.ZL - 1) * (a.YL - 1);
-    a.view_bytes = (unsigned)(a.view_stride * (long long)sizeof(T));
-    ARECON_REQUIRE(layout == ARECON_FP_ROWS || P->variant == 2,
-                   "bp: the row-pair and quad layouts need the z-column kernel (variant 2)");
+    a.view_bytes = (unsigned)(a.view_stride * (long long)elem);
+    return a;
+}
+
+template <int NZ, class T>
+static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipStream_t s) {
+    const BPArgs a = launch_args(P, img, layout, sizeof(T));
     if constexpr (NZ <= 16) {
         if (P->variant == 1) {
             hipLaunchKernelGGL((bp_fdk3dw<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp,
@@ -1058,6 +1153,132 @@ void bp_plan_launch(const BPPlan* P, float* img, const void* fp, int fp_dtype, i
         launch_t<__hip_bfloat16>(P, img, static_cast<const __hip_bfloat16*>(fp), fp_layout, s);
         break;
     default: fail(ARECON_ERR_ARG, "bp: unknown fp dtype");
+    }
+    ARECON_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Batches of z-column plans bound to their volumes and filtered sinograms.
+// ---------------------------------------------------------------------------
+struct BPBatch {
+    int device = 0;
+    int nz = 0, group = 0, dtype = 0, layout = 0;
+    int nslabs = 0;
+    long long grid = 0;
+    SlabDesc* d_slabs = nullptr;
+};
+
+BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const void* const* fps,
+                         int n, int fp_dtype, int fp_layout) {
+    ARECON_REQUIRE(n > 0 && plans && imgs && fps, "bp batch: empty or null arguments");
+    ARECON_REQUIRE(fp_dtype == ARECON_F32 || fp_dtype == ARECON_F16 || fp_dtype == ARECON_BF16,
+                   "bp batch: unknown fp dtype");
+    ARECON_REQUIRE(fp_layout == ARECON_FP_ROWS || fp_layout == ARECON_FP_ROWPAIRS ||
+                       fp_layout == ARECON_FP_QUADS,
+                   "bp batch: unknown fp layout");
+    std::vector<SlabDesc> slabs(n);
+    long long wg = 0;
+    for (int i = 0; i < n; ++i) {
+        const BPPlan* P = plans[i];
+        ARECON_REQUIRE(P && imgs[i] && fps[i], "bp batch: null plan or buffer");
+        ARECON_REQUIRE(P->variant == 2, "bp batch: every plan must use the z-column kernel");
+        ARECON_REQUIRE(P->nz == plans[0]->nz && P->group == plans[0]->group &&
+                           P->device == plans[0]->device,
+                       "bp batch: plans must share nz, group and device");
+        SlabDesc d{};
+        d.A = launch_args(P, imgs[i], fp_layout, dtype_size(fp_dtype));
+        d.img = imgs[i];
+        d.fp = fps[i];
+        d.vtab = P->vtab;
+        d.blocks = P->zblocks;
+        d.masks = P->zmasks;
+        d.table = P->vt_table;
+        d.wg_begin = (int)wg;
+        wg += P->grid;
+        ARECON_REQUIRE(wg < (1LL << 31), "bp batch: too many workgroups for one launch");
+        slabs[i] = d;
+    }
+    BPBatch* B = new BPBatch();
+    B->device = plans[0]->device;
+    B->nz = plans[0]->nz;
+    B->group = plans[0]->group;
+    B->dtype = fp_dtype;
+    B->layout = fp_layout;
+    B->nslabs = n;
+    B->grid = wg;
+    int cur = 0;
+    ARECON_HIP(hipGetDevice(&cur));
+    try {
+        ARECON_HIP(hipSetDevice(B->device));
+        ARECON_HIP(hipMalloc(&B->d_slabs, sizeof(SlabDesc) * slabs.size()));
+        ARECON_HIP(hipMemcpy(B->d_slabs, slabs.data(), sizeof(SlabDesc) * slabs.size(),
+                             hipMemcpyHostToDevice));
+        ARECON_HIP(hipSetDevice(cur));
+    } catch (...) {
+        (void)hipSetDevice(cur);
+        bp_batch_destroy(B);
+        throw;
+    }
+    return B;
+}
+
+void bp_batch_destroy(BPBatch* B) {
+    if (!B) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(B->device);
+    if (B->d_slabs) (void)hipFree(B->d_slabs);
+    (void)hipSetDevice(cur);
+    delete B;
+}
+
+template <int NZ, int G, class T>
+static void batch_launch_t(const BPBatch* B, hipStream_t s) {
+    const dim3 grid((unsigned)B->grid), block(256);
+    switch (B->layout) {
+    case ARECON_FP_ROWS:
+        hipLaunchKernelGGL((bp_fdk3dw_zc_batch<NZ, G, T, ARECON_FP_ROWS>), grid, block, 0, s,
+                           B->d_slabs, B->nslabs);
+        break;
+    case ARECON_FP_ROWPAIRS:
+        hipLaunchKernelGGL((bp_fdk3dw_zc_batch<NZ, G, T, ARECON_FP_ROWPAIRS>), grid, block, 0,
+                           s, B->d_slabs, B->nslabs);
+        break;
+    default:
+        hipLaunchKernelGGL((bp_fdk3dw_zc_batch<NZ, G, T, ARECON_FP_QUADS>), grid, block, 0, s,
+                           B->d_slabs, B->nslabs);
+        break;
+    }
+}
+
+template <int NZ, int G>
+static void batch_launch_g(const BPBatch* B, hipStream_t s) {
+    switch (B->dtype) {
+    case ARECON_F32: batch_launch_t<NZ, G, float>(B, s); break;
+    case ARECON_F16: batch_launch_t<NZ, G, __half>(B, s); break;
+    default: batch_launch_t<NZ, G, __hip_bfloat16>(B, s); break;
+    }
+}
+
+template <int NZ>
+static void batch_launch_nz(const BPBatch* B, hipStream_t s) {
+    switch (B->group) {
+    case 1: batch_launch_g<NZ, 1>(B, s); break;
+    case 2: batch_launch_g<NZ, 2>(B, s); break;
+    case 4: batch_launch_g<NZ, 4>(B, s); break;
+    default:
+        if constexpr (NZ <= 16) batch_launch_g<NZ, 0>(B, s);
+        else fail(ARECON_ERR_INTERNAL, "bp batch: split mode needs nz <= 16");
+        break;
+    }
+}
+
+void bp_batch_launch(const BPBatch* B, hipStream_t s) {
+    ARECON_REQUIRE(B != nullptr, "bp batch: null batch");
+    switch (B->nz) {
+    case 8: batch_launch_nz<8>(B, s); break;
+    case 16: batch_launch_nz<16>(B, s); break;
+    default: batch_launch_nz<32>(B, s); break;
     }
     ARECON_HIP(hipGetLastError());
 }

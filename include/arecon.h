@@ -194,6 +194,20 @@ int arecon_bp_plan_nz(const arecon_bp_plan* plan);
 int arecon_bp_plan_execute(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
                            int fp_dtype, int fp_layout, void* stream);
 
+/* Batched backprojection (no counterpart in the reference, whose MEX
+ * backprojects one volume per call): n plans -- z-slabs of one or more scans,
+ * all z-column plans (variant 2) with the same nz and group -- bound to their
+ * volumes d_imgs[i] and filtered sinograms d_fps[i] (all of fp_dtype and
+ * fp_layout).  One execute is one kernel launch over every slab, with no
+ * allocation or host synchronisation, so it can be captured in a HIP graph.
+ * The buffers must outlive the batch. */
+typedef struct arecon_bp_batch arecon_bp_batch;
+int arecon_bp_batch_create(const arecon_bp_plan* const* plans, float* const* d_imgs,
+                           const void* const* d_fps, int n, int fp_dtype, int fp_layout,
+                           arecon_bp_batch** out);
+int arecon_bp_batch_destroy(arecon_bp_batch* batch);
+int arecon_bp_batch_execute(const arecon_bp_batch* batch, void* stream);
+
 /* FP64 backprojection on device buffers (same math in double throughout).
  * Synchronises `stream` before returning (w is a host buffer). */
 int arecon_bp_dev_f64(double* d_img, const double* d_fp, const arecon_bp_geom* g,
