@@ -22,6 +22,7 @@ _d = C.c_double
 _i = C.c_int
 _p = C.c_void_p
 _fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)   # host int array (volume dims)
 _dp = C.POINTER(C.c_double)
 
 
@@ -71,6 +72,21 @@ SIGNATURES = {
                                     C.POINTER(_p)]),
     "arecon_bp_batch_destroy": (_i, [_p]),
     "arecon_bp_batch_execute": (_i, [_p, _p]),
+    # include/arecon_cnn.h
+    "arecon_cnn_conv3": (_i, [_p, _p, _p, _p, _p, _ip, _i, _i, _i, _p]),
+    "arecon_cnn_conv3_wgrad": (_i, [_p, _p, _p, _ip, _i, _i, _p]),
+    "arecon_cnn_relu_bwd": (_i, [_p, _p, _p, _p, _p, _ip, _i, _i, _p]),
+    "arecon_cnn_channel_major3": (_i, [_p, _p, _ip, _i, _p]),
+    "arecon_cnn_pool": (_i, [_p, _p, _ip, _i, _p]),
+    "arecon_cnn_pool_bwd": (_i, [_p, _p, _ip, _i, _i, _p]),
+    "arecon_cnn_upsample": (_i, [_p, _p, _ip, _i, _p]),
+    "arecon_cnn_upsample_bwd": (_i, [_p, _p, _ip, _i, _p]),
+    "arecon_cnn_add": (_i, [_p, _p, C.c_longlong, _p]),
+    "arecon_cnn_pack_weights": (_i, [_p, _p, _p, _i, _i, _p]),
+    "arecon_cnn_mse_residual": (_i, [_p, _p, _p, _i, _p, _p, C.c_longlong, _p]),
+    "arecon_cnn_adam": (_i, [_p, _p, _p, _p, C.c_longlong, _f, _f, _f, _f, _i, _p]),
+    "arecon_cnn_to_bf16": (_i, [_p, _p, C.c_longlong, _i, _p]),
+    "arecon_cnn_residual_out": (_i, [_p, _p, _i, _p, C.c_longlong, _p]),
     "arecon_bp_dev_f64": (_i, [_p, _p, C.POINTER(BPGeom), _dp, _p]),
     "arecon_fdk_weights": (_i, [_dp, _i, _d, _i]),
     "arecon_project_ellipsoids_dev": (_i, [_p, _fp, _i, _i, _i, _i, _d, _d, _d, _d, _d, _d,

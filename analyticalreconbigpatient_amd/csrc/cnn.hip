@@ -1,0 +1,617 @@
+// Kernels of the learned artifact-correction network (3-D residual
+// encoder-decoder, 3x3x3 convolutions): BF16 activations and weights on the
+// gfx950 matrix cores (v_mfma_f32_16x16x32_bf16), FP32 accumulation.  No
+// counterpart in the reference (it has no learned part).
+//
+// Layouts.
+//  * Activations: channels-last BF16, [B][D][H][W][C] with C a multiple of 8
+//    (the one-channel input is padded to 8 with zeros).
+//  * Forward weights: BF16 [Cout][Kp], k = tap * Cin + ci, tap = 9 (dz+1) +
+//    3 (dy+1) + (dx+1), Kp = 27 Cin rounded up to 32 (zero tail).
+//  * Weight gradients: FP32 [Cout][27][Cin].
+//
+// MFMA 16x16x32 BF16 lane layout (checked on the device): A (16 x 32): lane l
+// holds A[l%16][8(l/16) .. +8]; B (32 x 16): lane l holds B[8(l/16) .. +8][l%16];
+// D (16 x 16, FP32): lane l holds D[4(l/16) + v][l%16], v = 0..3.
+//
+// Convolution as implicit GEMM: out[co][n] = sum_k W[co][k] X[nb(n, tap(k))][ci(k)].
+// A lane's 8 consecutive k are 8 consecutive channels of one tap, i.e. one
+// 16-byte load from the channels-last tensor (zero outside the volume).  The
+// data gradient is the same convolution with flipped taps and transposed
+// weights (packed that way by pack_weights, so it runs as a plain forward
+// convolution); the weight gradient reduces over voxels, for which the activations
+// and output gradients are first rewritten channel-major (the input also in
+// three copies shifted by dx = -1, 0, +1, so every 8-voxel run is one aligned
+// 16-byte load).
+#include "arecon_internal.h"
+
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace arecon {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+struct Vol {            // batch of volumes, channels-last
+    int B, D, H, W;
+    __host__ __device__ long long nvox() const { return (long long)B * D * H * W; }
+};
+
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ u16 f2bf(float f) {            // round to nearest even
+    const unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (u16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    return (u16)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ bf16x8 zero8() {
+    return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
+}
+__device__ __forceinline__ bf16x8 load8(const u16* p) {
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+
+// ---------------------------------------------------------------------------
+// Forward convolution (also the data gradient).  A wavefront computes 32
+// voxels (two 16-voxel tiles) x all output channels (MT tiles of 16); a
+// workgroup is 4 wavefronts.  Epilogue: + bias, + skip (pre-activation), ReLU.
+// ---------------------------------------------------------------------------
+template <int MT>
+__global__ __launch_bounds__(256) void conv3_fwd(u16* __restrict__ y, const u16* __restrict__ x,
+                                                 const u16* __restrict__ w,
+                                                 const float* __restrict__ bias,
+                                                 const u16* __restrict__ skip, Vol v, int cin,
+                                                 int cout, int kp, int relu) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long nv = v.nvox();
+    const long long n0 = ((long long)blockIdx.x * 4 + wave) * 32;
+    if (n0 >= nv) return;
+    const int col = lane & 15, kg = lane >> 4;
+    // the two voxels this lane feeds as B columns
+    int zc[2], yc[2], xc[2];
+    long long nb0[2];
+    bool inb[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const long long n = n0 + 16 * u + col;
+        inb[u] = n < nv;
+        const long long nn = inb[u] ? n : 0;
+        xc[u] = (int)(nn % v.W);
+        const long long t = nn / v.W;
+        yc[u] = (int)(t % v.H);
+        const long long t2 = t / v.H;
+        zc[u] = (int)(t2 % v.D);
+        nb0[u] = nn;
+    }
+    f32x4 acc[MT][2];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int nchunks = kp / 32;
+    for (int c = 0; c < nchunks; ++c) {
+        const int k0 = c * 32 + kg * 8;
+        const int tap = k0 / cin, ci0 = k0 - tap * cin;
+        bf16x8 b[2];
+        if (tap < 27) {
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int zz = zc[u] + dz, yy = yc[u] + dy, xx = xc[u] + dx;
+                const bool ok = inb[u] && zz >= 0 && zz < v.D && yy >= 0 && yy < v.H &&
+                                xx >= 0 && xx < v.W;
+                const long long nbr = nb0[u] + ((long long)dz * v.H + dy) * v.W + dx;
+                b[u] = ok ? load8(x + nbr * cin + ci0) : zero8();
+            }
+        } else {
+            b[0] = b[1] = zero8();
+        }
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+            const bf16x8 a = load8(w + (long long)(16 * t + col) * kp + k0);
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[u], acc[t][u], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const long long n = n0 + 16 * u + col;
+        if (n >= nv) continue;
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+            const int co0 = 16 * t + 4 * kg;
+            if (co0 >= cout) continue;
+            u16 o[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float r = acc[t][u][q] + (bias ? bias[co0 + q] : 0.f);
+                if (skip) r += bf2f(skip[n * cout + co0 + q]);
+                if (relu) r = fmaxf(r, 0.f);
+                o[q] = f2bf(r);
+            }
+            *reinterpret_cast<uint2*>(y + n * cout + co0) =
+                make_uint2((unsigned)o[0] | ((unsigned)o[1] << 16),
+                           (unsigned)o[2] | ((unsigned)o[3] << 16));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dW[co][tap][ci] += sum_n dY[co][n] X_dx[ci][nb(n, dz, dy)],
+// with dY (already masked by the ReLU derivative) and X channel-major.  One
+// wavefront: 16 co x 16 ci of one tap over a range of voxels; the K dimension
+// of the MFMA is 32 voxels of one row segment (W is a multiple of 8, so each
+// lane's 8 voxels lie in one row).  Partial sums are added atomically.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void conv3_wgrad(float* __restrict__ dw,
+                                                   const u16* __restrict__ dyT,
+                                                   const u16* __restrict__ xT3, Vol v, int cin,
+                                                   int cout, long long vox_per_split,
+                                                   int nsplit) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ct = (cout + 15) / 16, it = (cin + 15) / 16;
+    long long job = (long long)blockIdx.x * 4 + wave;
+    const long long njobs = 27LL * ct * it * nsplit;
+    if (job >= njobs) return;
+    const int split = (int)(job % nsplit);
+    job /= nsplit;
+    const int ti = (int)(job % it);
+    job /= it;
+    const int tc = (int)(job % ct);
+    const int tap = (int)(job / ct);
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const long long nv = v.nvox();
+    const long long nbeg = split * vox_per_split;
+    const long long nend = std::min(nv, nbeg + vox_per_split);
+    const int col = lane & 15, kg = lane >> 4;
+    const int co = 16 * tc + col, ci = 16 * ti + col;
+    const u16* xs = xT3 + (long long)(dx + 1) * cin * nv;    // copy shifted by dx
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (long long n = nbeg; n < nend; n += 32) {
+        const long long m = n + 8 * kg;                       // this lane's 8 voxels
+        bf16x8 a = zero8(), b = zero8();
+        if (m < nend) {
+            if (co < cout) a = load8(dyT + (long long)co * nv + m);
+            if (ci < cin) {
+                const long long t = m / v.W;
+                const int yy = (int)(t % v.H) + dy;
+                const int zz = (int)((t / v.H) % v.D) + dz;
+                if (yy >= 0 && yy < v.H && zz >= 0 && zz < v.D)
+                    b = load8(xs + (long long)ci * nv + m + ((long long)dz * v.H + dy) * v.W);
+            }
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    // D[i = co in tile][j = ci in tile]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c_o = 16 * tc + 4 * kg + q, c_i = 16 * ti + col;
+        if (c_o < cout && c_i < cin)
+            atomicAdd(dw + ((long long)c_o * 27 + tap) * cin + c_i, acc[q]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise and layout kernels.
+// ---------------------------------------------------------------------------
+#define GRID_STRIDE(i, n) \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
+         i += (long long)gridDim.x * blockDim.x)
+
+// dYp = dY * (Y > 0) (or dY when relu == 0), channels-last, and its
+// channel-major copy.
+__global__ void relu_bwd(u16* __restrict__ dyp, u16* __restrict__ dypT,
+                         const u16* __restrict__ dy, const u16* __restrict__ y, long long nv,
+                         int c, int relu) {
+    GRID_STRIDE(i, nv * c) {
+        const long long n = i / c;
+        const int ch = (int)(i - n * c);
+        u16 g = dy[i];
+        if (relu && !(bf2f(y[i]) > 0.f)) g = 0;
+        dyp[i] = g;
+        dypT[(long long)ch * nv + n] = g;
+    }
+}
+
+// Bias gradient: db[ch] += sum_n dYT[ch][n]; blockIdx.y = channel, blocks
+// along x split the voxels; one atomic per block.
+__global__ __launch_bounds__(256) void channel_sum(float* __restrict__ db,
+                                                   const u16* __restrict__ dyT, long long nv) {
+    const int ch = blockIdx.y;
+    float s = 0.f;
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nv;
+         n += (long long)gridDim.x * blockDim.x)
+        s += bf2f(dyT[(long long)ch * nv + n]);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    __shared__ float part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(db + ch, part[0] + part[1] + part[2] + part[3]);
+}
+
+// X channels-last -> three channel-major copies shifted by dx = -1, 0, +1
+// (zero where x + dx leaves the row).
+__global__ void to_channel_major3(u16* __restrict__ xT3, const u16* __restrict__ x, Vol v,
+                                  int c) {
+    const long long nv = v.nvox();
+    GRID_STRIDE(i, nv * c) {
+        const int ch = (int)(i / nv);
+        const long long n = i - (long long)ch * nv;
+        const int xx = (int)(n % v.W);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const int dx = s - 1;
+            const bool ok = xx + dx >= 0 && xx + dx < v.W;
+            xT3[(long long)s * c * nv + i] = ok ? x[(n + dx) * c + ch] : (u16)0;
+        }
+    }
+}
+
+// 2x2x2 average pooling and its adjoint; nearest 2x upsampling and its adjoint.
+__global__ void avgpool2(u16* __restrict__ y, const u16* __restrict__ x, Vol vo, int c) {
+    GRID_STRIDE(i, vo.nvox() * c) {
+        const int ch = (int)(i % c);
+        long long n = i / c;
+        const int xo = (int)(n % vo.W); n /= vo.W;
+        const int yo = (int)(n % vo.H); n /= vo.H;
+        const int zo = (int)(n % vo.D);
+        const long long b = n / vo.D;
+        const int Wi = 2 * vo.W, Hi = 2 * vo.H, Di = 2 * vo.D;
+        float s = 0.f;
+        for (int d = 0; d < 8; ++d) {
+            const long long ni = ((b * Di + 2 * zo + (d >> 2)) * Hi + 2 * yo + ((d >> 1) & 1)) * Wi +
+                                 2 * xo + (d & 1);
+            s += bf2f(x[ni * c + ch]);
+        }
+        y[i] = f2bf(0.125f * s);
+    }
+}
+__global__ void avgpool2_bwd(u16* __restrict__ dx, const u16* __restrict__ dy, Vol vi, int c,
+                             int accumulate) {
+    GRID_STRIDE(i, vi.nvox() * c) {
+        const int ch = (int)(i % c);
+        long long n = i / c;
+        const int xi = (int)(n % vi.W); n /= vi.W;
+        const int yi = (int)(n % vi.H); n /= vi.H;
+        const int zi = (int)(n % vi.D);
+        const long long b = n / vi.D;
+        const long long no = ((b * (vi.D / 2) + zi / 2) * (vi.H / 2) + yi / 2) * (vi.W / 2) + xi / 2;
+        float g = 0.125f * bf2f(dy[no * c + ch]);
+        if (accumulate) g += bf2f(dx[i]);
+        dx[i] = f2bf(g);
+    }
+}
+__global__ void upsample2(u16* __restrict__ y, const u16* __restrict__ x, Vol vo, int c) {
+    GRID_STRIDE(i, vo.nvox() * c) {
+        const int ch = (int)(i % c);
+        long long n = i / c;
+        const int xo = (int)(n % vo.W); n /= vo.W;
+        const int yo = (int)(n % vo.H); n /= vo.H;
+        const int zo = (int)(n % vo.D);
+        const long long b = n / vo.D;
+        const long long ni = ((b * (vo.D / 2) + zo / 2) * (vo.H / 2) + yo / 2) * (vo.W / 2) + xo / 2;
+        y[i] = x[ni * c + ch];
+    }
+}
+__global__ void upsample2_bwd(u16* __restrict__ dx, const u16* __restrict__ dy, Vol vi, int c) {
+    GRID_STRIDE(i, vi.nvox() * c) {
+        const int ch = (int)(i % c);
+        long long n = i / c;
+        const int xi = (int)(n % vi.W); n /= vi.W;
+        const int yi = (int)(n % vi.H); n /= vi.H;
+        const int zi = (int)(n % vi.D);
+        const long long b = n / vi.D;
+        const int Wo = 2 * vi.W, Ho = 2 * vi.H, Do = 2 * vi.D;
+        float s = 0.f;
+        for (int d = 0; d < 8; ++d) {
+            const long long no = ((b * Do + 2 * zi + (d >> 2)) * Ho + 2 * yi + ((d >> 1) & 1)) * Wo +
+                                 2 * xi + (d & 1);
+            s += bf2f(dy[no * c + ch]);
+        }
+        dx[i] = f2bf(s);
+    }
+}
+
+// dst += src (BF16, elementwise)
+__global__ void add_bf16(u16* __restrict__ dst, const u16* __restrict__ src, long long n) {
+    GRID_STRIDE(i, n) dst[i] = f2bf(bf2f(dst[i]) + bf2f(src[i]));
+}
+
+// FP32 [Cout][27][Cin] master weights -> BF16 forward layout [Cout_p][Kp] and
+// the flipped, transposed data-gradient layout [Cin_p][Kp2] (k = tap' Cout + co,
+// tap' = 26 - tap); padding rows / tails are zero.
+__global__ void pack_weights(u16* __restrict__ wf, u16* __restrict__ wd,
+                             const float* __restrict__ w, int cout, int cin, int cout_p,
+                             int cin_p, int kp, int kp2) {
+    GRID_STRIDE(i, (long long)cout_p * kp) {
+        const int co = (int)(i / kp), k = (int)(i - (long long)co * kp);
+        const int tap = k / cin, ci = k - tap * cin;
+        wf[i] = (co < cout && tap < 27) ? f2bf(w[((long long)co * 27 + tap) * cin + ci]) : (u16)0;
+    }
+    if (wd) {
+        GRID_STRIDE(i, (long long)cin_p * kp2) {
+            const int ci = (int)(i / kp2), k = (int)(i - (long long)ci * kp2);
+            const int tp = k / cout, co = k - tp * cout;
+            wd[i] = (ci < cin && tp < 27)
+                        ? f2bf(w[((long long)co * 27 + (26 - tp)) * cin + ci]) : (u16)0;
+        }
+    }
+}
+
+// Residual output and loss gradient: pred = x0 + out[ch 0] (x0 in FP32);
+// dout[ch 0] = 2 (pred - target) / N (other channels 0); per-wavefront partial
+// sums of the squared error / N go to loss[0] atomically.
+__global__ void mse_residual(u16* __restrict__ dout, float* __restrict__ loss,
+                             const u16* __restrict__ out, int cout, const float* __restrict__ x0,
+                             const float* __restrict__ target, long long nv, float scale) {
+    float local = 0.f;
+    GRID_STRIDE(n, nv) {
+        const float pred = x0[n] + bf2f(out[n * cout]);
+        const float e = pred - target[n];
+        local += e * e;
+        dout[n * cout] = f2bf(2.f * e * scale);
+        for (int ch = 1; ch < cout; ++ch) dout[n * cout + ch] = 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, local * scale);
+}
+
+// Adam on FP32 master weights.
+__global__ void adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v2,
+                     const float* __restrict__ g, long long n, float lr, float b1, float b2,
+                     float eps, float c1, float c2) {
+    GRID_STRIDE(i, n) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v2[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v2[i] = vi;
+        w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    }
+}
+
+// FP32 -> BF16 channels-last input with the channel padded to c (zeros), and
+// BF16 channel 0 -> FP32.
+__global__ void to_bf16_padded(u16* __restrict__ y, const float* __restrict__ x, long long nv,
+                               int c) {
+    GRID_STRIDE(i, nv * c) {
+        const long long n = i / c;
+        y[i] = (i - n * c) == 0 ? f2bf(x[n]) : (u16)0;
+    }
+}
+__global__ void residual_out(float* __restrict__ y, const u16* __restrict__ out, int cout,
+                             const float* __restrict__ x0, long long nv) {
+    GRID_STRIDE(n, nv) y[n] = x0[n] + bf2f(out[n * cout]);
+}
+
+unsigned grid_for(long long n, int per = 256) {
+    return (unsigned)std::max<long long>(1, std::min<long long>((n + per - 1) / per, 1 << 20));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers (device pointers, stream-ordered).
+// ---------------------------------------------------------------------------
+static Vol mkvol(const int* dims) {
+    ARECON_REQUIRE(dims && dims[0] > 0 && dims[1] > 0 && dims[2] > 0 && dims[3] > 0,
+                   "cnn: volume dimensions must be positive");
+    return Vol{dims[0], dims[1], dims[2], dims[3]};
+}
+
+void cnn_conv3_fwd(void* y, const void* x, const void* w, const float* bias, const void* skip,
+                   const int* dims, int cin, int cout, int relu, hipStream_t s) {
+    const Vol v = mkvol(dims);
+    ARECON_REQUIRE(cin % 8 == 0 && cin >= 8, "cnn: input channels must be a multiple of 8");
+    ARECON_REQUIRE(cout % 4 == 0 && cout >= 4 && cout <= 128,
+                   "cnn: output channels must be a multiple of 4 in [4, 128]");
+    const int kp = (27 * cin + 31) / 32 * 32;
+    const int mt = (cout + 15) / 16;
+    const long long waves = (v.nvox() + 31) / 32;
+    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+    auto* Y = static_cast<u16*>(y);
+    auto* X = static_cast<const u16*>(x);
+    auto* Wt = static_cast<const u16*>(w);
+    auto* S = static_cast<const u16*>(skip);
+    switch (mt) {
+    case 1: hipLaunchKernelGGL(conv3_fwd<1>, grid, block, 0, s, Y, X, Wt, bias, S, v, cin, cout, kp, relu); break;
+    case 2: hipLaunchKernelGGL(conv3_fwd<2>, grid, block, 0, s, Y, X, Wt, bias, S, v, cin, cout, kp, relu); break;
+    case 4: hipLaunchKernelGGL(conv3_fwd<4>, grid, block, 0, s, Y, X, Wt, bias, S, v, cin, cout, kp, relu); break;
+    case 8: hipLaunchKernelGGL(conv3_fwd<8>, grid, block, 0, s, Y, X, Wt, bias, S, v, cin, cout, kp, relu); break;
+    default: fail(ARECON_ERR_ARG, "cnn: output channels must be 16, 32, 64 or 128 (or <= 16)");
+    }
+    ARECON_HIP(hipGetLastError());
+}
+
+void cnn_conv3_wgrad(float* dw, const void* dyT, const void* xT3, const int* dims, int cin,
+                     int cout, hipStream_t s) {
+    const Vol v = mkvol(dims);
+    ARECON_REQUIRE(v.W % 8 == 0, "cnn: weight gradient needs W to be a multiple of 8");
+    ARECON_REQUIRE(cin % 8 == 0 && cout % 4 == 0, "cnn: channel counts");
+    const long long nv = v.nvox();
+    const int ct = (cout + 15) / 16, it = (cin + 15) / 16;
+    const long long tiles = 27LL * ct * it;
+    // enough waves to fill the GPU: split the voxel range (multiples of 32)
+    long long nsplit = std::max<long long>(1, std::min<long long>((nv + 2047) / 2048,
+                                                                  (8192 + tiles - 1) / tiles));
+    long long per = (nv + nsplit - 1) / nsplit;
+    per = (per + 31) / 32 * 32;
+    nsplit = (nv + per - 1) / per;
+    const long long waves = tiles * nsplit;
+    hipLaunchKernelGGL(conv3_wgrad, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, dw,
+                       static_cast<const u16*>(dyT), static_cast<const u16*>(xT3), v, cin, cout,
+                       per, (int)nsplit);
+    ARECON_HIP(hipGetLastError());
+}
+
+void cnn_relu_bwd(void* dyp, void* dypT, float* db, const void* dy, const void* y,
+                  const int* dims, int c, int relu, hipStream_t s) {
+    const Vol v = mkvol(dims);
+    hipLaunchKernelGGL(relu_bwd, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+                       static_cast<u16*>(dyp), static_cast<u16*>(dypT),
+                       static_cast<const u16*>(dy), static_cast<const u16*>(y), v.nvox(), c, relu);
+    ARECON_HIP(hipGetLastError());
+    if (db) {
+        const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>(
+                                                                 (v.nvox() + 4095) / 4096, 256));
+        hipLaunchKernelGGL(channel_sum, dim3(bx, (unsigned)c), dim3(256), 0, s, db,
+                           static_cast<const u16*>(dypT), v.nvox());
+        ARECON_HIP(hipGetLastError());
+    }
+}
+
+void cnn_channel_major3(void* xT3, const void* x, const int* dims, int c, hipStream_t s) {
+    const Vol v = mkvol(dims);
+    hipLaunchKernelGGL(to_channel_major3, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+                       static_cast<u16*>(xT3), static_cast<const u16*>(x), v, c);
+    ARECON_HIP(hipGetLastError());
+}
+
+void cnn_pool(void* y, const void* x, const int* dims_out, int c, hipStream_t s) {
+    const Vol v = mkvol(dims_out);
+    hipLaunchKernelGGL(avgpool2, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+                       static_cast<u16*>(y), static_cast<const u16*>(x), v, c);
+    ARECON_HIP(hipGetLastError());
+}
+void cnn_pool_bwd(void* dx, const void* dy, const int* dims_in, int c, int accumulate,
+                  hipStream_t s) {
+    const Vol v = mkvol(dims_in);
+    ARECON_REQUIRE(v.D % 2 == 0 && v.H % 2 == 0 && v.W % 2 == 0, "cnn: pooling needs even dims");
+    hipLaunchKernelGGL(avgpool2_bwd, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+                       static_cast<u16*>(dx), static_cast<const u16*>(dy), v, c, accumulate);
+    ARECON_HIP(hipGetLastError());
+}
+void cnn_upsample(void* y, const void* x, const int* dims_out, int c, hipStream_t s) {
+    const Vol v = mkvol(dims_out);
+    hipLaunchKernelGGL(upsample2, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+                       static_cast<u16*>(y), static_cast<const u16*>(x), v, c);
+    ARECON_HIP(hipGetLastError());
+}
+void cnn_upsample_bwd(void* dx, const void* dy, const int* dims_in, int c, hipStream_t s) {
+    const Vol v = mkvol(dims_in);
+    hipLaunchKernelGGL(upsample2_bwd, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+                       static_cast<u16*>(dx), static_cast<const u16*>(dy), v, c);
+    ARECON_HIP(hipGetLastError());
+}
+void cnn_add(void* dst, const void* src, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(add_bf16, dim3(grid_for(n)), dim3(256), 0, s, static_cast<u16*>(dst),
+                       static_cast<const u16*>(src), n);
+    ARECON_HIP(hipGetLastError());
+}
+void cnn_pack_weights(void* wf, void* wd, const float* w, int cout, int cin, hipStream_t s) {
+    ARECON_REQUIRE(cin % 8 == 0 && cout % 4 == 0, "cnn: channel counts");
+    const int cout_p = (cout + 15) / 16 * 16, cin_p = (cin + 15) / 16 * 16;
+    const int kp = (27 * cin + 31) / 32 * 32, kp2 = (27 * cout + 31) / 32 * 32;
+    hipLaunchKernelGGL(pack_weights, dim3(grid_for((long long)std::max(cout_p * kp, cin_p * kp2))),
+                       dim3(256), 0, s, static_cast<u16*>(wf), static_cast<u16*>(wd), w, cout,
+                       cin, cout_p, cin_p, kp, kp2);
+    ARECON_HIP(hipGetLastError());
+}
+void cnn_mse_residual(void* dout, float* loss, const void* out, int cout, const float* x0,
+                      const float* target, long long nv, hipStream_t s) {
+    hipLaunchKernelGGL(mse_residual, dim3(grid_for(nv)), dim3(256), 0, s,
+                       static_cast<u16*>(dout), loss, static_cast<const u16*>(out), cout, x0,
+                       target, nv, 1.f / (float)nv);
+    ARECON_HIP(hipGetLastError());
+}
+void cnn_adam(float* w, float* m, float* v, const float* g, long long n, float lr, float b1,
+              float b2, float eps, int step, hipStream_t s) {
+    ARECON_REQUIRE(step >= 1, "cnn: adam step counts from 1");
+    const float c1 = 1.f / (1.f - std::pow(b1, (float)step));
+    const float c2 = 1.f / (1.f - std::pow(b2, (float)step));
+    hipLaunchKernelGGL(adam, dim3(grid_for(n)), dim3(256), 0, s, w, m, v, g, n, lr, b1, b2, eps,
+                       c1, c2);
+    ARECON_HIP(hipGetLastError());
+}
+void cnn_to_bf16(void* y, const float* x, long long nv, int c, hipStream_t s) {
+    hipLaunchKernelGGL(to_bf16_padded, dim3(grid_for(nv * c)), dim3(256), 0, s,
+                       static_cast<u16*>(y), x, nv, c);
+    ARECON_HIP(hipGetLastError());
+}
+void cnn_residual_out(float* y, const void* out, int cout, const float* x0, long long nv,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(residual_out, dim3(grid_for(nv)), dim3(256), 0, s, y,
+                       static_cast<const u16*>(out), cout, x0, nv);
+    ARECON_HIP(hipGetLastError());
+}
+
+}  // namespace arecon
+
+// ---------------------------------------------------------------------------
+// C ABI (include/arecon_cnn.h).
+// ---------------------------------------------------------------------------
+extern "C" {
+
+using namespace arecon;
+
+int arecon_cnn_conv3(void* y, const void* x, const void* w, const float* bias, const void* skip,
+                     const int* dims, int cin, int cout, int relu, void* stream) {
+    return guarded([&] {
+        cnn_conv3_fwd(y, x, w, bias, skip, dims, cin, cout, relu,
+                      static_cast<hipStream_t>(stream));
+    });
+}
+int arecon_cnn_conv3_wgrad(float* dw, const void* dyT, const void* xT3, const int* dims,
+                           int cin, int cout, void* stream) {
+    return guarded([&] {
+        cnn_conv3_wgrad(dw, dyT, xT3, dims, cin, cout, static_cast<hipStream_t>(stream));
+    });
+}
+int arecon_cnn_relu_bwd(void* dyp, void* dypT, float* db, const void* dy, const void* y,
+                        const int* dims, int c, int relu, void* stream) {
+    return guarded([&] {
+        cnn_relu_bwd(dyp, dypT, db, dy, y, dims, c, relu, static_cast<hipStream_t>(stream));
+    });
+}
+int arecon_cnn_channel_major3(void* xT3, const void* x, const int* dims, int c, void* stream) {
+    return guarded([&] { cnn_channel_major3(xT3, x, dims, c, static_cast<hipStream_t>(stream)); });
+}
+int arecon_cnn_pool(void* y, const void* x, const int* dims_out, int c, void* stream) {
+    return guarded([&] { cnn_pool(y, x, dims_out, c, static_cast<hipStream_t>(stream)); });
+}
+int arecon_cnn_pool_bwd(void* dx, const void* dy, const int* dims_in, int c, int accumulate,
+                        void* stream) {
+    return guarded([&] {
+        cnn_pool_bwd(dx, dy, dims_in, c, accumulate, static_cast<hipStream_t>(stream));
+    });
+}
+int arecon_cnn_upsample(void* y, const void* x, const int* dims_out, int c, void* stream) {
+    return guarded([&] { cnn_upsample(y, x, dims_out, c, static_cast<hipStream_t>(stream)); });
+}
+int arecon_cnn_upsample_bwd(void* dx, const void* dy, const int* dims_in, int c, void* stream) {
+    return guarded([&] { cnn_upsample_bwd(dx, dy, dims_in, c, static_cast<hipStream_t>(stream)); });
+}
+int arecon_cnn_add(void* dst, const void* src, long long n, void* stream) {
+    return guarded([&] { cnn_add(dst, src, n, static_cast<hipStream_t>(stream)); });
+}
+int arecon_cnn_pack_weights(void* wf, void* wd, const float* w, int cout, int cin, void* stream) {
+    return guarded([&] { cnn_pack_weights(wf, wd, w, cout, cin, static_cast<hipStream_t>(stream)); });
+}
+int arecon_cnn_mse_residual(void* dout, float* loss, const void* out, int cout, const float* x0,
+                            const float* target, long long nv, void* stream) {
+    return guarded([&] {
+        cnn_mse_residual(dout, loss, out, cout, x0, target, nv, static_cast<hipStream_t>(stream));
+    });
+}
+int arecon_cnn_adam(float* w, float* m, float* v, const float* g, long long n, float lr, float b1,
+                    float b2, float eps, int step, void* stream) {
+    return guarded([&] {
+        cnn_adam(w, m, v, g, n, lr, b1, b2, eps, step, static_cast<hipStream_t>(stream));
+    });
+}
+int arecon_cnn_to_bf16(void* y, const float* x, long long nv, int c, void* stream) {
+    return guarded([&] { cnn_to_bf16(y, x, nv, c, static_cast<hipStream_t>(stream)); });
+}
+int arecon_cnn_residual_out(float* y, const void* out, int cout, const float* x0, long long nv,
+                            void* stream) {
+    return guarded([&] { cnn_residual_out(y, out, cout, x0, nv, static_cast<hipStream_t>(stream)); });
+}
+
+}  // extern "C"

@@ -1,4 +1,4 @@
-"""The C ABI library loads and exports every entry point include/arecon.h
+"""The C ABI library loads and exports every entry point include/*.h
 declares; host-only logic works without a GPU (no compute calls here)."""
 import ctypes as C
 import re
@@ -10,13 +10,15 @@ import pytest
 import oracle
 from analyticalreconbigpatient_amd import _lib, geometry as G, mexapi
 
-HEADER = Path(__file__).resolve().parent.parent / "include" / "arecon.h"
+HEADERS = sorted((Path(__file__).resolve().parent.parent / "include").glob("*.h"))
 
 
 def declared_functions():
-    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    names = re.findall(r"\b(?:void|int)\s+(\w+)\s*\(", text)
-    return sorted(set(names))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names.update(re.findall(r"\b(?:void|int)\s+(\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_reference_entry_points():

