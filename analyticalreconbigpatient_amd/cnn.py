@@ -269,18 +269,20 @@ class Net:
 
     def correct(self, x0: torch.Tensor, bound: float = 0.3, block: int = 64):
         """Corrected volume with the plain-FDK fallback: blocks of block^3 voxels
-        whose correction RMS exceeds ``bound`` keep x0.  Returns (volume,
-        fraction of blocks that fell back)."""
+        (the last ones partial) whose correction RMS exceeds ``bound`` keep x0.
+        Returns (volume, fraction of blocks that fell back)."""
         y = self.apply(x0)
         c = y - x0
         B, D, H, W = x0.shape
-        bd, bh, bw = min(block, D), min(block, H), min(block, W)
-        cb = c[:, :D // bd * bd, :H // bh * bh, :W // bw * bw].reshape(
-            B, D // bd, bd, H // bh, bh, W // bw, bw)
-        rms = cb.pow(2).mean(dim=(2, 4, 6)).sqrt()
+        nb = [-(-n // block) for n in (D, H, W)]
+        pad = (0, nb[2] * block - W, 0, nb[1] * block - H, 0, nb[0] * block - D)
+        sq = torch.nn.functional.pad(c * c, pad)
+        ones = torch.nn.functional.pad(torch.ones_like(c), pad)
+        shape = (B, nb[0], block, nb[1], block, nb[2], block)
+        rms = (sq.reshape(shape).sum(dim=(2, 4, 6)) / ones.reshape(shape).sum(dim=(2, 4, 6))).sqrt()
         bad = rms > bound
         if bad.any():
-            mask = bad.repeat_interleave(bd, 1).repeat_interleave(bh, 2).repeat_interleave(bw, 3)
-            region = y[:, :mask.shape[1], :mask.shape[2], :mask.shape[3]]
-            region[mask] = x0[:, :mask.shape[1], :mask.shape[2], :mask.shape[3]][mask]
+            mask = (bad.repeat_interleave(block, 1).repeat_interleave(block, 2)
+                    .repeat_interleave(block, 3))[:, :D, :H, :W]
+            y[mask] = x0[mask]
         return y, float(bad.float().mean())
