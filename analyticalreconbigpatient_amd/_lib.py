@@ -64,10 +64,14 @@ SIGNATURES = {
     "arecon_device_count": (_i, []),
     "arecon_rebin_dev": (_i, [_p, _p, _i, _i, _i, _f, _f, _f, _f, _f, _f, _f, _i, _p]),
     "arecon_filter_dev": (_i, [_p, _i, _i, _p, _i, _i, _i, _f, _f, _f, _f, _f, _p]),
+    "arecon_rebin_views_dev": (_i, [_p, _p, _i, _i, _i, _f, _f, _f, _f, _f, _f, _f, _i, _i, _p]),
     "arecon_bp_plan_create": (_i, [C.POINTER(BPGeom), _dp, C.POINTER(_p)]),
     "arecon_bp_plan_destroy": (_i, [_p]),
     "arecon_bp_plan_nz": (_i, [_p]),
     "arecon_bp_plan_execute": (_i, [_p, _p, _p, _i, _i, _p]),
+    "arecon_bp_plan_nblocks": (_i, [_p]),
+    "arecon_bp_plan_block_views": (_i, [_p, _i, C.POINTER(_i), C.POINTER(_i)]),
+    "arecon_bp_plan_execute_blocks": (_i, [_p, _p, _p, _i, _i, _i, _i, _p]),
     "arecon_bp_batch_create": (_i, [C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), _i, _i, _i,
                                     C.POINTER(_p)]),
     "arecon_bp_batch_destroy": (_i, [_p]),

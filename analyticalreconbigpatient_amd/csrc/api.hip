@@ -294,6 +294,17 @@ int arecon_rebin_dev(float* d_out, const float* d_in, int YL, int ZL, int ViewN,
     });
 }
 
+int arecon_rebin_views_dev(float* d_out, const float* d_in, int YL, int ZL, int ViewN,
+                           float YLC, float dYA, float DeltaTheta, float PLC, float DeltaT,
+                           float DeltaFai, float SO, int view_begin, int view_count,
+                           void* stream) {
+    return guarded([&] {
+        launch_rebin(d_out, d_in, YL, ZL, ViewN, YLC, dYA, DeltaTheta, PLC, DeltaT, DeltaFai,
+                     SO, ARECON_LAYOUT_VZY, static_cast<hipStream_t>(stream), view_begin,
+                     view_count);
+    });
+}
+
 int arecon_filter_dev(void* d_out, int out_dtype, int out_layout, const float* d_in, int YL,
                       int ZL, int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
                       void* stream) {
@@ -344,6 +355,25 @@ int arecon_bp_batch_destroy(arecon_bp_batch* batch) {
 int arecon_bp_batch_execute(const arecon_bp_batch* batch, void* stream) {
     return guarded([&] {
         bp_batch_launch(reinterpret_cast<const BPBatch*>(batch), static_cast<hipStream_t>(stream));
+    });
+}
+
+int arecon_bp_plan_nblocks(const arecon_bp_plan* plan) {
+    return bp_plan_nblocks(reinterpret_cast<const BPPlan*>(plan));
+}
+
+int arecon_bp_plan_block_views(const arecon_bp_plan* plan, int block, int* pmin, int* pmax) {
+    return guarded([&] {
+        bp_plan_block_views(reinterpret_cast<const BPPlan*>(plan), block, pmin, pmax);
+    });
+}
+
+int arecon_bp_plan_execute_blocks(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
+                                  int fp_dtype, int fp_layout, int block_begin, int block_end,
+                                  void* stream) {
+    return guarded([&] {
+        bp_plan_launch(reinterpret_cast<const BPPlan*>(plan), d_img, d_fp, fp_dtype, fp_layout,
+                       static_cast<hipStream_t>(stream), block_begin, block_end);
     });
 }
 

@@ -75,9 +75,12 @@ struct DevBuf {
 // ---------------------------------------------------------------------------
 // Stage launchers (device pointers, stream-ordered, no synchronisation).
 // ---------------------------------------------------------------------------
+// Output views [view_begin, view_begin + view_count) (all views when
+// view_count < 0); the input holds all ViewN views.
 void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN,
                   float YLC, float dYA, float DeltaTheta, float PLC, float DeltaT,
-                  float DeltaFai, float SO, int out_layout, hipStream_t s);
+                  float DeltaFai, float SO, int out_layout, hipStream_t s,
+                  int view_begin = 0, int view_count = -1);
 
 void launch_filter(void* out, int out_dtype, int out_layout, const float* in, int YL, int ZL,
                    int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
@@ -90,7 +93,9 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w);
 void bp_plan_destroy(BPPlan* p);
 int bp_plan_nz(const BPPlan* p);
 void bp_plan_launch(const BPPlan* p, float* img, const void* fp, int fp_dtype, int fp_layout,
-                    hipStream_t s);
+                    hipStream_t s, int block_begin = 0, int block_end = -1);
+int bp_plan_nblocks(const BPPlan* p);
+void bp_plan_block_views(const BPPlan* p, int block, int* pmin, int* pmax);
 struct BPBatch;
 BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const void* const* fps,
                          int n, int fp_dtype, int fp_layout);

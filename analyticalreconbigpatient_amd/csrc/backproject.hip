@@ -59,6 +59,7 @@ struct BPZ {        // per slice
 struct BPArgs {
     int XN, YN, ZN, YL, ZL, PN, W;
     int tilesx, ntiles, nzsets, order, k1zero;
+    int lb_offset;                    // first logical workgroup of a partial launch
     int vec4;                         // float4 stores allowed (ZN % 4 == 0, aligned output)
     float XNC, YNC, dx, dy;
     float YLC, ZLC, invdYL, KSD;      // KSD = SO SD / dZL
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(256) void bp_fdk3dw(float* __restrict__ img,
     // contiguous run of logical workgroups.
     const int nb = gridDim.x, b = blockIdx.x;
     const int qx = nb >> 3, rx = nb & 7, xcd = b & 7;
-    const int lb = xcd * qx + min(xcd, rx) + (b >> 3);
+    const int lb = xcd * qx + min(xcd, rx) + (b >> 3) + A.lb_offset;
     int zs, tile;
     if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
     else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_vt(float* __restrict__ img,
                                                     const float2* __restrict__ table, BPArgs A) {
     const int nb = gridDim.x, b = blockIdx.x;
     const int qx = nb >> 3, rx = nb & 7, xcd = b & 7;
-    const int lb = xcd * qx + min(xcd, rx) + (b >> 3);
+    const int lb = xcd * qx + min(xcd, rx) + (b >> 3) + A.lb_offset;
     int zs, tile;
     if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
     else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
@@ -577,6 +578,7 @@ __device__ __forceinline__ void zc_run(int lb, const BPArgs& A, float* __restric
                                        const float2* __restrict__ table) {
     static_assert(G == 0 ? ZB <= 16 : (ZB % G == 0 && ZB <= 32 && (G == 1 || G == 2 || G == 4)),
                   "bad block shape");
+    lb += A.lb_offset;
     fp = uniform_ptr(fp);
     vtab = uniform_ptr(vtab);
     blocks = uniform_ptr(blocks);
@@ -827,6 +829,7 @@ struct BPPlan {
     int nz = 1;
     int group = 1;      // z-column kernel: slices per branch-free group
     int grid = 0;
+    std::vector<int> blk_pmin, blk_pmax;   // per slice set: union of the windows
 };
 
 // One slice's entry for one view, FP64 on the host: temp2 = z - h V / 2pi (:79) and
@@ -1027,6 +1030,19 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
     A.view_stride = (long long)g.ZL * g.YL;
     const long long grid = (long long)A.ntiles * A.nzsets;
     P->grid = (int)grid;
+    P->blk_pmin.assign(A.nzsets, 1);
+    P->blk_pmax.assign(A.nzsets, 0);
+    for (int zs = 0; zs < A.nzsets; ++zs) {
+        int lo = INT_MAX, hi = INT_MIN;
+        for (int j = 0; j < nz; ++j) {
+            const int zi = zs * nz + j;
+            if (zi < g.ZN && zt[zi].s1 <= zt[zi].s2) {
+                lo = std::min(lo, zt[zi].s1);
+                hi = std::max(hi, zt[zi].s2);
+            }
+        }
+        if (lo <= hi) { P->blk_pmin[zs] = lo; P->blk_pmax[zs] = hi; }
+    }
     try {
         // one view in any layout (rows, row pairs at twice and quads at four times
         // the size) stays below 2 GiB, and rows and row strides fit the kernels'
@@ -1052,6 +1068,15 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
 }
 
 int bp_plan_nz(const BPPlan* P) { return P ? P->nz : 0; }
+
+int bp_plan_nblocks(const BPPlan* P) { return P ? P->a.nzsets : 0; }
+
+void bp_plan_block_views(const BPPlan* P, int block, int* pmin, int* pmax) {
+    ARECON_REQUIRE(P != nullptr && pmin != nullptr && pmax != nullptr, "bp: null argument");
+    ARECON_REQUIRE(block >= 0 && block < P->a.nzsets, "bp: slice block out of range");
+    *pmin = P->blk_pmin[block];
+    *pmax = P->blk_pmax[block];
+}
 
 void bp_plan_destroy(BPPlan* P) {
     if (!P) return;
@@ -1083,16 +1108,18 @@ static BPArgs launch_args(const BPPlan* P, const float* img, int layout, size_t 
 }
 
 template <int NZ, class T>
-static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipStream_t s) {
-    const BPArgs a = launch_args(P, img, layout, sizeof(T));
+static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipStream_t s,
+                      int lb_offset, int grid) {
+    BPArgs a = launch_args(P, img, layout, sizeof(T));
+    a.lb_offset = lb_offset;
     if constexpr (NZ <= 16) {
         if (P->variant == 1) {
-            hipLaunchKernelGGL((bp_fdk3dw<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp,
+            hipLaunchKernelGGL((bp_fdk3dw<NZ, T>), dim3(grid), dim3(256), 0, s, img, fp,
                                P->vtab, P->ktab, P->ztab, a);
             return;
         }
         if (P->variant == 0) {
-            hipLaunchKernelGGL((bp_fdk3dw_vt<NZ, T>), dim3(P->grid), dim3(256), 0, s, img, fp,
+            hipLaunchKernelGGL((bp_fdk3dw_vt<NZ, T>), dim3(grid), dim3(256), 0, s, img, fp,
                                P->vtab, P->zsets, P->vt_table, a);
             return;
         }
@@ -1100,7 +1127,7 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
     if constexpr (NZ % 8 == 0 && NZ <= 32) {
         if (P->variant == 2) {
 #define ARECON_ZC(G, LY)                                                                      \
-    hipLaunchKernelGGL((bp_fdk3dw_zc<NZ, G, T, LY>), dim3(P->grid), dim3(256), 0, s, img, fp, \
+    hipLaunchKernelGGL((bp_fdk3dw_zc<NZ, G, T, LY>), dim3(grid), dim3(256), 0, s, img, fp, \
                        P->vtab, P->zblocks, P->zmasks, P->vt_table, a)
             if (layout == ARECON_FP_ROWS) {
                 if (P->group == 1) ARECON_ZC(1, ARECON_FP_ROWS);
@@ -1126,31 +1153,42 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
 }
 
 template <class T>
-static void launch_t(const BPPlan* P, float* img, const T* fp, int layout, hipStream_t s) {
+static void launch_t(const BPPlan* P, float* img, const T* fp, int layout, hipStream_t s,
+                     int lb_offset, int grid) {
     switch (P->nz) {
-    case 1: launch_nz<1, T>(P, img, fp, layout, s); break;
-    case 2: launch_nz<2, T>(P, img, fp, layout, s); break;
-    case 4: launch_nz<4, T>(P, img, fp, layout, s); break;
-    case 8: launch_nz<8, T>(P, img, fp, layout, s); break;
-    case 16: launch_nz<16, T>(P, img, fp, layout, s); break;
-    default: launch_nz<32, T>(P, img, fp, layout, s); break;
+    case 1: launch_nz<1, T>(P, img, fp, layout, s, lb_offset, grid); break;
+    case 2: launch_nz<2, T>(P, img, fp, layout, s, lb_offset, grid); break;
+    case 4: launch_nz<4, T>(P, img, fp, layout, s, lb_offset, grid); break;
+    case 8: launch_nz<8, T>(P, img, fp, layout, s, lb_offset, grid); break;
+    case 16: launch_nz<16, T>(P, img, fp, layout, s, lb_offset, grid); break;
+    default: launch_nz<32, T>(P, img, fp, layout, s, lb_offset, grid); break;
     }
 }
 
 void bp_plan_launch(const BPPlan* P, float* img, const void* fp, int fp_dtype, int fp_layout,
-                    hipStream_t s) {
+                    hipStream_t s, int block_begin, int block_end) {
     ARECON_REQUIRE(P != nullptr, "bp: null plan");
     ARECON_REQUIRE(img != nullptr && fp != nullptr, "bp: null buffer");
     ARECON_REQUIRE(fp_layout == ARECON_FP_ROWS || fp_layout == ARECON_FP_ROWPAIRS ||
                        fp_layout == ARECON_FP_QUADS,
                    "bp: unknown fp layout");
+    if (block_end < 0) block_end = P->a.nzsets;
+    ARECON_REQUIRE(block_begin >= 0 && block_begin < block_end && block_end <= P->a.nzsets,
+                   "bp: slice-block range outside the plan");
+    ARECON_REQUIRE(P->a.order == 0 || (block_begin == 0 && block_end == P->a.nzsets),
+                   "bp: slice-block ranges need the slice-major workgroup order");
+    const int lb_offset = block_begin * P->a.ntiles;
+    const int grid = (block_end - block_begin) * P->a.ntiles;
     switch (fp_dtype) {
-    case ARECON_F32: launch_t<float>(P, img, static_cast<const float*>(fp), fp_layout, s); break;
+    case ARECON_F32:
+        launch_t<float>(P, img, static_cast<const float*>(fp), fp_layout, s, lb_offset, grid);
+        break;
     case ARECON_F16:
-        launch_t<__half>(P, img, static_cast<const __half*>(fp), fp_layout, s);
+        launch_t<__half>(P, img, static_cast<const __half*>(fp), fp_layout, s, lb_offset, grid);
         break;
     case ARECON_BF16:
-        launch_t<__hip_bfloat16>(P, img, static_cast<const __hip_bfloat16*>(fp), fp_layout, s);
+        launch_t<__hip_bfloat16>(P, img, static_cast<const __hip_bfloat16*>(fp), fp_layout, s,
+                                 lb_offset, grid);
         break;
     default: fail(ARECON_ERR_ARG, "bp: unknown fp dtype");
     }

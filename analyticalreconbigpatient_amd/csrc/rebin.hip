@@ -196,11 +196,11 @@ template <int JB>
 __global__ __launch_bounds__(256) void rebin_vzy(
     float* __restrict__ out, const float* __restrict__ in,
     const ChanCoef* __restrict__ chan, int YL, int ZL, int ViewN, float DeltaTheta,
-    float DeltaFai) {
+    float DeltaFai, int view_begin) {
     extern __shared__ float tile[];             // ZL * (JB + 1)
     __shared__ Blend sb[JB];
     __shared__ int sbase[JB][4];
-    const int i = blockIdx.x;
+    const int i = view_begin + blockIdx.x;
     const int j0 = blockIdx.y * JB;
     const int nj = min(JB, YL - j0);
     const int tid = threadIdx.x;
@@ -238,8 +238,13 @@ __global__ __launch_bounds__(256) void rebin_vzy(
 
 void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN, float YLC,
                   float dYA, float DeltaTheta, float PLC, float DeltaT, float DeltaFai,
-                  float SO, int out_layout, hipStream_t s) {
+                  float SO, int out_layout, hipStream_t s, int view_begin, int view_count) {
     ARECON_REQUIRE(YL > 0 && ZL > 0 && ViewN > 0, "rebin: YL, ZL and ViewN must be positive");
+    if (view_count < 0) view_count = ViewN - view_begin;
+    ARECON_REQUIRE(view_begin >= 0 && view_count > 0 && view_begin + view_count <= ViewN,
+                   "rebin: view range outside [0, ViewN)");
+    ARECON_REQUIRE(out_layout == ARECON_LAYOUT_VZY || (view_begin == 0 && view_count == ViewN),
+                   "rebin: view ranges need the transposed output layout");
     ARECON_REQUIRE(out != nullptr && in != nullptr, "rebin: null buffer");
     ARECON_REQUIRE(out != in, "rebin: input and output must not alias");
     ARECON_REQUIRE((long long)YL * ViewN < (1LL << 31),
@@ -266,9 +271,9 @@ void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN, float 
         constexpr int JB = 64;
         const size_t lds = sizeof(float) * (size_t)ZL * (JB + 1);
         ARECON_REQUIRE(lds <= 64 * 1024, "rebin: ZL too large for the transposing kernel");
-        dim3 grid(ViewN, (YL + JB - 1) / JB);
+        dim3 grid(view_count, (YL + JB - 1) / JB);
         hipLaunchKernelGGL(rebin_vzy<JB>, grid, dim3(256), lds, s, out, in,
-                           chan, YL, ZL, ViewN, DeltaTheta, DeltaFai);
+                           chan, YL, ZL, ViewN, DeltaTheta, DeltaFai, view_begin);
     }
     ARECON_HIP(hipGetLastError());
 }

@@ -144,6 +144,15 @@ int arecon_rebin_dev(float* d_out, const float* d_in, int YL, int ZL, int ViewN,
                      float YLC, float dYA, float DeltaTheta, float PLC, float DeltaT,
                      float DeltaFai, float SO, int out_layout, void* stream);
 
+/* Rebinning of the output views [view_begin, view_begin + view_count) only,
+ * written in ARECON_LAYOUT_VZY into the full-size d_out (d_in holds all ViewN
+ * views).  Lets a caller filter and backproject early views while later ones
+ * are still being rebinned. */
+int arecon_rebin_views_dev(float* d_out, const float* d_in, int YL, int ZL, int ViewN,
+                           float YLC, float dYA, float DeltaTheta, float PLC, float DeltaT,
+                           float DeltaFai, float SO, int view_begin, int view_count,
+                           void* stream);
+
 /* Preweight + ramp filter on device buffers; input layout [ViewN][ZL][YL],
  * output layout out_layout (ARECON_FP_ROWS, ARECON_FP_ROWPAIRS or
  * ARECON_FP_QUADS).  d_out may equal d_in only for ARECON_F32 rows.  For
@@ -193,6 +202,17 @@ int arecon_bp_plan_nz(const arecon_bp_plan* plan);
  * the z-column kernel (variant 2). */
 int arecon_bp_plan_execute(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
                            int fp_dtype, int fp_layout, void* stream);
+
+/* The plan's slices come in blocks of arecon_bp_plan_nz consecutive slices.
+ * arecon_bp_plan_nblocks: their number; arecon_bp_plan_block_views: the views
+ * [*pmin, *pmax] block b reads (pmin > pmax: none); execute_blocks: writes only
+ * the slices of blocks [block_begin, block_end) (slice-major order only), so a
+ * caller can start early blocks once their views are filtered. */
+int arecon_bp_plan_nblocks(const arecon_bp_plan* plan);
+int arecon_bp_plan_block_views(const arecon_bp_plan* plan, int block, int* pmin, int* pmax);
+int arecon_bp_plan_execute_blocks(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
+                                  int fp_dtype, int fp_layout, int block_begin, int block_end,
+                                  void* stream);
 
 /* Batched backprojection (no counterpart in the reference, whose MEX
  * backprojects one volume per call): n plans -- z-slabs of one or more scans,
