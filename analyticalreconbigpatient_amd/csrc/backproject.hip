@@ -513,6 +513,45 @@ __device__ __forceinline__ void zc_group(cptr<float> e, float K, float zlc,
     }
 }
 
+// In-place 3D-weighted accumulation of one slice's sample t:
+//   acc += t / (1 + a B)   when bit J of the wavefront-uniform mask `wm` is set,
+//   acc += t               otherwise (no conjugate view: weight exactly 1).
+// One asm block with the uniform branch inside, and the accumulator tied to the
+// destination: written as C++, the two paths give the accumulator two new
+// registers and the compiler copies every accumulator back at every view.  The
+// s_nop is the one wait state that a VALU read of a transcendental's result
+// (v_rcp_f32) needs; the compiler does not insert it for inline asm.
+template <int J>
+__device__ __forceinline__ void acc_weighted(float& acc, float t, unsigned wm, float a, float B) {
+    float w;
+    asm volatile(
+        "s_bitcmp1_b32 %[wm], %[j]\n\t"
+        "s_cbranch_scc0 1f\n\t"
+        "v_fma_f32 %[w], %[a], %[B], 1.0\n\t"
+        "v_rcp_f32 %[w], %[w]\n\t"
+        "s_nop 0\n\t"
+        "v_mul_f32 %[t], %[w], %[t]\n"
+        "1:\n\t"
+        "v_add_f32 %[acc], %[acc], %[t]"
+        : [acc] "+v"(acc), [t] "+v"(t), [w] "=&v"(w)
+        : [wm] "s"(wm), [j] "i"(J), [a] "s"(a), [B] "v"(B)
+        : "scc");
+}
+
+// Blend and accumulate slices J..ZB-1 of a view whose gathers are in flight:
+// each active slice (bit of m.x) blends its sample and adds it with its 3D
+// weight (bit of m.y: conjugate view present).
+template <int J, int ZB, class Gt>
+__device__ __forceinline__ void zc_blend(const typename Gt::Raw* q, const float* fr,
+                                         const float2* ent, uint2 m, float B, float Dey,
+                                         float wu, float* acc) {
+    if constexpr (J < ZB) {
+        if ((m.x >> J) & 1u)
+            acc_weighted<J>(acc[J], Gt::lerp(q[J], fr[J], Dey, wu), m.y, ent[J].y, B);
+        zc_blend<J + 1, ZB, Gt>(q, fr, ent, m, B, Dey, wu, acc);
+    }
+}
+
 // All ZB slices of the block for one view, each behind its own uniform window
 // test: first every active slice's gather, then every blend.
 template <int ZB, class T, int LAYOUT>
@@ -531,17 +570,7 @@ __device__ __forceinline__ void zc_split(cptr<float> e, uint2 m, float K,
         if ((m.x >> j) & 1u)
             q[j] = Gt::load(rs, zc_offset<LAYOUT>(fmaf(ent[j].x, K, zlc), rowb, ucolb, zlm1,
                                                   fr[j]), rowb);
-    if (m.y != 0u) {
-#pragma unroll
-        for (int j = 0; j < ZB; ++j)
-            if ((m.x >> j) & 1u)   // 1 / (1 + 2^Q B); exactly 1 where a = 0
-                acc[j] = fmaf(fast_rcp(fmaf(ent[j].y, B, 1.f)),
-                              Gt::lerp(q[j], fr[j], Dey, wu), acc[j]);
-    } else {
-#pragma unroll
-        for (int j = 0; j < ZB; ++j)
-            if ((m.x >> j) & 1u) acc[j] += Gt::lerp(q[j], fr[j], Dey, wu);
-    }
+    zc_blend<0, ZB, Gt>(q, fr, ent, m, B, Dey, wu, acc);
 }
 
 // XCD-aware remap: hardware workgroup b runs on XCD b % 8; give each XCD a
