@@ -368,6 +368,11 @@ int arecon_bp_plan_block_views(const arecon_bp_plan* plan, int block, int* pmin,
     });
 }
 
+int arecon_bp_plan_stats(const arecon_bp_plan* plan, unsigned long long* staged,
+                         unsigned long long* direct) {
+    return guarded([&] { bp_plan_stats(reinterpret_cast<const BPPlan*>(plan), staged, direct); });
+}
+
 int arecon_bp_plan_execute_blocks(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
                                   int fp_dtype, int fp_layout, int block_begin, int block_end,
                                   void* stream) {

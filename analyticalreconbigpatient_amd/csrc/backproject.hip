@@ -44,6 +44,7 @@
 #include <climits>
 #include <cmath>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 namespace arecon {
@@ -67,6 +68,7 @@ struct BPArgs {
     float SO2, hstep, k1, DeltaFai;   // hstep = h DeltaFai / 2pi
     long long view_stride;            // ZL * YL samples
     unsigned view_bytes;              // ZL * YL * sizeof(sample)
+    unsigned long long* stats;        // LDS-staged kernel: workgroups staged / direct
 };
 
 // Two adjacent detector samples (channels U-1, U of one row) as floats.
@@ -538,6 +540,30 @@ __device__ __forceinline__ void acc_weighted(float& acc, float t, unsigned wm, f
         : "scc");
 }
 
+// acc_weighted with the wavefront-uniform weighted flag as a value (0 or 1).
+__device__ __forceinline__ void acc_weighted_dyn(float& acc, float t, unsigned weighted, float a,
+                                                 float B) {
+    float w;
+    asm volatile(
+        "s_cmp_eq_u32 %[wt], 0\n\t"
+        "s_cbranch_scc1 1f\n\t"
+        "v_fma_f32 %[w], %[a], %[B], 1.0\n\t"
+        "v_rcp_f32 %[w], %[w]\n\t"
+        "s_nop 0\n\t"
+        "v_mul_f32 %[t], %[w], %[t]\n"
+        "1:\n\t"
+        "v_add_f32 %[acc], %[acc], %[t]"
+        : [acc] "+v"(acc), [t] "+v"(t), [w] "=&v"(w)
+        : [wt] "s"(weighted), [a] "s"(a), [B] "v"(B)
+        : "scc");
+}
+
+// Gives a sample register an (unspecified) value without an instruction.  The
+// samples of slices outside a view's window are never read, but left undefined
+// the compiler merges them with the previous view's registers and copies every
+// one of them around the loop.
+template <class R> __device__ __forceinline__ void undef_reg(R& r) { asm volatile("" : "=v"(r)); }
+
 // Blend and accumulate slices J..ZB-1 of a view whose gathers are in flight:
 // each active slice (bit of m.x) blends its sample and adds it with its 3D
 // weight (bit of m.y: conjugate view present).
@@ -714,6 +740,309 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
     zc_run<ZB, G, T, LAYOUT>(xcd_logical_block(), A, img, fp, vtab, blocks, masks, table);
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged z-column kernel.  The z-column kernel issues one 64-lane gather per
+// voxel-update, and the texture addresser takes 4 lane addresses per clock, so
+// its gathers alone cost 16 clocks per wavefront-update.  Here the workgroup's
+// detector footprint of each view is copied once into LDS with coalesced
+// direct-to-LDS buffer loads (64 lanes x 16 B per instruction, no VGPRs), and
+// every voxel-update reads LDS instead.  The arithmetic is that of the
+// z-column kernel step for step, so the result is bit-identical.
+//
+// Footprint of view p for a 16x16-column tile and the block's active slices
+// (bounds from the tile's corners, with margins for FP32 rounding):
+//  * channels: Yr = -x sinV + y cosV is affine in (x, y), so its range over the
+//    tile is centre +- (7.5 dx |sinV| + 7.5 dy |cosV|); CW = 32 records
+//    (channels) per staged row starting at c0 cover it with a margin.
+//  * rows: V1 = temp2 K + ZLC with K = SO SD / (dZL t1 (t1 + UU)),
+//    t1 = sqrt(SO^2 - Yr^2).  t1 (t1 + UU) grows with UU and with t1 (when
+//    2 t1 + UU > 0), so the box of (|Yr|, UU) over the tile bounds K, and the
+//    first and last active slices' temp2 bound temp2 (affine in the slice).
+//    Record rows [rA, rB] are staged, clipped to [-1, ZL-1]; rows -1 and ZL-1
+//    are off the detector and load as zeros (out-of-range buffer reads).
+//  * a view whose footprint does not fit (rows > RMAX, channels > CW, or a tile
+//    reaching the source circle) is read straight from global memory as in the
+//    z-column kernel; a view whose rows all lie off the detector is skipped
+//    (every sample it would read is zero).
+// Records are 8 bytes: FP32 row pairs (rows r, r+1 of a channel) or 16-bit
+// difference quads; a staged row is CW records = 256 B.  Each wave stages one
+// view of a chunk of four, then every wave consumes the four views.
+// ---------------------------------------------------------------------------
+constexpr int kZsCW = 32;                    // records per staged row
+constexpr int kZsRowBytes = kZsCW * 8;       // bytes per staged row
+constexpr int kZsMaxViews = 1024;            // views per block in the footprint table
+enum { ZS_SKIP = 0, ZS_LDS = 1, ZS_DIRECT = 2 };
+
+typedef __attribute__((address_space(3))) char lds_char;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Sample record(s) of one update at LDS address `a`.
+template <class T, int LAYOUT> struct LdsGather;
+template <> struct LdsGather<float, ARECON_FP_ROWPAIRS> {   // records c, c+1 (16 B)
+    typedef Gather<float, ARECON_FP_ROWPAIRS>::Raw Raw;
+    __device__ static __forceinline__ Raw load(const lds_char* a) {
+        typedef const __attribute__((address_space(3))) f32x2_v lds_f2;
+        const f32x2_v lo = *reinterpret_cast<lds_f2*>(a);
+        const f32x2_v hi = *reinterpret_cast<lds_f2*>(a + 8);
+        return Raw{lo.x, lo.y, hi.x, hi.y};
+    }
+};
+template <class T> struct LdsGather<T, ARECON_FP_QUADS> {         // one quad (8 B)
+    typedef typename Gather<T, ARECON_FP_QUADS>::Raw Raw;
+    __device__ static __forceinline__ Raw load(const lds_char* a) {
+        return *reinterpret_cast<const __attribute__((address_space(3))) unsigned long long*>(a);
+    }
+};
+
+struct ZsView {      // footprint of one staged view (LDS)
+    int mode, rA, rB, c0;
+};
+
+// Footprint of view p for the tile centred at (xc, yc) and the active slices m
+// of the block (temp2 of slice j at erow[2 j]).
+template <int RMAX>
+__device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsigned m,
+                                               cptr<float> erow, float xc, float yc,
+                                               float ex, float ey) {
+    ZsView v{ZS_SKIP, 0, -1, 0};
+    if (m == 0u) return v;
+    v.mode = ZS_DIRECT;
+    const float Yrc = -xc * cs.y + yc * cs.x;
+    const float UUc = -xc * cs.x - yc * cs.y;
+    const float eY = ex * fabsf(cs.y) + ey * fabsf(cs.x);
+    const float eU = ex * fabsf(cs.x) + ey * fabsf(cs.y);
+    const float ua = (Yrc - eY) * A.invdYL + A.YLC, ub = (Yrc + eY) * A.invdYL + A.YLC;
+    const int c0 = (int)floorf(fminf(ua, ub)) - 2;
+    if ((int)ceilf(fmaxf(ua, ub)) + 2 > c0 + kZsCW - 1) return v;
+    const float ya = fmaxf(fabsf(Yrc) - eY, 0.f), yb = fabsf(Yrc) + eY;
+    const float t1lo2 = A.SO2 - yb * yb;
+    if (!(t1lo2 > 0.f)) return v;
+    const float t1lo = sqrtf(t1lo2), t1hi = sqrtf(A.SO2 - ya * ya);
+    const float uulo = UUc - eU, uuhi = UUc + eU;
+    if (!(t1lo + uulo > 1e-3f * t1lo)) return v;       // den > 0 and 2 t1 + UU > 0
+    const float Klo = A.KSD / (t1hi * (t1hi + uuhi)) * (1.f - 1e-5f);
+    const float Khi = A.KSD / (t1lo * (t1lo + uulo)) * (1.f + 1e-5f);
+    const int jlo = __builtin_ctz(m), jhi = 31 - __builtin_clz(m);
+    const float ta = erow[2 * jlo], tb = erow[2 * jhi];
+    const float vlo = fminf(fminf(ta * Klo, ta * Khi), fminf(tb * Klo, tb * Khi)) + A.ZLC;
+    const float vhi = fmaxf(fmaxf(ta * Klo, ta * Khi), fmaxf(tb * Klo, tb * Khi)) + A.ZLC;
+    if (!(vlo > -1e6f && vhi < 1e6f)) return v;
+    const int zlm1 = A.ZL - 1;
+    const int rA = max((int)floorf(vlo) - 1, -1);
+    const int rB = min((int)floorf(vhi) + 1, zlm1);
+    if (rB <= -1 || rA >= zlm1) { v.mode = ZS_SKIP; return v; }   // only zero rows
+    // rows [rA, rB] in 4-row loads, then one zero row after them
+    if (4 * ((rB - rA + 4) / 4) >= RMAX) return v;
+    v.mode = ZS_LDS; v.rA = rA; v.rB = rB; v.c0 = c0;
+    return v;
+}
+
+// Footprint table entry: c0 (16 bits), rA + 1 and rB + 1 (8 bits each); a
+// view with rB < rA is skipped.
+__device__ __forceinline__ unsigned zs_pack(const ZsView& v) {
+    if (v.mode != ZS_LDS) return 1u << 16;                 // rA = 0, rB = -1: skip
+    return ((unsigned)v.c0 & 0xffffu) | ((unsigned)(v.rA + 1) << 16) |
+           ((unsigned)(v.rB + 1) << 24);
+}
+
+template <int ZB, class T, int LAYOUT, int RMAX>
+__device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restrict__ img,
+                                       const T* __restrict__ fp,
+                                       const float2* __restrict__ vtab,
+                                       const ZBlock* __restrict__ blocks,
+                                       const uint2* __restrict__ masks,
+                                       const float2* __restrict__ table) {
+    static_assert(Gather<T, LAYOUT>::kElem == 8, "LDS staging takes 8-byte records");
+    static_assert(ZB <= 16 && RMAX % 4 == 0, "bad block shape");
+    constexpr int kSlot = RMAX * kZsRowBytes;
+    __shared__ __attribute__((aligned(16))) char stage[4 * kSlot + 16];
+    __shared__ __attribute__((aligned(16))) unsigned fpt[kZsMaxViews];
+    typedef Gather<T, LAYOUT> Gt;
+    typedef LdsGather<T, LAYOUT> Lg;
+
+    const int lb0 = lb;
+    lb += A.lb_offset;
+    int zs, tile;
+    if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
+    else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
+    const int tyb = tile / A.tilesx, txb = tile - tyb * A.tilesx;
+    const cptr<int> hbp = (cptr<int>)const_ptr(uniform_ptr(blocks) + zs);
+    const int pmin = hbp[0], pmax = hbp[1];
+    const long long hoff = *(cptr<long long>)(hbp + 2);
+    const cptr<unsigned> mrow =
+        (cptr<unsigned>)const_ptr(uniform_ptr(masks) + (hoff / ZB - pmin));
+    const cptr<float> trow =
+        (cptr<float>)const_ptr(uniform_ptr(table) + (hoff - (long long)pmin * ZB));
+    const cptr<float> vt = (cptr<float>)const_ptr(uniform_ptr(vtab));
+    const float xc = ((float)(txb * 16) + 7.5f - A.XNC) * A.dx;
+    const float yc = ((float)(tyb * 16) + 7.5f - A.YNC) * A.dy;
+    const float ex = 7.5f * fabsf(A.dx), ey = 7.5f * fabsf(A.dy);
+
+    // Pass 1: every view's footprint into the table.  A workgroup with a view
+    // that does not fit runs the z-column kernel's body instead.
+    int fits = pmax - pmin + 1 <= kZsMaxViews;
+    if (fits)
+        for (int p = pmin + (int)threadIdx.x; p <= pmax; p += 256) {
+            const ZsView v = zs_footprint<RMAX>(A, cld_f2(vt, p), mrow[2LL * p],
+                                                trow + 2LL * p * ZB, xc, yc, ex, ey);
+            if (v.mode == ZS_DIRECT) fits = 0;
+            fpt[p - pmin] = zs_pack(v);
+        }
+    if (threadIdx.x < 4)
+        reinterpret_cast<__attribute__((address_space(3))) unsigned*>(
+            (lds_char*)stage)[(4 * kSlot) / 4 + threadIdx.x] = 0u;      // zero record
+    const int staged = __syncthreads_and(fits);
+    if (threadIdx.x == 0 && A.stats) atomicAdd(A.stats + (staged ? 0 : 1), 1ull);
+    if (!staged) {
+        zc_run<ZB, 0, T, LAYOUT>(lb0, A, img, fp, vtab, blocks, masks, table);
+        return;
+    }
+
+    fp = uniform_ptr(fp);
+    const long long view_stride = uniform_i64(A.view_stride);
+    const unsigned view_bytes = __builtin_amdgcn_readfirstlane(A.view_bytes);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int xi = txb * 16 + (wave & 1) * 8 + (lane & 7);
+    const int yi = tyb * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const float x = ((float)xi - A.XNC) * A.dx;
+    const float y = ((float)yi - A.YNC) * A.dy;
+    const int z0 = zs * ZB;
+    constexpr unsigned kElem = Gt::kElem;
+    const int rowb = (LAYOUT == ARECON_FP_QUADS ? A.YL - 1 : A.YL) * (int)kElem;
+    lds_char* const lstage = (lds_char*)stage;
+    float zlc, zoff;
+    asm("v_mov_b32 %0, %1" : "=v"(zlc) : "s"(A.ZLC));
+    // a lane off the detector gets row -1e9, clamped to rA, and reads the zero
+    // record after the slots
+    asm("v_mov_b32 %0, %1" : "=v"(zoff) : "s"(-1e9f));
+    float acc[ZB];
+#pragma unroll
+    for (int j = 0; j < ZB; ++j) acc[j] = 0.f;
+
+    for (int p0 = pmin; p0 <= pmax; p0 += 4) {
+        // stage: wave w copies the footprint of view p0 + w into slot w
+        {
+            const int p = p0 + wave;
+            if (p <= pmax) {
+                const unsigned e = __builtin_amdgcn_readfirstlane(fpt[p - pmin]);
+                const int rA = (int)((e >> 16) & 0xffu) - 1, rB = (int)(e >> 24) - 1;
+                const int c0 = (int)(short)(e & 0xffffu);
+                if (rB >= rA) {
+                    const __amdgpu_buffer_rsrc_t rs =
+                        view_rsrc(fp + (long long)p * view_stride, view_bytes);
+                    // lane: row rA + i + lane / 16, records c0 + 2 (lane % 16) + {0, 1};
+                    // rows -1 and ZL-1 and records outside the row are never read
+                    // by an update or lie outside the view (and load as zeros)
+                    const int off = (rA + (lane >> 4)) * rowb + (c0 + 2 * (lane & 15)) * (int)kElem;
+                    lds_char* dst = lstage + wave * kSlot;
+                    for (int i = 0; i <= rB - rA; i += 4)
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                            rs, (lds_void*)(dst + i * kZsRowBytes), 16,
+                            (unsigned)(off + i * rowb), 0, 0, 0);
+                    // the zero row after the loaded ones: slices outside the view's
+                    // window (temp2 = 1e30) clamp onto it and add exactly zero
+                    reinterpret_cast<__attribute__((address_space(3))) unsigned*>(
+                        dst + 4 * ((rB - rA + 4) / 4) * kZsRowBytes)[lane] = 0u;
+                }
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        // consume the chunk's views; their four table entries in one LDS read
+        const uint4 e4 = *reinterpret_cast<const uint4*>(fpt + (p0 - pmin));
+        const unsigned ech[4] = {(unsigned)__builtin_amdgcn_readfirstlane(e4.x),
+                                 (unsigned)__builtin_amdgcn_readfirstlane(e4.y),
+                                 (unsigned)__builtin_amdgcn_readfirstlane(e4.z),
+                                 (unsigned)__builtin_amdgcn_readfirstlane(e4.w)};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int p = p0 + s;
+            if (p > pmax) break;
+            const unsigned e = ech[s];
+            const int rA = (int)((e >> 16) & 0xffu) - 1, rB = (int)(e >> 24) - 1;
+            const int c0 = (int)(short)(e & 0xffffu);
+            if (rB < rA) continue;                      // every sample would be zero
+            const uint2 m = cld_u2(mrow, p);               // scalar: active, weighted
+            const float2 cs = cld_f2(vt, p);
+            const float UU = -x * cs.x - y * cs.y;
+            const float Yr = -x * cs.y + y * cs.x;
+            const float t1sq = A.SO2 - Yr * Yr;
+            const float temp1 = fast_sqrt(t1sq);
+            const float U1 = Yr * A.invdYL + A.YLC;
+            const float den = fmaf(UU, temp1, t1sq);
+            const bool uok = (U1 > 0.f) && (U1 <= A.YLm1) && (t1sq >= 0.f) && (den != 0.f);
+            const float Uc = ceilf(U1);
+            const float Dey = Uc - U1;
+            const float wu = 1.f - Dey;
+            const float inv = fast_rcp(den);
+            const float K = uok ? A.KSD * inv : 0.f;
+            float B = 1.f;
+            if (m.y != 0u)
+                B = fminf(fmaxf(fast_exp2(A.k1 * fast_log2(fabsf((temp1 - UU) * temp1 * inv))),
+                                1e-30f), 1e30f);
+            const cptr<float> erow = trow + 2LL * p * ZB;
+            float2 ent[ZB];
+#pragma unroll
+            for (int j = 0; j < ZB; ++j) ent[j] = cld_f2(erow, j);
+            // byte offset in `stage` = row * kZsRowBytes + colterm, row clamped to
+            // [rA, rB]; a lane off the detector gets row -1e9 (clamped to rA) and
+            // the zero record after the slots
+            const float zl = uok ? zlc : zoff;
+            const int cterm = s * kSlot - (c0 + 1) * 8 - rA * kZsRowBytes;
+            int zterm;
+            asm("v_mov_b32 %0, %1" : "=v"(zterm) : "s"(4 * kSlot - rA * kZsRowBytes));
+            int rBv;   // clamp bound: the zero row after the loaded rows
+            asm("v_mov_b32 %0, %1" : "=v"(rBv) : "s"(rA + 4 * ((rB - rA + 4) / 4)));
+            const int colterm = uok ? (int)Uc * 8 + cterm : zterm;
+            // every slice, branch-free: a slice outside the view's window has
+            // temp2 = 1e30, so its row clamps onto the zero row (and a lane off the
+            // detector reads the zero record): it adds exactly zero
+            typename Gt::Raw q[ZB];
+            float fr[ZB];
+#pragma unroll
+            for (int j = 0; j < ZB; ++j) {
+                const float V1 = fmaf(ent[j].x, K, zl);
+                int vf;
+                asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(vf) : "v"(V1));
+                fr[j] = __builtin_amdgcn_fractf(V1);
+                int r;   // one VGPR bound: a VOP3 reads at most one SGPR here
+                asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(vf), "s"(rA), "v"(rBv));
+                q[j] = Lg::load(lstage + (r * kZsRowBytes + colterm));
+            }
+#pragma unroll
+            for (int j = 0; j < ZB; ++j)
+                acc_weighted_dyn(acc[j], Gt::lerp(q[j], fr[j], Dey, wu), (m.y >> j) & 1u,
+                                 ent[j].y, B);
+        }
+        __syncthreads();
+    }
+    if (xi < A.XN && yi < A.YN) {
+        float* o = img + ((long long)yi * A.XN + xi) * A.ZN + z0;
+        if (A.vec4 && z0 + ZB <= A.ZN) {
+#pragma unroll
+            for (int j = 0; j < ZB; j += 4)
+                *reinterpret_cast<float4*>(o + j) =
+                    make_float4(acc[j] * A.DeltaFai, acc[j + 1] * A.DeltaFai,
+                                acc[j + 2] * A.DeltaFai, acc[j + 3] * A.DeltaFai);
+        } else {
+#pragma unroll
+            for (int j = 0; j < ZB; ++j)
+                if (z0 + j < A.ZN) o[j] = acc[j] * A.DeltaFai;
+        }
+    }
+}
+
+template <int ZB, class T, int LAYOUT, int RMAX>
+__global__ __launch_bounds__(256) void bp_fdk3dw_zs(float* __restrict__ img,
+                                                    const T* __restrict__ fp,
+                                                    const float2* __restrict__ vtab,
+                                                    const ZBlock* __restrict__ blocks,
+                                                    const uint2* __restrict__ masks,
+                                                    const float2* __restrict__ table, BPArgs A) {
+    zs_run<ZB, T, LAYOUT, RMAX>(xcd_logical_block(), A, img, fp, vtab, blocks, masks, table);
+}
+
 // Batched z-column launch: several plans (z-slabs of one or more scans, each
 // with its own geometry, volume and filtered sinogram) in one grid.  Slab s owns
 // logical workgroups [wg_begin, wg_begin + its grid); the XCD remap runs over
@@ -853,6 +1182,7 @@ struct BPPlan {
     ZBlock* zblocks = nullptr;  // z-column kernel: per block view range and offset
     uint2* zmasks = nullptr;    // z-column kernel: per (block, view) slice masks
     BPZ* ztab = nullptr;
+    unsigned long long* stats = nullptr;   // LDS-staged kernel: workgroups staged / direct
     BPArgs a{};
     int variant = 0;
     int nz = 1;
@@ -1020,11 +1350,16 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         e.dz = (float)(z - g.h * (g.BetaS + s0 * g.DeltaFai) * kInvTwoPi);
         zt[zi] = e;
     }
-    ARECON_REQUIRE(g.variant >= 0 && g.variant <= 2, "bp: variant must be 0, 1 or 2");
+    ARECON_REQUIRE(g.variant >= 0 && g.variant <= 3, "bp: variant must be 0, 1, 2 or 3");
     ARECON_REQUIRE(g.order == 0 || g.order == 1, "bp: order must be 0 or 1");
     int nz;
     int group = 1;
-    if (g.variant == 2) {
+    if (g.variant == 3) {
+        nz = g.nz > 0 ? g.nz : 8;
+        ARECON_REQUIRE(nz == 8 || nz == 16,
+                       "bp: nz of the LDS-staged z-column kernel must be 0, 8 or 16");
+        group = 8;
+    } else if (g.variant == 2) {
         nz = g.nz > 0 ? g.nz : 32;
         ARECON_REQUIRE(nz == 8 || nz == 16 || nz == 32,
                        "bp: nz of the z-column kernel must be 0, 8, 16 or 32");
@@ -1085,7 +1420,13 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         ARECON_HIP(hipMalloc(&P->ktab, sizeof(float4) * kt.size()));
         ARECON_HIP(hipMalloc(&P->ztab, sizeof(BPZ) * zt.size()));
         if (P->variant == 0) build_view_table(P, w, zt);
-        if (P->variant == 2) build_block_table(P, w, zt);
+        if (P->variant >= 2) build_block_table(P, w, zt);
+        if (P->variant == 3) {
+            ARECON_REQUIRE(g.ZL <= 250, "bp: variant 3 takes at most 250 detector rows");
+            ARECON_HIP(hipMalloc(&P->stats, 2 * sizeof(unsigned long long)));
+            ARECON_HIP(hipMemset(P->stats, 0, 2 * sizeof(unsigned long long)));
+            A.stats = P->stats;
+        }
         ARECON_HIP(hipMemcpy(P->vtab, vt.data(), sizeof(float2) * vt.size(), hipMemcpyHostToDevice));
         ARECON_HIP(hipMemcpy(P->ktab, kt.data(), sizeof(float4) * kt.size(), hipMemcpyHostToDevice));
         ARECON_HIP(hipMemcpy(P->ztab, zt.data(), sizeof(BPZ) * zt.size(), hipMemcpyHostToDevice));
@@ -1099,6 +1440,21 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
 int bp_plan_nz(const BPPlan* P) { return P ? P->nz : 0; }
 
 int bp_plan_nblocks(const BPPlan* P) { return P ? P->a.nzsets : 0; }
+
+void bp_plan_stats(const BPPlan* P, unsigned long long* staged, unsigned long long* direct) {
+    ARECON_REQUIRE(P != nullptr && staged != nullptr && direct != nullptr, "bp: null argument");
+    unsigned long long h[2] = {0ull, 0ull};
+    if (P->stats) {
+        int cur = 0;
+        ARECON_HIP(hipGetDevice(&cur));
+        ARECON_HIP(hipSetDevice(P->device));
+        const hipError_t e = hipMemcpy(h, P->stats, sizeof(h), hipMemcpyDeviceToHost);
+        (void)hipSetDevice(cur);
+        ARECON_HIP(e);
+    }
+    *staged = h[0];
+    *direct = h[1];
+}
 
 void bp_plan_block_views(const BPPlan* P, int block, int* pmin, int* pmax) {
     ARECON_REQUIRE(P != nullptr && pmin != nullptr && pmax != nullptr, "bp: null argument");
@@ -1119,6 +1475,7 @@ void bp_plan_destroy(BPPlan* P) {
     if (P->vt_table) (void)hipFree(P->vt_table);
     if (P->zblocks) (void)hipFree(P->zblocks);
     if (P->zmasks) (void)hipFree(P->zmasks);
+    if (P->stats) (void)hipFree(P->stats);
     (void)hipSetDevice(cur);
     delete P;
 }
@@ -1126,8 +1483,10 @@ void bp_plan_destroy(BPPlan* P) {
 // Kernel arguments of one launch of plan P: the view stride of the filtered
 // layout and whether the volume takes float4 stores.
 static BPArgs launch_args(const BPPlan* P, const float* img, int layout, size_t elem) {
-    ARECON_REQUIRE(layout == ARECON_FP_ROWS || P->variant == 2,
-                   "bp: the row-pair and quad layouts need the z-column kernel (variant 2)");
+    ARECON_REQUIRE(layout == ARECON_FP_ROWS || P->variant >= 2,
+                   "bp: the row-pair and quad layouts need a z-column kernel (variant 2 or 3)");
+    ARECON_REQUIRE(P->variant != 3 || layout != ARECON_FP_ROWS,
+                   "bp: the LDS-staged z-column kernel (variant 3) reads row pairs or quads");
     BPArgs a = P->a;
     a.vec4 = (a.ZN % 4 == 0) && ((reinterpret_cast<uintptr_t>(img) & 15) == 0);
     if (layout == ARECON_FP_ROWPAIRS) a.view_stride = 2LL * (a.ZL - 1) * a.YL;
@@ -1150,6 +1509,26 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
         if (P->variant == 0) {
             hipLaunchKernelGGL((bp_fdk3dw_vt<NZ, T>), dim3(grid), dim3(256), 0, s, img, fp,
                                P->vtab, P->zsets, P->vt_table, a);
+            return;
+        }
+    }
+    if constexpr (NZ == 8 || NZ == 16) {
+        if (P->variant == 3) {
+            // rows staged per view: the row span of NZ slices plus margins
+            constexpr int RMAX = NZ == 8 ? 24 : 40;
+            if constexpr (std::is_same<T, float>::value) {
+                ARECON_REQUIRE(layout == ARECON_FP_ROWPAIRS,
+                               "bp: variant 3 reads FP32 row pairs or 16-bit quads");
+                hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, float, ARECON_FP_ROWPAIRS, RMAX>),
+                                   dim3(grid), dim3(256), 0, s, img, fp, P->vtab, P->zblocks,
+                                   P->zmasks, P->vt_table, a);
+            } else {
+                ARECON_REQUIRE(layout == ARECON_FP_QUADS,
+                               "bp: variant 3 reads FP32 row pairs or 16-bit quads");
+                hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, ARECON_FP_QUADS, RMAX>), dim3(grid),
+                                   dim3(256), 0, s, img, fp, P->vtab, P->zblocks, P->zmasks,
+                                   P->vt_table, a);
+            }
             return;
         }
     }

@@ -121,6 +121,15 @@ class BPPlan:
             self._h, C.c_void_p(img.data_ptr()), C.c_void_p(fp.data_ptr()), dtype,
             LAYOUT[layout], b0, b1, _stream_ptr(stream)), "arecon_bp_plan_execute_blocks")
 
+    def stats(self) -> tuple[int, int]:
+        """LDS-staged kernel (variant 3): workgroups that read their views from
+        LDS and workgroups that fell back to global gathers, over every execute
+        so far (both 0 for other variants)."""
+        a, b = C.c_ulonglong(), C.c_ulonglong()
+        _lib.check(_lib.lib().arecon_bp_plan_stats(self._h, C.byref(a), C.byref(b)),
+                   "arecon_bp_plan_stats")
+        return a.value, b.value
+
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib.lib().arecon_bp_plan_destroy(self._h)

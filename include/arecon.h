@@ -178,7 +178,10 @@ typedef struct arecon_bp_geom {
                                    1/2/4/8/16; variant 2: 8/16/32 */
     int order;                  /* workgroup order: 0 = slice-major, 1 = column-major */
     int variant;                /* 0 = view-table kernel (default), 1 = per-voxel weight,
-                                   2 = z-column kernel (slice blocks with view masks) */
+                                   2 = z-column kernel (slice blocks with view masks),
+                                   3 = z-column kernel reading each view's detector
+                                   footprint from LDS (nz 8 or 16; FP32 row pairs or
+                                   16-bit quads; ZL <= 250) */
     int group;                  /* z-column kernel: 1/2/4 = slices per branch-free
                                    group; 8 = each slice behind its own window test,
                                    all gathers of a view issued before any blend
@@ -213,6 +216,14 @@ int arecon_bp_plan_block_views(const arecon_bp_plan* plan, int block, int* pmin,
 int arecon_bp_plan_execute_blocks(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
                                   int fp_dtype, int fp_layout, int block_begin, int block_end,
                                   void* stream);
+
+/* Diagnostics of the LDS-staged kernel (variant 3; no counterpart in the
+ * reference): workgroups that read their views from LDS (*staged) and those
+ * that fell back to global-memory gathers because a view's footprint did not
+ * fit (*direct), summed over every execute of the plan so far.  Synchronises
+ * the device.  Both are 0 for other variants. */
+int arecon_bp_plan_stats(const arecon_bp_plan* plan, unsigned long long* staged,
+                         unsigned long long* direct);
 
 /* Batched backprojection (no counterpart in the reference, whose MEX
  * backprojects one volume per call): n plans -- z-slabs of one or more scans,
