@@ -267,11 +267,41 @@ class Net:
               self.stream())
         return y
 
-    def correct(self, x0: torch.Tensor, bound: float = 0.3, block: int = 64):
+    def apply_tiled(self, x0: torch.Tensor, tile: int = 64, stride: int = 32,
+                    batch: int = 16) -> torch.Tensor:
+        """x0 + c(x0) evaluated on overlapping tile^3 crops, as in training: each
+        tile keeps its central stride^3 region (its outer part at the volume's
+        faces), so every voxel is predicted with the context a training patch
+        gives its centre.  x0 is [1][D][H][W] with D, H, W >= tile."""
+        assert x0.shape[0] == 1 and min(x0.shape[1:]) >= tile
+        m = (tile - stride) // 2
+
+        def starts(n):
+            k = [i * stride for i in range(max(1, -(-(n - tile) // stride) + 1))]
+            return sorted({min(s, n - tile) for s in k})
+
+        def keep(s, n):
+            return (0 if s == 0 else s + m), (n if s + tile >= n else s + tile - m)
+
+        D, H, W = x0.shape[1:]
+        y = torch.empty_like(x0)
+        boxes = [(a, b, c) for a in starts(D) for b in starts(H) for c in starts(W)]
+        for i in range(0, len(boxes), batch):
+            grp = boxes[i:i + batch]
+            xt = torch.stack([x0[0, a:a + tile, b:b + tile, c:c + tile] for a, b, c in grp])
+            yt = self.apply(xt.contiguous())
+            for k, (a, b, c) in enumerate(grp):
+                (a0, a1), (b0, b1), (c0, c1) = keep(a, D), keep(b, H), keep(c, W)
+                y[0, a0:a1, b0:b1, c0:c1] = yt[k, a0 - a:a1 - a, b0 - b:b1 - b, c0 - c:c1 - c]
+        return y
+
+    def correct(self, x0: torch.Tensor, bound: float = 0.3, block: int = 64,
+                tiled: bool = False):
         """Corrected volume with the plain-FDK fallback: blocks of block^3 voxels
         (the last ones partial) whose correction RMS exceeds ``bound`` keep x0.
-        Returns (volume, fraction of blocks that fell back)."""
-        y = self.apply(x0)
+        ``tiled`` evaluates the network on overlapping 64^3 tiles
+        (``apply_tiled``).  Returns (volume, fraction of blocks that fell back)."""
+        y = self.apply_tiled(x0) if tiled else self.apply(x0)
         c = y - x0
         B, D, H, W = x0.shape
         nb = [-(-n // block) for n in (D, H, W)]

@@ -1,12 +1,18 @@
 """Training data, training and evaluation of the learned correction.
 
 Pairs are made by the method itself: a seeded random-ellipsoid phantom, its
-exact helical projections on the 368x16 curved detector (720 views/rotation,
-pitch drawn from [0.5, 1.5]), reconstructed by the plain GPU pipeline into a
-192x192x64 volume of 1 mm voxels; the target is the phantom sampled on the
-same grid (2x2x2 supersampled).  Values are HU / 1000.  The network
-(``cnn.Net``) learns the residual target - FDK on 64^3 patches; the plain FDK
-is both its input and the yardstick of its error.
+exact helical projections, reconstructed by the plain GPU pipeline; the target
+is the phantom sampled on the same grid (2x2x2 supersampled).  Values are
+HU / 1000.  Two scanners:
+
+* ``PairSpec()``: the 368x16 curved detector (720 views/rotation, pitch drawn
+  from [0.5, 1.5]) into a 192x192x64 volume of 1 mm voxels;
+* ``clinical_spec()``: the 888x64 detector (984 views/rotation, pitch drawn
+  from [0.5, 1.5]) into a 192x192x64 crop of the 500 mm field at
+  0.977 x 0.977 x 1.25 mm, centred at a random point of the body.
+
+The network (``cnn.Net``) learns the residual target - FDK on 64^3 patches;
+the plain FDK is both its input and the yardstick of its error.
 """
 from __future__ import annotations
 
@@ -31,6 +37,15 @@ class PairSpec:
     nz: int = 64
     dx: float = 1.0
     radius: float = 88.0          # phantom support (inside the fan's 106.7 mm FOV)
+    scanner: str = "368x16"       # or "888x64" (crops of the 500 mm field)
+    dz: float | None = None       # slice pitch (default dx)
+    field: float = 250.0          # 888x64: radius of the body's field
+    body_z_half: float = 160.0    # 888x64: half length of the random body's ellipsoid range
+
+
+def clinical_spec(nx: int = 192, nz: int = 64) -> PairSpec:
+    """Crops of the 888x64 scanner's 512x512 grid (500 mm field, 1.25 mm slices)."""
+    return PairSpec(nx=nx, nz=nz, dx=500.0 / 512, radius=None, scanner="888x64", dz=1.25)
 
 
 def scan_setup(pitch: float, spec: PairSpec):
@@ -84,8 +99,47 @@ def voxelize_gpu(E: np.ndarray, xs: torch.Tensor, ys: torch.Tensor, zs: torch.Te
     return out / ss ** 3
 
 
+def make_clinical_pair(seed: int, spec: PairSpec, pitch: float | None = None):
+    """A crop of a random body on the 888x64 scanner: the scan covers the crop's
+    slices (plus the windows and the rebinning margin), the crop's centre is
+    drawn so that it stays inside the body."""
+    rng = np.random.default_rng(seed + 7919)
+    pitch = float(rng.uniform(0.5, 1.5)) if pitch is None else pitch
+    dz = spec.dz or spec.dx
+    h = pitch * 64 * 1.0963
+    rotations = math.ceil(spec.nz * dz / h + 1.0) + 1
+    hdr = G._header(888, 64, 984, rotations, pitch)
+    half = 0.5 * spec.nx * spec.dx
+    # crop centre uniform in the body's cross-section (0.98 x 0.8 of the field),
+    # so crops also hold the body's edge and the air around it
+    r, ang = 0.98 * spec.field * math.sqrt(rng.uniform()), rng.uniform(0, 2 * math.pi)
+    xc, yc = r * math.cos(ang), 0.8 / 0.98 * r * math.sin(ang)
+    wl = G.Workload(f"888x64 crop pitch {pitch:.3f}", hdr, spec.nx, spec.nx, spec.nz, half,
+                    G.centred_ZC(hdr, spec.nz, dz) + 0.137)
+    s = wl.setup()
+    s.geom.Obj[4] = (spec.nx - 1) * 0.5 - xc / spec.dx + 1.0     # 1-based centres
+    s.geom.Obj[5] = (spec.nx - 1) * 0.5 - yc / spec.dx + 1.0
+    p = G.bp_params(G.bp_scan_geom(s))
+    z_mid = ((spec.nz - 1) * 0.5 - p.ZNC) * dz
+    # a long body as in the clinical scans (the body runs far beyond the scanned
+    # slab; a body that ends near the slab biases the plain FDK differently)
+    E = phantom.random_body(seed, fov_radius=spec.field, z_centre=z_mid, z_half=spec.body_z_half)
+    raw = phantom.project(E, s, hdr)
+    fdk = HelicalFDK(s, dz=dz)
+    vol = fdk.run(raw)
+    x0 = vol / MU_W - 1.0
+    dev = vol.device
+    xs = (torch.arange(p.XN, device=dev, dtype=torch.float32) - p.XNC) * p.dx
+    ys = (torch.arange(p.YN, device=dev, dtype=torch.float32) - p.YNC) * p.dy
+    zs = (torch.arange(p.ZN, device=dev, dtype=torch.float32) - p.ZNC) * dz
+    target = voxelize_gpu(E, xs, ys, zs) / MU_W - 1.0
+    return x0.contiguous(), target.contiguous(), pitch
+
+
 def make_pair(seed: int, spec: PairSpec = PairSpec(), pitch: float | None = None):
     """(plain FDK, phantom) in HU / 1000, FP32 [nx][nx][nz] on the device."""
+    if spec.scanner == "888x64":
+        return make_clinical_pair(seed, spec, pitch)
     rng = np.random.default_rng(seed + 7919)
     pitch = float(rng.uniform(0.5, 1.5)) if pitch is None else pitch
     wl, s = scan_setup(pitch, spec)
@@ -105,6 +159,9 @@ def make_pair(seed: int, spec: PairSpec = PairSpec(), pitch: float | None = None
 
 
 def fov_mask(spec: PairSpec, device) -> torch.Tensor:
+    """The phantom's support (368x16 pairs); every voxel of a clinical crop."""
+    if spec.radius is None:
+        return torch.ones((spec.nx, spec.nx, spec.nz), dtype=torch.bool, device=device)
     c = (torch.arange(spec.nx, device=device, dtype=torch.float32) - (spec.nx - 1) / 2) * spec.dx
     r2 = c[:, None] ** 2 + c[None, :] ** 2
     return (r2 <= spec.radius ** 2)[:, :, None].expand(spec.nx, spec.nx, spec.nz)
@@ -148,21 +205,37 @@ def train(net: Net, pairs, steps: int, batch: int = 4, patch: int = 64, lr: floa
     return hist
 
 
+def smooth_mask(target: torch.Tensor, tol: float = 0.02) -> torch.Tensor:
+    """Voxels whose phantom value is within ``tol`` (HU / 1000) of its 3x3x3
+    neighbourhood mean: away from the phantom's edges, where the error of a
+    reconstruction is artifact (streaks, bias, cupping) rather than the edge
+    blur of its resolution."""
+    t = target[None, None]
+    mean = torch.nn.functional.avg_pool3d(t, 3, 1, 1, count_include_pad=False)[0, 0]
+    return (target - mean).abs() <= tol
+
+
 def evaluate(net: Net, pairs, spec: PairSpec, bound: float = 0.3):
     """RMSE in HU against the phantom inside the support, for the plain FDK
-    and the corrected volume (with the fallback), summed over the pairs."""
+    and the corrected volume (with the fallback), summed over the pairs; the
+    ``artifact`` figures cover only the voxels of ``smooth_mask``."""
     m = fov_mask(spec, pairs[0][0].device)
-    se_fdk = se_cnn = 0.0
-    n = 0
+    se = {"fdk": 0.0, "cnn": 0.0, "fdk_s": 0.0, "cnn_s": 0.0}
+    n = ns = 0
     fb = []
     for x0, t, *_ in pairs:
         y, frac = net.correct(x0[None], bound=bound)
         y = y[0]
-        se_fdk += float(((x0 - t)[m] ** 2).sum())
-        se_cnn += float(((y - t)[m] ** 2).sum())
+        ms = m & smooth_mask(t)
+        se["fdk"] += float(((x0 - t)[m] ** 2).sum())
+        se["cnn"] += float(((y - t)[m] ** 2).sum())
+        se["fdk_s"] += float(((x0 - t)[ms] ** 2).sum())
+        se["cnn_s"] += float(((y - t)[ms] ** 2).sum())
         n += int(m.sum())
+        ns += int(ms.sum())
         fb.append(frac)
-    r_fdk = 1000.0 * math.sqrt(se_fdk / n)
-    r_cnn = 1000.0 * math.sqrt(se_cnn / n)
-    return {"rmse_fdk_hu": r_fdk, "rmse_cnn_hu": r_cnn, "reduction": 1 - r_cnn / r_fdk,
+    r = {k: 1000.0 * math.sqrt(v / (ns if k.endswith("_s") else n)) for k, v in se.items()}
+    return {"rmse_fdk_hu": r["fdk"], "rmse_cnn_hu": r["cnn"], "reduction": 1 - r["cnn"] / r["fdk"],
+            "artifact_rmse_fdk_hu": r["fdk_s"], "artifact_rmse_cnn_hu": r["cnn_s"],
+            "artifact_reduction": 1 - r["cnn_s"] / r["fdk_s"],
             "fallback_fraction": float(np.mean(fb))}

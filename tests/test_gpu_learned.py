@@ -45,3 +45,16 @@ def test_fallback_keeps_plain_fdk_when_the_correction_is_too_large(gpu, trained)
     x0 = held_out[0][0][None]
     y, frac = net.correct(x0, bound=0.0)        # every block exceeds a zero bound
     assert frac == 1.0 and torch.equal(y, x0)
+
+
+def test_clinical_crop_pairs_are_plain_fdk_and_phantom(gpu):
+    """A crop of a random body on the 888x64 scanner: the plain reconstruction
+    differs from the phantom by a reconstruction's error, not by a copy's or a
+    misplaced grid's."""
+    spec = learned.clinical_spec(nx=64, nz=32)
+    x0, t, pitch = learned.make_pair(3000, spec)
+    assert x0.shape == t.shape == (64, 64, 32) and 0.5 <= pitch <= 1.5
+    err = float(((x0 - t) ** 2).mean().sqrt()) * 1000
+    print(f"888x64 crop: RMSE vs phantom {err:.1f} HU", flush=True)
+    assert 5 < err < 500
+    assert float(t.abs().max()) > 0.05                    # structure inside the body
