@@ -69,6 +69,8 @@ struct BPArgs {
     long long view_stride;            // ZL * YL samples
     unsigned view_bytes;              // ZL * YL * sizeof(sample)
     unsigned long long* stats;        // LDS-staged kernel: workgroups staged / direct
+    int inside;                       // every column well inside the source circle and
+                                      // the detector (z-column kernels' INSIDE path)
 };
 
 // Two adjacent detector samples (channels U-1, U of one row) as floats.
@@ -599,6 +601,57 @@ __device__ __forceinline__ void zc_split(cptr<float> e, uint2 m, float K,
     zc_blend<0, ZB, Gt>(q, fr, ent, m, B, Dey, wu, acc);
 }
 
+// Per-(column, view) terms of the z-column kernels: the channel blend (channel
+// ub = U-1 of the reference, weight Dey on it, wu = 1 - Dey on channel U), the
+// row scale K = SO SD / (dZL t1 (t1 + UU)) and B = ((t1-UU)/(t1+UU))^k1 (only
+// when the view has a weighted slice).  uok: 0 < ceil(U1) < YL and a finite
+// magnification -- the reference's row test rejects the NaN / infinite Zr of a
+// voxel outside the source circle; K = 0 off the detector.
+// INSIDE: the host has shown that every column of the launch lies well inside
+// the source circle (t1 + UU > 0) and projects strictly inside (1, YL-2), so the
+// tests and clamps drop out; floor / fract pick the same two samples with the
+// same weights as ceil(U1) - 1 and ceil(U1) - U1 (for an integer U1 both give
+// the sample at U1).
+struct ColTerms {
+    float Dey, wu, K, B;
+    int ub;
+    bool uok;
+};
+template <bool INSIDE>
+__device__ __forceinline__ ColTerms column_terms(float x, float y, float2 cs, const BPArgs& A,
+                                                 bool weighted) {
+    ColTerms c;
+    const float UU = -x * cs.x - y * cs.y;
+    const float Yr = -x * cs.y + y * cs.x;
+    const float t1sq = A.SO2 - Yr * Yr;
+    const float temp1 = fast_sqrt(t1sq);
+    const float U1 = Yr * A.invdYL + A.YLC;
+    const float den = fmaf(UU, temp1, t1sq);              // t1 (t1 + UU)
+    const float inv = fast_rcp(den);
+    c.B = 1.f;
+    if constexpr (INSIDE) {
+        int ub;   // U1 comes from an FMA: no transcendental-result hazard to miss
+        asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(ub) : "v"(U1));
+        c.ub = ub;
+        c.wu = __builtin_amdgcn_fractf(U1);
+        c.Dey = 1.f - c.wu;
+        c.uok = true;
+        c.K = A.KSD * inv;
+        if (weighted) c.B = fast_exp2(A.k1 * fast_log2((temp1 - UU) * temp1 * inv));
+    } else {
+        c.uok = (U1 > 0.f) && (U1 <= A.YLm1) && (t1sq >= 0.f) && (den != 0.f);
+        const float Uc = ceilf(U1);
+        c.Dey = Uc - U1;
+        c.wu = 1.f - c.Dey;
+        c.ub = (int)Uc - 1;
+        c.K = c.uok ? A.KSD * inv : 0.f;
+        if (weighted)   // kept finite and nonzero
+            c.B = fminf(fmaxf(fast_exp2(A.k1 * fast_log2(fabsf((temp1 - UU) * temp1 * inv))),
+                              1e-30f), 1e30f);
+    }
+    return c;
+}
+
 // XCD-aware remap: hardware workgroup b runs on XCD b % 8; give each XCD a
 // contiguous run of logical workgroups.
 __device__ __forceinline__ int xcd_logical_block() {
@@ -624,7 +677,7 @@ __device__ __forceinline__ long long uniform_i64(long long x) {
 }
 
 // Body of the z-column kernel for logical workgroup lb of one plan.
-template <int ZB, int G, class T, int LAYOUT>
+template <int ZB, int G, class T, int LAYOUT, bool INSIDE>
 __device__ __forceinline__ void zc_run(int lb, const BPArgs& A, float* __restrict__ img,
                                        const T* __restrict__ fp,
                                        const float2* __restrict__ vtab,
@@ -676,27 +729,11 @@ __device__ __forceinline__ void zc_run(int lb, const BPArgs& A, float* __restric
         const uint2 m = cld_u2(mrow, p);               // scalar: active, weighted
         const float2 cs = cld_f2(vt, p);
         const __amdgpu_buffer_rsrc_t rs = view_rsrc(fp + (long long)p * view_stride, view_bytes);
-        const float UU = -x * cs.x - y * cs.y;
-        const float Yr = -x * cs.y + y * cs.x;
-        const float t1sq = A.SO2 - Yr * Yr;
-        const float temp1 = fast_sqrt(t1sq);
-        const float U1 = Yr * A.invdYL + A.YLC;
-        const float den = fmaf(UU, temp1, t1sq);              // t1 (t1 + UU)
-        // 0 < ceil(U1) < YL, and a finite magnification: the reference's row test
-        // rejects the NaN / infinite Zr of a voxel outside the source circle
-        const bool uok = (U1 > 0.f) && (U1 <= A.YLm1) && (t1sq >= 0.f) && (den != 0.f);
-        const float Uc = ceilf(U1);
-        const float Dey = Uc - U1;
-        const float wu = 1.f - Dey;
+        const ColTerms ct = column_terms<INSIDE>(x, y, cs, A, m.y != 0u);
+        const float Dey = ct.Dey, wu = ct.wu, K = ct.K, B = ct.B;
         // byte offset of channel U-1; off the detector, past any view (< 2 GiB) so
         // both gathers read 0 whatever the row
-        const unsigned ucolb = uok ? ((unsigned)Uc - 1u) * kElem : 0x80000000u;
-        const float inv = fast_rcp(den);
-        const float K = uok ? A.KSD * inv : 0.f;
-        float B = 1.f;
-        if (m.y != 0u)   // B = ((t1-UU)/(t1+UU))^k1, kept finite and nonzero
-            B = fminf(fmaxf(fast_exp2(A.k1 * fast_log2(fabsf((temp1 - UU) * temp1 * inv))),
-                            1e-30f), 1e30f);
+        const unsigned ucolb = ct.uok ? (unsigned)ct.ub * kElem : 0x80000000u;
         const cptr<float> erow = trow + 2LL * p * ZB;
         if constexpr (G == 0) {
             zc_split<ZB, T, LAYOUT>(erow, m, K, zlc, B, ucolb, rowb, zlm1, rs, Dey, wu, acc);
@@ -730,14 +767,14 @@ __device__ __forceinline__ void zc_run(int lb, const BPArgs& A, float* __restric
     }
 }
 
-template <int ZB, int G, class T, int LAYOUT>
+template <int ZB, int G, class T, int LAYOUT, bool INSIDE>
 __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
                                                     const T* __restrict__ fp,
                                                     const float2* __restrict__ vtab,
                                                     const ZBlock* __restrict__ blocks,
                                                     const uint2* __restrict__ masks,
                                                     const float2* __restrict__ table, BPArgs A) {
-    zc_run<ZB, G, T, LAYOUT>(xcd_logical_block(), A, img, fp, vtab, blocks, masks, table);
+    zc_run<ZB, G, T, LAYOUT, INSIDE>(xcd_logical_block(), A, img, fp, vtab, blocks, masks, table);
 }
 
 // ---------------------------------------------------------------------------
@@ -845,7 +882,7 @@ __device__ __forceinline__ unsigned zs_pack(const ZsView& v) {
            ((unsigned)(v.rB + 1) << 24);
 }
 
-template <int ZB, class T, int LAYOUT, int RMAX>
+template <int ZB, class T, int LAYOUT, int RMAX, bool INSIDE>
 __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restrict__ img,
                                        const T* __restrict__ fp,
                                        const float2* __restrict__ vtab,
@@ -894,7 +931,7 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     const int staged = __syncthreads_and(fits);
     if (threadIdx.x == 0 && A.stats) atomicAdd(A.stats + (staged ? 0 : 1), 1ull);
     if (!staged) {
-        zc_run<ZB, 0, T, LAYOUT>(lb0, A, img, fp, vtab, blocks, masks, table);
+        zc_run<ZB, 0, T, LAYOUT, INSIDE>(lb0, A, img, fp, vtab, blocks, masks, table);
         return;
     }
 
@@ -965,22 +1002,9 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             if (rB < rA) continue;                      // every sample would be zero
             const uint2 m = cld_u2(mrow, p);               // scalar: active, weighted
             const float2 cs = cld_f2(vt, p);
-            const float UU = -x * cs.x - y * cs.y;
-            const float Yr = -x * cs.y + y * cs.x;
-            const float t1sq = A.SO2 - Yr * Yr;
-            const float temp1 = fast_sqrt(t1sq);
-            const float U1 = Yr * A.invdYL + A.YLC;
-            const float den = fmaf(UU, temp1, t1sq);
-            const bool uok = (U1 > 0.f) && (U1 <= A.YLm1) && (t1sq >= 0.f) && (den != 0.f);
-            const float Uc = ceilf(U1);
-            const float Dey = Uc - U1;
-            const float wu = 1.f - Dey;
-            const float inv = fast_rcp(den);
-            const float K = uok ? A.KSD * inv : 0.f;
-            float B = 1.f;
-            if (m.y != 0u)
-                B = fminf(fmaxf(fast_exp2(A.k1 * fast_log2(fabsf((temp1 - UU) * temp1 * inv))),
-                                1e-30f), 1e30f);
+            const ColTerms ct = column_terms<INSIDE>(x, y, cs, A, m.y != 0u);
+            const float Dey = ct.Dey, wu = ct.wu, K = ct.K, B = ct.B;
+            const bool uok = ct.uok;
             const cptr<float> erow = trow + 2LL * p * ZB;
             float2 ent[ZB];
 #pragma unroll
@@ -994,26 +1018,31 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             asm("v_mov_b32 %0, %1" : "=v"(zterm) : "s"(4 * kSlot - rA * kZsRowBytes));
             int rBv;   // clamp bound: the zero row after the loaded rows
             asm("v_mov_b32 %0, %1" : "=v"(rBv) : "s"(rA + 4 * ((rB - rA + 4) / 4)));
-            const int colterm = uok ? (int)Uc * 8 + cterm : zterm;
-            // every slice, branch-free: a slice outside the view's window has
-            // temp2 = 1e30, so its row clamps onto the zero row (and a lane off the
-            // detector reads the zero record): it adds exactly zero
-            typename Gt::Raw q[ZB];
-            float fr[ZB];
+            const int colterm = uok ? ct.ub * 8 + 8 + cterm : zterm;
+            // groups of four slices, each skipped when none of its slices has the
+            // view in its window, and branch-free inside: a slice outside the
+            // window has temp2 = 1e30, so its row clamps onto the zero row (and a
+            // lane off the detector reads the zero record): it adds exactly zero
 #pragma unroll
-            for (int j = 0; j < ZB; ++j) {
-                const float V1 = fmaf(ent[j].x, K, zl);
-                int vf;
-                asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(vf) : "v"(V1));
-                fr[j] = __builtin_amdgcn_fractf(V1);
-                int r;   // one VGPR bound: a VOP3 reads at most one SGPR here
-                asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(vf), "s"(rA), "v"(rBv));
-                q[j] = Lg::load(lstage + (r * kZsRowBytes + colterm));
+            for (int g = 0; g < ZB; g += 4) {
+                if (((m.x >> g) & 0xfu) == 0u) continue;
+                typename Gt::Raw q[4];
+                float fr[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float V1 = fmaf(ent[g + i].x, K, zl);
+                    int vf;
+                    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(vf) : "v"(V1));
+                    fr[i] = __builtin_amdgcn_fractf(V1);
+                    int r;   // one VGPR bound: a VOP3 reads at most one SGPR here
+                    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(vf), "s"(rA), "v"(rBv));
+                    q[i] = Lg::load(lstage + (r * kZsRowBytes + colterm));
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    acc_weighted_dyn(acc[g + i], Gt::lerp(q[i], fr[i], Dey, wu),
+                                     (m.y >> (g + i)) & 1u, ent[g + i].y, B);
             }
-#pragma unroll
-            for (int j = 0; j < ZB; ++j)
-                acc_weighted_dyn(acc[j], Gt::lerp(q[j], fr[j], Dey, wu), (m.y >> j) & 1u,
-                                 ent[j].y, B);
         }
         __syncthreads();
     }
@@ -1033,14 +1062,14 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     }
 }
 
-template <int ZB, class T, int LAYOUT, int RMAX>
+template <int ZB, class T, int LAYOUT, int RMAX, bool INSIDE>
 __global__ __launch_bounds__(256) void bp_fdk3dw_zs(float* __restrict__ img,
                                                     const T* __restrict__ fp,
                                                     const float2* __restrict__ vtab,
                                                     const ZBlock* __restrict__ blocks,
                                                     const uint2* __restrict__ masks,
                                                     const float2* __restrict__ table, BPArgs A) {
-    zs_run<ZB, T, LAYOUT, RMAX>(xcd_logical_block(), A, img, fp, vtab, blocks, masks, table);
+    zs_run<ZB, T, LAYOUT, RMAX, INSIDE>(xcd_logical_block(), A, img, fp, vtab, blocks, masks, table);
 }
 
 // Batched z-column launch: several plans (z-slabs of one or more scans, each
@@ -1066,7 +1095,7 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc_batch(const SlabDesc* __rest
     int s = 0;
     while (s + 1 < nslabs && lb >= slabs[s + 1].wg_begin) ++s;
     const SlabDesc& d = slabs[s];
-    zc_run<ZB, G, T, LAYOUT>(lb - d.wg_begin, d.A, d.img, static_cast<const T*>(d.fp), d.vtab,
+    zc_run<ZB, G, T, LAYOUT, false>(lb - d.wg_begin, d.A, d.img, static_cast<const T*>(d.fp), d.vtab,
                              d.blocks, d.masks, d.table);
 }
 
@@ -1394,6 +1423,21 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
     A.view_stride = (long long)g.ZL * g.YL;
     const long long grid = (long long)A.ntiles * A.nzsets;
     P->grid = (int)grid;
+    {
+        // Largest radius of any column a workgroup computes (tiles padded to 16),
+        // in FP64.  Inside when t1 + UU >= sqrt(SO^2 - R^2) - R stays well above 0
+        // (R < SO / sqrt 2 with a margin) and every U1 = Yr / dYL + YLC lies in
+        // (1, YL - 2).
+        double R2 = 0.0;
+        const double xs[2] = {(0 - g.XNC) * g.dx, (A.tilesx * 16 - 1 - g.XNC) * g.dx};
+        const double ys[2] = {(0 - g.YNC) * g.dy, ((A.ntiles / A.tilesx) * 16 - 1 - g.YNC) * g.dy};
+        for (double xv : xs)
+            for (double yv : ys) R2 = std::max(R2, xv * xv + yv * yv);
+        const double R = std::sqrt(R2);
+        const double u = R / std::fabs(g.dYL);
+        A.inside = (R < 0.99 * g.SO / std::sqrt(2.0)) && (g.YLC - u > 1.0) &&
+                   (g.YLC + u < g.YL - 2.0) && std::isfinite(g.k1) ? 1 : 0;
+    }
     P->blk_pmin.assign(A.nzsets, 1);
     P->blk_pmax.assign(A.nzsets, 0);
     for (int zs = 0; zs < A.nzsets; ++zs) {
@@ -1516,27 +1560,32 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
         if (P->variant == 3) {
             // rows staged per view: the row span of NZ slices plus margins
             constexpr int RMAX = NZ == 8 ? 24 : 40;
-            if constexpr (std::is_same<T, float>::value) {
-                ARECON_REQUIRE(layout == ARECON_FP_ROWPAIRS,
-                               "bp: variant 3 reads FP32 row pairs or 16-bit quads");
-                hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, float, ARECON_FP_ROWPAIRS, RMAX>),
-                                   dim3(grid), dim3(256), 0, s, img, fp, P->vtab, P->zblocks,
-                                   P->zmasks, P->vt_table, a);
-            } else {
-                ARECON_REQUIRE(layout == ARECON_FP_QUADS,
-                               "bp: variant 3 reads FP32 row pairs or 16-bit quads");
-                hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, ARECON_FP_QUADS, RMAX>), dim3(grid),
-                                   dim3(256), 0, s, img, fp, P->vtab, P->zblocks, P->zmasks,
-                                   P->vt_table, a);
-            }
+            constexpr int LY = std::is_same<T, float>::value ? ARECON_FP_ROWPAIRS : ARECON_FP_QUADS;
+            ARECON_REQUIRE(layout == LY, "bp: variant 3 reads FP32 row pairs or 16-bit quads");
+            if (a.inside)
+                hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, true>), dim3(grid), dim3(256),
+                                   0, s, img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a);
+            else
+                hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, false>), dim3(grid), dim3(256),
+                                   0, s, img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a);
             return;
         }
     }
     if constexpr (NZ % 8 == 0 && NZ <= 32) {
         if (P->variant == 2) {
-#define ARECON_ZC(G, LY)                                                                      \
-    hipLaunchKernelGGL((bp_fdk3dw_zc<NZ, G, T, LY>), dim3(grid), dim3(256), 0, s, img, fp, \
+#define ARECON_ZC_(G, LY, IN)                                                                 \
+    hipLaunchKernelGGL((bp_fdk3dw_zc<NZ, G, T, LY, IN>), dim3(grid), dim3(256), 0, s, img, fp, \
                        P->vtab, P->zblocks, P->zmasks, P->vt_table, a)
+// split mode (G = 0) has the INSIDE specialisation
+#define ARECON_ZC(G, LY)                                                                      \
+    do {                                                                                      \
+        if constexpr (G == 0) {                                                               \
+            if (a.inside) ARECON_ZC_(0, LY, true);                                           \
+            else ARECON_ZC_(0, LY, false);                                                   \
+        } else {                                                                              \
+            ARECON_ZC_(G, LY, false);                                                        \
+        }                                                                                     \
+    } while (0)
             if (layout == ARECON_FP_ROWS) {
                 if (P->group == 1) ARECON_ZC(1, ARECON_FP_ROWS);
                 else if (P->group == 2) ARECON_ZC(2, ARECON_FP_ROWS);
@@ -1554,6 +1603,7 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
                 else if constexpr (NZ <= 16) ARECON_ZC(0, ARECON_FP_QUADS);
             }
 #undef ARECON_ZC
+#undef ARECON_ZC_
             return;
         }
     }
