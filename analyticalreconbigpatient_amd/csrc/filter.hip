@@ -28,6 +28,8 @@
 // HBM roofline: 8 B per sample FP32 -> FP32 (6 B with a 16-bit output).
 #include "arecon_internal.h"
 
+#include <array>
+
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -327,7 +329,7 @@ enum { DST_LDS = 0, DST_LDS_SPEC = 1, DST_GLOBAL = 2 };
 
 struct RowCtx {
     const float* ia; const float* ib;      // input rows (ib valid when has_b)
-    float ba, bb;                           // row heights for the preweight
+    const float* wa; const float* wb;      // their preweights (table rows ka, kb)
     bool has_b;
     int YL;
     float PLC, dYL, SO;
@@ -357,11 +359,8 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
             } else if (r < R / 2) {                          // n < N/2; n >= YL reads zero
                 float2 x = make_float2(0.f, 0.f);
                 if (n < c.YL) {
-                    const float t = ((float)n - c.PLC) * c.dYL;
-                    const float so2 = c.SO * c.SO;
-                    x.x = c.ia[n] * (so2 / sqrtf(so2 * (so2 + c.ba * c.ba) - c.ba * t * c.ba * t));
-                    if (c.has_b)
-                        x.y = c.ib[n] * (so2 / sqrtf(so2 * (so2 + c.bb * c.bb) - c.bb * t * c.bb * t));
+                    x.x = c.ia[n] * c.wa[n];
+                    if (c.has_b) x.y = c.ib[n] * c.wb[n];
                 }
                 v[b][r] = x;
             } else {
@@ -424,8 +423,8 @@ __device__ __forceinline__ void wave_fft(float2* lds, const float2* __restrict__
 template <int N, class OutT>
 __global__ __launch_bounds__(tpt<N>()) void filter_rows_wave(
     OutT* out, int layout, const float* in, const float2* __restrict__ twg,
-    const float* __restrict__ Hg, int YL, int ZL, long long nrows, float PLC, float ZLC,
-    float dYL, float dZL, float SO) {
+    const float* __restrict__ Hg, const float* __restrict__ Wg, int YL, int ZL,
+    long long nrows, float PLC, float ZLC, float dYL, float dZL, float SO) {
     __shared__ float2 lds[N];
     const long long npairs = (nrows + 1) >> 1;
     for (long long pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
@@ -436,8 +435,8 @@ __global__ __launch_bounds__(tpt<N>()) void filter_rows_wave(
         c.ib = c.ia + YL;
         const int ka = (int)(ra % ZL);
         const int kb = (ka + 1 == ZL) ? 0 : ka + 1;
-        c.ba = ((float)ka - ZLC) * dZL;
-        c.bb = ((float)kb - ZLC) * dZL;
+        c.wa = Wg + (long long)ka * YL;
+        c.wb = Wg + (long long)kb * YL;
         c.YL = YL; c.PLC = PLC; c.dYL = dYL; c.SO = SO; c.H = Hg;
         const RowOut<OutT> o = row_out(out, layout, ra, c.has_b, YL, ZL);
         wave_fft<N, 0, OutT>(lds, twg, c, o);
@@ -499,6 +498,53 @@ const FilterTables& filter_tables(int device, int YL, float dYL_f) {
     ARECON_HIP(hipMemcpy(t.tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
     ARECON_HIP(hipMemcpy(t.H, H.data(), sizeof(float) * N, hipMemcpyHostToDevice));
     return g_filter_cache.emplace(key, t).first->second;
+}
+
+// Cone and row preweight SO^2 / sqrt(SO^2 (SO^2 + b^2) - b^2 t^2) with
+// t = (j - PLC) dYL and b = (k - ZLC) dZL (src/filtering.cu:202-205), for every
+// row k and channel j: [ZL][YL], FP64 on the host, rounded to FP32.  The
+// reference's GPU path evaluates it per sample in FP32; its CPU twin
+// (src/analyticalReconBigPatient.m:109-115) in FP64.
+std::mutex g_pw_mu;
+std::map<std::array<uint32_t, 8>, float*> g_pw_cache;
+
+uint32_t fbits(float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    return u;
+}
+
+const float* preweight_table(int device, int YL, int ZL, float PLC, float ZLC, float dYL,
+                             float dZL, float SO) {
+    const std::array<uint32_t, 8> key{(uint32_t)device, (uint32_t)YL, (uint32_t)ZL,
+                                      fbits(PLC), fbits(ZLC), fbits(dYL), fbits(dZL),
+                                      fbits(SO)};
+    std::lock_guard<std::mutex> lk(g_pw_mu);
+    auto it = g_pw_cache.find(key);
+    if (it != g_pw_cache.end()) return it->second;
+    if (g_pw_cache.size() >= 64) {
+        ARECON_HIP(hipDeviceSynchronize());
+        for (auto& kv : g_pw_cache) {
+            (void)hipSetDevice((int)kv.first[0]);
+            (void)hipFree(kv.second);
+        }
+        g_pw_cache.clear();
+        ARECON_HIP(hipSetDevice(device));
+    }
+    std::vector<float> w((size_t)ZL * YL);
+    const double so2 = (double)SO * SO;
+    for (int k = 0; k < ZL; ++k) {
+        const double b = ((double)k - ZLC) * dZL;
+        for (int j = 0; j < YL; ++j) {
+            const double t = ((double)j - PLC) * dYL;
+            w[(size_t)k * YL + j] = (float)(so2 / std::sqrt(so2 * (so2 + b * b) - b * b * t * t));
+        }
+    }
+    float* d = nullptr;
+    ARECON_HIP(hipMalloc(&d, sizeof(float) * w.size()));
+    ARECON_HIP(hipMemcpy(d, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
+    g_pw_cache.emplace(key, d);
+    return d;
 }
 
 }  // namespace
@@ -630,18 +676,19 @@ static void launch_filter_rows(void* out, int out_dtype, int out_layout, const f
                                   hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
     });
     if (N >= 256 && N <= 2048) {
+        const float* W = preweight_table(dev, YL, ZL, PLC, ZLC, dYL, dZL, SO);
         const long long wgrid = std::min<long long>(npairs, (long long)cus * 16);
         auto go = [&](auto outp) {
             using OT = std::remove_pointer_t<decltype(outp)>;
             switch (N) {
             case 256: hipLaunchKernelGGL((filter_rows_wave<256, OT>), dim3((unsigned)wgrid), dim3(tpt<256>()), 0, s,
-                                         outp, out_layout, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+                                         outp, out_layout, in, t.tw, t.H, W, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
             case 512: hipLaunchKernelGGL((filter_rows_wave<512, OT>), dim3((unsigned)wgrid), dim3(tpt<512>()), 0, s,
-                                         outp, out_layout, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+                                         outp, out_layout, in, t.tw, t.H, W, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
             case 1024: hipLaunchKernelGGL((filter_rows_wave<1024, OT>), dim3((unsigned)wgrid), dim3(tpt<1024>()), 0, s,
-                                          outp, out_layout, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+                                          outp, out_layout, in, t.tw, t.H, W, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
             default: hipLaunchKernelGGL((filter_rows_wave<2048, OT>), dim3((unsigned)wgrid), dim3(tpt<2048>()), 0, s,
-                                        outp, out_layout, in, t.tw, t.H, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+                                        outp, out_layout, in, t.tw, t.H, W, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
             }
         };
         switch (out_dtype) {
