@@ -73,6 +73,7 @@ SIGNATURES = {
     "arecon_bp_plan_block_views": (_i, [_p, _i, C.POINTER(_i), C.POINTER(_i)]),
     "arecon_bp_plan_execute_blocks": (_i, [_p, _p, _p, _i, _i, _i, _i, _p]),
     "arecon_bp_plan_stats": (_i, [_p, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
+    "arecon_bp_plan_inside": (_i, [_p]),
     "arecon_bp_batch_create": (_i, [C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), _i, _i, _i,
                                     C.POINTER(_p)]),
     "arecon_bp_batch_destroy": (_i, [_p]),

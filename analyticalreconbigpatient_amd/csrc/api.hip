@@ -368,6 +368,10 @@ int arecon_bp_plan_block_views(const arecon_bp_plan* plan, int block, int* pmin,
     });
 }
 
+int arecon_bp_plan_inside(const arecon_bp_plan* plan) {
+    return bp_plan_inside(reinterpret_cast<const BPPlan*>(plan));
+}
+
 int arecon_bp_plan_stats(const arecon_bp_plan* plan, unsigned long long* staged,
                          unsigned long long* direct) {
     return guarded([&] { bp_plan_stats(reinterpret_cast<const BPPlan*>(plan), staged, direct); });

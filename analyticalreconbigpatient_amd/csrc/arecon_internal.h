@@ -95,6 +95,7 @@ int bp_plan_nz(const BPPlan* p);
 void bp_plan_launch(const BPPlan* p, float* img, const void* fp, int fp_dtype, int fp_layout,
                     hipStream_t s, int block_begin = 0, int block_end = -1);
 int bp_plan_nblocks(const BPPlan* p);
+int bp_plan_inside(const BPPlan* p);
 void bp_plan_stats(const BPPlan* p, unsigned long long* staged, unsigned long long* direct);
 void bp_plan_block_views(const BPPlan* p, int block, int* pmin, int* pmax);
 struct BPBatch;

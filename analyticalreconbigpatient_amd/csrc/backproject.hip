@@ -1485,6 +1485,8 @@ int bp_plan_nz(const BPPlan* P) { return P ? P->nz : 0; }
 
 int bp_plan_nblocks(const BPPlan* P) { return P ? P->a.nzsets : 0; }
 
+int bp_plan_inside(const BPPlan* P) { return P ? P->a.inside : 0; }
+
 void bp_plan_stats(const BPPlan* P, unsigned long long* staged, unsigned long long* direct) {
     ARECON_REQUIRE(P != nullptr && staged != nullptr && direct != nullptr, "bp: null argument");
     unsigned long long h[2] = {0ull, 0ull};

@@ -121,6 +121,12 @@ class BPPlan:
             self._h, C.c_void_p(img.data_ptr()), C.c_void_p(fp.data_ptr()), dtype,
             LAYOUT[layout], b0, b1, _stream_ptr(stream)), "arecon_bp_plan_execute_blocks")
 
+    @property
+    def inside(self) -> bool:
+        """Every column inside the source circle and the detector: the z-column
+        kernels run their specialisation without per-column validity tests."""
+        return bool(_lib.lib().arecon_bp_plan_inside(self._h))
+
     def stats(self) -> tuple[int, int]:
         """LDS-staged kernel (variant 3): workgroups that read their views from
         LDS and workgroups that fell back to global gathers, over every execute

@@ -62,11 +62,15 @@ def parse():
 
 
 def kernel_name(fdk) -> str:
-    """The backprojector's kernel as rocprofv3 names it."""
+    """The backprojector's kernel as rocprofv3 names it (FP32 storage)."""
+    lay = {"rows": 0, "rowpairs": 1, "quads": 2}[fdk.layout]
+    inside = "true" if fdk.plan.inside else "false"
     if fdk.variant == 2:
-        lay = 1 if fdk.layout == "rowpairs" else 0
         g = 0 if fdk.group == 8 else fdk.group
-        return f"bp_fdk3dw_zc<{fdk.plan_nz}, {g}, float, {lay}>"
+        return f"bp_fdk3dw_zc<{fdk.plan_nz}, {g}, float, {lay}, {g == 0 and inside or 'false'}>"
+    if fdk.variant == 3:
+        rmax = 24 if fdk.plan_nz == 8 else 40
+        return f"bp_fdk3dw_zs<{fdk.plan_nz}, float, {lay}, {rmax}, {inside}>"
     return f"bp_fdk3dw{'_vt' if fdk.variant == 0 else ''}<{fdk.plan_nz}, float>"
 
 

@@ -217,6 +217,12 @@ int arecon_bp_plan_execute_blocks(const arecon_bp_plan* plan, float* d_img, cons
                                   int fp_dtype, int fp_layout, int block_begin, int block_end,
                                   void* stream);
 
+/* 1 when every column of the plan's grid lies well inside the source circle
+ * and projects strictly inside the detector, so the z-column kernels (variants
+ * 2 and 3) run without per-column validity tests; 0 otherwise or for a null
+ * plan.  No counterpart in the reference (a diagnostic of this library). */
+int arecon_bp_plan_inside(const arecon_bp_plan* plan);
+
 /* Diagnostics of the LDS-staged kernel (variant 3; no counterpart in the
  * reference): workgroups that read their views from LDS (*staged) and those
  * that fell back to global-memory gathers because a view's footprint did not
