@@ -34,6 +34,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -323,9 +324,10 @@ template <int R> __device__ __forceinline__ void twiddle(float2* v, float2 w1) {
 }
 
 // Pass modes: first forward pass reads global rows (preweighted); last forward
-// pass multiplies by H and conjugates; last inverse pass stores rows.
+// pass multiplies by H and conjugates; last inverse pass stores rows, to global
+// memory or, as two real rows, to LDS (floats [0, YL) and [N, N + YL)).
 enum { SRC_LDS = 0, SRC_GLOBAL = 1 };
-enum { DST_LDS = 0, DST_LDS_SPEC = 1, DST_GLOBAL = 2 };
+enum { DST_LDS = 0, DST_LDS_SPEC = 1, DST_GLOBAL = 2, DST_LDS_ROWS = 3 };
 
 struct RowCtx {
     const float* ia; const float* ib;      // input rows (ib valid when has_b)
@@ -336,14 +338,18 @@ struct RowCtx {
     const float* H;
 };
 
-// Threads per transform: one wavefront up to N = 1024, two at N = 2048 (one
-// radix-16 butterfly per thread keeps the register count near 100).
-template <int N> constexpr int tpt() { return N >= 2048 ? N / 16 : 64; }
+// Threads per transform: one wavefront up to N = 1024.  At N = 2048, schedule
+// S = 0 uses two wavefronts, one radix-16 butterfly per thread (16, 16, 8);
+// S = 1 four wavefronts, one radix-8 butterfly per thread (8, 8, 8, 4): one
+// pass more, but fewer registers per thread and so more wavefronts per SIMD.
+template <int N, int S = 0> constexpr int tpt() {
+    return N >= 2048 ? (S == 1 ? N / 8 : N / 16) : 64;
+}
 
-template <int N, int R, int Ns, int SRC, int DST, class OutT>
+template <int N, int R, int Ns, int SRC, int DST, class OutT, int S>
 __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict__ twg,
                                           const RowCtx& c, const RowOut<OutT>& o) {
-    constexpr int T = tpt<N>();
+    constexpr int T = tpt<N, S>();
     constexpr int NB = N / R, PER = NB / T;
     static_assert(PER >= 1 && PER * T == NB, "radix schedule does not fit the thread count");
     const int lane = threadIdx.x;
@@ -384,6 +390,12 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
             } else if (DST == DST_LDS_SPEC) {
                 const float hs = c.H[k];
                 lds[k] = make_float2(hs * v[b][r].x, -hs * v[b][r].y);
+            } else if (DST == DST_LDS_ROWS) {
+                if (k < c.YL) {
+                    float* rows = reinterpret_cast<float*>(lds);
+                    rows[k] = v[b][r].x;
+                    rows[N + k] = -v[b][r].y;
+                }
             } else if (k < c.YL) {
                 o.put(k, v[b][r].x, -v[b][r].y);
             }
@@ -393,35 +405,41 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
 }
 
 // The whole forward (MODE 0: global rows -> spectrum in LDS) or inverse
-// (MODE 1: LDS -> global rows) transform of one pair of rows.
-template <int N, int MODE, class OutT>
+// (MODE 1: LDS -> global rows; MODE 2: LDS -> the two real rows in LDS)
+// transform of one pair of rows.
+template <int N, int MODE, class OutT, int S>
 __device__ __forceinline__ void wave_fft(float2* lds, const float2* __restrict__ twg,
                                          const RowCtx& c, const RowOut<OutT>& o) {
     constexpr int S0 = (MODE == 0) ? SRC_GLOBAL : SRC_LDS;
-    constexpr int DL = (MODE == 0) ? DST_LDS_SPEC : DST_GLOBAL;
-    if constexpr (N == 2048) {
-        wave_pass<N, 16, 1, S0, DST_LDS, OutT>(lds, twg, c, o);
-        wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, o);
-        wave_pass<N, 8, 256, SRC_LDS, DL, OutT>(lds, twg, c, o);
+    constexpr int DL = (MODE == 0) ? DST_LDS_SPEC : (MODE == 1 ? DST_GLOBAL : DST_LDS_ROWS);
+    if constexpr (N == 2048 && S == 1) {
+        wave_pass<N, 8, 1, S0, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 8, 8, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 8, 64, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 4, 512, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
+    } else if constexpr (N == 2048) {
+        wave_pass<N, 16, 1, S0, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 8, 256, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
     } else if constexpr (N == 1024) {
-        wave_pass<N, 16, 1, S0, DST_LDS, OutT>(lds, twg, c, o);
-        wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, o);
-        wave_pass<N, 4, 256, SRC_LDS, DL, OutT>(lds, twg, c, o);
+        wave_pass<N, 16, 1, S0, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 4, 256, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
     } else if constexpr (N == 512) {
-        wave_pass<N, 8, 1, S0, DST_LDS, OutT>(lds, twg, c, o);
-        wave_pass<N, 8, 8, SRC_LDS, DST_LDS, OutT>(lds, twg, c, o);
-        wave_pass<N, 8, 64, SRC_LDS, DL, OutT>(lds, twg, c, o);
+        wave_pass<N, 8, 1, S0, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 8, 8, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 8, 64, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
     } else {
         static_assert(N == 256, "wave FFT supports N = 256 .. 2048");
-        wave_pass<N, 4, 1, S0, DST_LDS, OutT>(lds, twg, c, o);
-        wave_pass<N, 4, 4, SRC_LDS, DST_LDS, OutT>(lds, twg, c, o);
-        wave_pass<N, 4, 16, SRC_LDS, DST_LDS, OutT>(lds, twg, c, o);
-        wave_pass<N, 4, 64, SRC_LDS, DL, OutT>(lds, twg, c, o);
+        wave_pass<N, 4, 1, S0, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 4, 4, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 4, 16, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 4, 64, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
     }
 }
 
-template <int N, class OutT>
-__global__ __launch_bounds__(tpt<N>()) void filter_rows_wave(
+template <int N, class OutT, int S>
+__global__ __launch_bounds__((tpt<N, S>())) void filter_rows_wave(
     OutT* out, int layout, const float* in, const float2* __restrict__ twg,
     const float* __restrict__ Hg, const float* __restrict__ Wg, int YL, int ZL,
     long long nrows, float PLC, float ZLC, float dYL, float dZL, float SO) {
@@ -439,8 +457,8 @@ __global__ __launch_bounds__(tpt<N>()) void filter_rows_wave(
         c.wb = Wg + (long long)kb * YL;
         c.YL = YL; c.PLC = PLC; c.dYL = dYL; c.SO = SO; c.H = Hg;
         const RowOut<OutT> o = row_out(out, layout, ra, c.has_b, YL, ZL);
-        wave_fft<N, 0, OutT>(lds, twg, c, o);
-        wave_fft<N, 1, OutT>(lds, twg, c, o);
+        wave_fft<N, 0, OutT, S>(lds, twg, c, o);
+        wave_fft<N, 1, OutT, S>(lds, twg, c, o);
     }
 }
 
@@ -595,6 +613,74 @@ __global__ __launch_bounds__(256) void expand_quads(OutT* __restrict__ out,
     }
 }
 
+// Filter and difference quads in one pass (N = 256 .. 2048).  A workgroup
+// takes whole views and walks their rows in pairs (k, k+1): the inverse
+// transform leaves both filtered rows in LDS, where they give quad rows k-1
+// (with row k-1, kept from the previous pair) and k.  The quads are the same
+// FP32 differences as expand_quads forms from stored rows, rounded once to
+// OutT, so the output is identical; the FP32 rows never reach HBM.
+template <int N, class OutT, int S>
+__global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
+    OutT* __restrict__ out, const float* in, const float2* __restrict__ twg,
+    const float* __restrict__ Hg, const float* __restrict__ Wg, int YL, int ZL, int ViewN,
+    float PLC, float dYL, float SO) {
+    constexpr int T = tpt<N, S>();
+    __shared__ float2 lds[N];
+    __shared__ float prev[2][N / 2];             // filtered row k-1, double-buffered
+    const float* la = reinterpret_cast<const float*>(lds);
+    const float* lb = la + N;
+    const int nc = YL - 1;
+    RowOut<OutT> o{};                            // the rows stay in LDS
+    for (int v = blockIdx.x; v < ViewN; v += gridDim.x) {
+        OutT* ov = out + (long long)v * (ZL - 1) * nc * 4;
+        for (int k = 0; k < ZL; k += 2) {
+            RowCtx c;
+            c.has_b = k + 1 < ZL;
+            c.ia = in + ((long long)v * ZL + k) * YL;
+            c.ib = c.ia + YL;
+            c.wa = Wg + (long long)k * YL;
+            c.wb = Wg + (long long)(c.has_b ? k + 1 : 0) * YL;
+            c.YL = YL; c.PLC = PLC; c.dYL = dYL; c.SO = SO; c.H = Hg;
+            wave_fft<N, 0, OutT, S>(lds, twg, c, o);
+            wave_fft<N, 2, OutT, S>(lds, twg, c, o);
+            // rows k (la) and k+1 (lb) are in LDS, row k-1 in prev[(k/2) & 1]
+            // (k > 0); row k+1 goes to the other buffer for the next pair, which
+            // reads it only after the barriers of its transforms
+            const float* pr = prev[(k >> 1) & 1];
+            float* pn = prev[((k >> 1) & 1) ^ 1];
+            for (int ch = threadIdx.x; ch < YL; ch += T) {
+                const float a0 = la[ch];
+                const float b0 = c.has_b ? lb[ch] : 0.f;
+                if (c.has_b) pn[ch] = b0;
+                if (ch >= nc) continue;
+                const float a1 = la[ch + 1];
+                if (k > 0) {
+                    const float p0 = pr[ch], p1 = pr[ch + 1];
+                    const float dr = a0 - p0;
+                    Quad4<OutT>::put(ov + ((long long)(k - 1) * nc + ch) * 4, p0, dr, p1 - p0,
+                                     (a1 - p1) - dr);
+                }
+                if (c.has_b) {
+                    const float b1 = lb[ch + 1];
+                    const float dr = b0 - a0;
+                    Quad4<OutT>::put(ov + ((long long)k * nc + ch) * 4, a0, dr, a1 - a0,
+                                     (b1 - a1) - dr);
+                }
+            }
+            __syncthreads();     // every LDS read of this pair before the next writes
+        }
+    }
+}
+
+// FFT schedule of the 2048-point kernels (see tpt): 1 (radix 8, four
+// wavefronts; 3.05 ms against 3.63 ms for the 888 x 64 x 11,808 sinogram) unless
+// ARECON_FILTER_SCHEDULE=0 asks for the radix-16 schedule.  Both give
+// the same result up to FP32 rounding of the passes.  Read at every launch.
+static int filter_schedule() {
+    const char* e = std::getenv("ARECON_FILTER_SCHEDULE");
+    return (e && e[0] == '0') ? 0 : 1;
+}
+
 static void launch_filter_rows(void* out, int out_dtype, int out_layout, const float* in,
                                int YL, int ZL, int ViewN, float PLC, float ZLC, float dYL,
                                float dZL, float SO, hipStream_t s);
@@ -610,12 +696,49 @@ void launch_filter(void* out, int out_dtype, int out_layout, const float* in, in
     ARECON_REQUIRE(YL >= 2 && ZL >= 2, "filter: the quad layout needs YL >= 2 and ZL >= 2");
     ARECON_REQUIRE(out != nullptr && out != (const void*)in,
                    "filter: the quad output cannot alias the input");
+    ARECON_REQUIRE(YL > 0 && ViewN > 0, "filter: YL, ZL and ViewN must be positive");
+    ARECON_REQUIRE(dYL != 0.f && SO > 0.f, "filter: dYL must be nonzero and SO positive");
+    ARECON_REQUIRE(out_dtype == ARECON_F32 || out_dtype == ARECON_F16 || out_dtype == ARECON_BF16,
+                   "filter: unknown output dtype");
+    int dev = 0;
+    ARECON_HIP(hipGetDevice(&dev));
+    const int N = filter_length(YL);
+    if (N >= 256 && N <= 2048) {      // fused: filtered rows stay in LDS
+        const FilterTables& t = filter_tables(dev, YL, dYL);
+        const float* W = preweight_table(dev, YL, ZL, PLC, ZLC, dYL, dZL, SO);
+        const unsigned grid = (unsigned)ViewN;
+        auto go = [&](auto outp) {
+            using OT = std::remove_pointer_t<decltype(outp)>;
+            switch (N) {
+            case 256: hipLaunchKernelGGL((filter_quads_wave<256, OT, 0>), dim3(grid), dim3(tpt<256>()), 0, s,
+                                         outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
+            case 512: hipLaunchKernelGGL((filter_quads_wave<512, OT, 0>), dim3(grid), dim3(tpt<512>()), 0, s,
+                                         outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
+            case 1024: hipLaunchKernelGGL((filter_quads_wave<1024, OT, 0>), dim3(grid), dim3(tpt<1024>()), 0, s,
+                                          outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
+            default:
+                if (filter_schedule() == 1)
+                    hipLaunchKernelGGL((filter_quads_wave<2048, OT, 1>), dim3(grid), dim3(tpt<2048, 1>()), 0, s,
+                                       outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO);
+                else
+                    hipLaunchKernelGGL((filter_quads_wave<2048, OT, 0>), dim3(grid), dim3(tpt<2048, 0>()), 0, s,
+                                       outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO);
+                break;
+            }
+        };
+        switch (out_dtype) {
+        case ARECON_F32: go(static_cast<float*>(out)); break;
+        case ARECON_F16: go(static_cast<__half*>(out)); break;
+        default: go(static_cast<__hip_bfloat16*>(out)); break;
+        }
+        ARECON_HIP(hipGetLastError());
+        return;
+    }
     float* rows = const_cast<float*>(in);
     launch_filter_rows(rows, ARECON_F32, ARECON_FP_ROWS, in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL,
                        SO, s);
     const long long n = (long long)ViewN * (ZL - 1) * (YL - 1);
-    int dev = 0, cus = 0;
-    ARECON_HIP(hipGetDevice(&dev));
+    int cus = 0;
     ARECON_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const long long grid = std::max<long long>(1, std::min<long long>((n + 255) / 256,
                                                                       (long long)cus * 64));
@@ -681,14 +804,20 @@ static void launch_filter_rows(void* out, int out_dtype, int out_layout, const f
         auto go = [&](auto outp) {
             using OT = std::remove_pointer_t<decltype(outp)>;
             switch (N) {
-            case 256: hipLaunchKernelGGL((filter_rows_wave<256, OT>), dim3((unsigned)wgrid), dim3(tpt<256>()), 0, s,
+            case 256: hipLaunchKernelGGL((filter_rows_wave<256, OT, 0>), dim3((unsigned)wgrid), dim3(tpt<256>()), 0, s,
                                          outp, out_layout, in, t.tw, t.H, W, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
-            case 512: hipLaunchKernelGGL((filter_rows_wave<512, OT>), dim3((unsigned)wgrid), dim3(tpt<512>()), 0, s,
+            case 512: hipLaunchKernelGGL((filter_rows_wave<512, OT, 0>), dim3((unsigned)wgrid), dim3(tpt<512>()), 0, s,
                                          outp, out_layout, in, t.tw, t.H, W, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
-            case 1024: hipLaunchKernelGGL((filter_rows_wave<1024, OT>), dim3((unsigned)wgrid), dim3(tpt<1024>()), 0, s,
+            case 1024: hipLaunchKernelGGL((filter_rows_wave<1024, OT, 0>), dim3((unsigned)wgrid), dim3(tpt<1024>()), 0, s,
                                           outp, out_layout, in, t.tw, t.H, W, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
-            default: hipLaunchKernelGGL((filter_rows_wave<2048, OT>), dim3((unsigned)wgrid), dim3(tpt<2048>()), 0, s,
-                                        outp, out_layout, in, t.tw, t.H, W, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO); break;
+            default:
+                if (filter_schedule() == 1)
+                    hipLaunchKernelGGL((filter_rows_wave<2048, OT, 1>), dim3((unsigned)wgrid), dim3(tpt<2048, 1>()), 0, s,
+                                       outp, out_layout, in, t.tw, t.H, W, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO);
+                else
+                    hipLaunchKernelGGL((filter_rows_wave<2048, OT, 0>), dim3((unsigned)wgrid), dim3(tpt<2048, 0>()), 0, s,
+                                       outp, out_layout, in, t.tw, t.H, W, YL, ZL, nrows, PLC, ZLC, dYL, dZL, SO);
+                break;
             }
         };
         switch (out_dtype) {

@@ -26,6 +26,7 @@
 
 #include <array>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -196,7 +197,7 @@ template <int JB>
 __global__ __launch_bounds__(256) void rebin_vzy(
     float* __restrict__ out, const float* __restrict__ in,
     const ChanCoef* __restrict__ chan, int YL, int ZL, int ViewN, float DeltaTheta,
-    float DeltaFai, int view_begin) {
+    float DeltaFai, int view_begin, int vec4) {
     extern __shared__ float tile[];             // ZL * (JB + 1)
     __shared__ Blend sb[JB];
     __shared__ int sbase[JB][4];
@@ -214,16 +215,38 @@ __global__ __launch_bounds__(256) void rebin_vzy(
         sbase[tid][3] = b.ixu * YL + cc.iyu;
     }
     __syncthreads();
-    const int n = nj * ZL;
-    for (int e = tid; e < n; e += blockDim.x) {
-        const int jj = e / ZL;
-        const int k = e - jj * ZL;
-        const Blend b = sb[jj];
-        const float p0 = in[(long long)sbase[jj][0] * ZL + k];
-        const float p1 = in[(long long)sbase[jj][1] * ZL + k];
-        const float p2 = in[(long long)sbase[jj][2] * ZL + k];
-        const float p3 = in[(long long)sbase[jj][3] * ZL + k];
-        tile[k * (JB + 1) + jj] = blend4(b, p0, p1, p2, p3);
+    if (vec4) {   // ZL % 4 == 0 and a 16-byte aligned input
+        // four rows per thread: 16-byte taps (a quarter of the address work of
+        // 4-byte loads; the 4-tap gather is otherwise bound by it)
+        const int ZL4 = ZL >> 2;
+        const int n = nj * ZL4;
+        const float4* in4 = reinterpret_cast<const float4*>(in);
+        for (int e = tid; e < n; e += blockDim.x) {
+            const int jj = e / ZL4;
+            const int k4 = e - jj * ZL4;
+            const Blend b = sb[jj];
+            const float4 p0 = in4[(long long)sbase[jj][0] * ZL4 + k4];
+            const float4 p1 = in4[(long long)sbase[jj][1] * ZL4 + k4];
+            const float4 p2 = in4[(long long)sbase[jj][2] * ZL4 + k4];
+            const float4 p3 = in4[(long long)sbase[jj][3] * ZL4 + k4];
+            float* t = tile + 4 * k4 * (JB + 1) + jj;
+            t[0] = blend4(b, p0.x, p1.x, p2.x, p3.x);
+            t[JB + 1] = blend4(b, p0.y, p1.y, p2.y, p3.y);
+            t[2 * (JB + 1)] = blend4(b, p0.z, p1.z, p2.z, p3.z);
+            t[3 * (JB + 1)] = blend4(b, p0.w, p1.w, p2.w, p3.w);
+        }
+    } else {
+        const int n = nj * ZL;
+        for (int e = tid; e < n; e += blockDim.x) {
+            const int jj = e / ZL;
+            const int k = e - jj * ZL;
+            const Blend b = sb[jj];
+            const float p0 = in[(long long)sbase[jj][0] * ZL + k];
+            const float p1 = in[(long long)sbase[jj][1] * ZL + k];
+            const float p2 = in[(long long)sbase[jj][2] * ZL + k];
+            const float p3 = in[(long long)sbase[jj][3] * ZL + k];
+            tile[k * (JB + 1) + jj] = blend4(b, p0, p1, p2, p3);
+        }
     }
     __syncthreads();
     for (int e = tid; e < ZL * JB; e += blockDim.x) {
@@ -272,8 +295,9 @@ void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN, float 
         const size_t lds = sizeof(float) * (size_t)ZL * (JB + 1);
         ARECON_REQUIRE(lds <= 64 * 1024, "rebin: ZL too large for the transposing kernel");
         dim3 grid(view_count, (YL + JB - 1) / JB);
+        const int vec4 = (ZL % 4 == 0) && ((reinterpret_cast<uintptr_t>(in) & 15) == 0);
         hipLaunchKernelGGL(rebin_vzy<JB>, grid, dim3(256), lds, s, out, in,
-                           chan, YL, ZL, ViewN, DeltaTheta, DeltaFai, view_begin);
+                           chan, YL, ZL, ViewN, DeltaTheta, DeltaFai, view_begin, vec4);
     }
     ARECON_HIP(hipGetLastError());
 }

@@ -156,8 +156,10 @@ int arecon_rebin_views_dev(float* d_out, const float* d_in, int YL, int ZL, int 
 /* Preweight + ramp filter on device buffers; input layout [ViewN][ZL][YL],
  * output layout out_layout (ARECON_FP_ROWS, ARECON_FP_ROWPAIRS or
  * ARECON_FP_QUADS).  d_out may equal d_in only for ARECON_F32 rows.  For
- * ARECON_FP_QUADS the filter runs in place in d_in (which then holds the FP32
- * filtered rows) and the quads are derived from it. */
+ * ARECON_FP_QUADS with YL <= 1024 the quads are formed from the filtered rows
+ * in LDS and d_in is left unchanged; above that the filter runs in place in
+ * d_in (which then holds the FP32 filtered rows) and the quads are derived from
+ * it, so callers must treat d_in as scratch for the quad layout. */
 int arecon_filter_dev(void* d_out, int out_dtype, int out_layout, const float* d_in, int YL,
                       int ZL, int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
                       void* stream);

@@ -119,21 +119,66 @@ def test_filter_row_pair_layout(gpu, ViewN, ZL, YL, dtype):
     assert np.array_equal(pairs[..., 1], rows[:, 1:, :])
 
 
-@pytest.mark.parametrize("ViewN,ZL,YL", [(9, 64, 888), (7, 3, 20), (4, 2, 2048)])
-@pytest.mark.parametrize("dtype", [_lib.ARECON_F32, _lib.ARECON_F16])
+# YL <= 1024: fused kernel (filtered rows kept in LDS, one workgroup per view),
+# every transform length 256..2048, odd ZL, ZL = 2 and YL = N/2; YL = 20 and
+# 2048: filter in place, then expand_quads.
+@pytest.mark.parametrize("ViewN,ZL,YL", [(9, 64, 888), (5, 5, 130), (4, 7, 200), (6, 3, 300),
+                                         (3, 2, 1024), (5, 4, 128), (7, 3, 20), (4, 2, 2048)])
+@pytest.mark.parametrize("dtype", [_lib.ARECON_F32, _lib.ARECON_F16, _lib.ARECON_BF16])
 def test_filter_quad_layout(gpu, ViewN, ZL, YL, dtype):
     """Quad output = the differences of the filtered rows, rounded once to the
-    output type (the filter runs in place in its FP32 input)."""
+    output type.  With even ZL both filters transform the same row pairs
+    (a + ib), so the quads are bit-identical to those formed from the rows
+    output.  With odd ZL the rows filter pairs the last row of a view with the
+    first of the next, the fused one with zeros; the rounding of a transform
+    depends on its partner row, so the two agree to FP32 rounding only."""
     print(f"filter quads ViewN={ViewN} ZL={ZL} YL={YL} dtype={dtype}", flush=True)
     P = np.random.default_rng(YL + 7 * ZL).normal(size=(ViewN, ZL, YL)).astype(np.float32)
     args = ((YL - 1) / 2.0, (ZL - 1) / 2.0, 0.9493, 1.9231)
     rows = run_filter(P, *args)
     got = run_filter(P, *args, dtype=dtype, layout=_lib.ARECON_FP_QUADS)
     want = quads(rows)
-    if dtype == _lib.ARECON_F32:
-        assert np.array_equal(got, want)
+    cast = {_lib.ARECON_F32: lambda a: a,
+            _lib.ARECON_F16: lambda a: torch.from_numpy(a).half().float().numpy(),
+            _lib.ARECON_BF16: lambda a: torch.from_numpy(a).bfloat16().float().numpy()}[dtype]
+    if ZL % 2 == 0 or YL > 1024:
+        assert np.array_equal(got, cast(want))
     else:
-        assert np.array_equal(got, torch.from_numpy(want).half().float().numpy())
+        tol = {_lib.ARECON_F32: 1e-6, _lib.ARECON_F16: 1e-3, _lib.ARECON_BF16: 8e-3}[dtype]
+        assert rel_l2(got, want) < tol
+        want64 = quads(oracle.filter_f64(P.astype(np.float64),
+                                         *[float(np.float32(v)) for v in args], SO))
+        assert rel_l2(got, want64) < max(tol, 1e-5)
+
+
+def test_filter_quads_leave_input_unchanged(gpu):
+    """The fused quad filter (YL <= 1024) only reads its input."""
+    P = np.random.default_rng(5).normal(size=(6, 8, 400)).astype(np.float32)
+    din = torch.from_numpy(P).cuda()
+    dout = torch.empty((6, 7, 399, 4), dtype=torch.float16, device="cuda")
+    _lib.check(_lib.lib().arecon_filter_dev(
+        C.c_void_p(dout.data_ptr()), _lib.ARECON_F16, _lib.ARECON_FP_QUADS,
+        C.c_void_p(din.data_ptr()), 400, 8, 6, np.float32(199.5), np.float32(3.5),
+        np.float32(0.95), np.float32(1.92), np.float32(SO), stream()), "filter")
+    torch.cuda.synchronize()
+    assert np.array_equal(din.cpu().numpy(), P)
+
+
+@pytest.mark.parametrize("layout", [_lib.ARECON_FP_ROWS, _lib.ARECON_FP_QUADS])
+def test_filter_2048_point_schedules(gpu, layout, monkeypatch):
+    """Both FFT schedules of the 2048-point transform (radix 16/16/8 on two
+    wavefronts, radix 8/8/8/4 on four) match the FP64 oracle and each other."""
+    P = np.random.default_rng(11).normal(size=(5, 6, 888)).astype(np.float32)
+    args = (443.5, 2.5, 0.9493, 1.9231)
+    want = oracle.filter_f64(P.astype(np.float64), *[float(np.float32(v)) for v in args], SO)
+    if layout == _lib.ARECON_FP_QUADS:
+        want = quads(want)
+    got = {}
+    for sched in ("0", "1"):
+        monkeypatch.setenv("ARECON_FILTER_SCHEDULE", sched)
+        got[sched] = run_filter(P, *args, layout=layout)
+        assert rel_l2(got[sched], want) < 1e-5
+    assert rel_l2(got["0"], got["1"]) < 1e-6
 
 
 def test_filter_in_place_and_16_bit_outputs(gpu):
