@@ -518,8 +518,9 @@ __device__ __forceinline__ void zc_group(cptr<float> e, float K, float zlc,
 }
 
 // In-place 3D-weighted accumulation of one slice's sample t:
-//   acc += t / (1 + a B)   when bit J of the wavefront-uniform mask `wm` is set,
-//   acc += t               otherwise (no conjugate view: weight exactly 1).
+//   acc = fma(rcp(1 + a B), t, acc)   when bit J of the wavefront-uniform mask
+//                                     `wm` is set,
+//   acc += t                          otherwise (no conjugate view: weight 1).
 // One asm block with the uniform branch inside, and the accumulator tied to the
 // destination: written as C++, the two paths give the accumulator two new
 // registers and the compiler copies every accumulator back at every view.  The
@@ -534,11 +535,13 @@ __device__ __forceinline__ void acc_weighted(float& acc, float t, unsigned wm, f
         "v_fma_f32 %[w], %[a], %[B], 1.0\n\t"
         "v_rcp_f32 %[w], %[w]\n\t"
         "s_nop 0\n\t"
-        "v_mul_f32 %[t], %[w], %[t]\n"
+        "v_fmac_f32 %[acc], %[w], %[t]\n\t"
+        "s_branch 2f\n"
         "1:\n\t"
-        "v_add_f32 %[acc], %[acc], %[t]"
-        : [acc] "+v"(acc), [t] "+v"(t), [w] "=&v"(w)
-        : [wm] "s"(wm), [j] "i"(J), [a] "s"(a), [B] "v"(B)
+        "v_add_f32 %[acc], %[acc], %[t]\n"
+        "2:"
+        : [acc] "+v"(acc), [w] "=&v"(w)
+        : [t] "v"(t), [wm] "s"(wm), [j] "i"(J), [a] "s"(a), [B] "v"(B)
         : "scc");
 }
 
@@ -552,11 +555,100 @@ __device__ __forceinline__ void acc_weighted_dyn(float& acc, float t, unsigned w
         "v_fma_f32 %[w], %[a], %[B], 1.0\n\t"
         "v_rcp_f32 %[w], %[w]\n\t"
         "s_nop 0\n\t"
-        "v_mul_f32 %[t], %[w], %[t]\n"
+        "v_fmac_f32 %[acc], %[w], %[t]\n\t"
+        "s_branch 2f\n"
         "1:\n\t"
-        "v_add_f32 %[acc], %[acc], %[t]"
-        : [acc] "+v"(acc), [t] "+v"(t), [w] "=&v"(w)
-        : [wt] "s"(weighted), [a] "s"(a), [B] "v"(B)
+        "v_add_f32 %[acc], %[acc], %[t]\n"
+        "2:"
+        : [acc] "+v"(acc), [w] "=&v"(w)
+        : [t] "v"(t), [wt] "s"(weighted), [a] "s"(a), [B] "v"(B)
+        : "scc");
+}
+
+// 3D-weighted accumulation of a group of four slices (samples t0..t3) with one
+// uniform branch per group instead of one per slice: `wt` holds the group's
+// weighted bits, `act` its active bits.  No weighted slice: four adds.  Every
+// active slice weighted: acc_i = fma(rcp(1 + a_i B), t_i, acc_i) for all four (an
+// inactive slice has t_i = 0 and a_i = 0, so it adds exactly zero).  Mixed:
+// slice by slice as acc_weighted.  The transcendental results are read three or
+// more instructions after they are written, so no wait states are needed there.
+__device__ __forceinline__ void acc_weighted_4(float& acc0, float& acc1, float& acc2,
+                                               float& acc3, float t0, float t1, float t2,
+                                               float t3, unsigned wt, unsigned act, float a0,
+                                               float a1, float a2, float a3, float B) {
+    float w0, w1, w2, w3;
+    asm volatile(
+        "s_cmp_eq_u32 %[wt], 0\n\t"
+        "s_cbranch_scc1 13f\n\t"
+        "s_cmp_eq_u32 %[wt], %[act]\n\t"
+        "s_cbranch_scc1 12f\n\t"
+        // mixed
+        "s_bitcmp1_b32 %[wt], 0\n\t"
+        "s_cbranch_scc0 20f\n\t"
+        "v_fma_f32 %[w0], %[a0], %[B], 1.0\n\t"
+        "v_rcp_f32 %[w0], %[w0]\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32 %[c0], %[w0], %[t0]\n\t"
+        "s_branch 21f\n"
+        "20:\n\t"
+        "v_add_f32 %[c0], %[c0], %[t0]\n"
+        "21:\n\t"
+        "s_bitcmp1_b32 %[wt], 1\n\t"
+        "s_cbranch_scc0 22f\n\t"
+        "v_fma_f32 %[w1], %[a1], %[B], 1.0\n\t"
+        "v_rcp_f32 %[w1], %[w1]\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32 %[c1], %[w1], %[t1]\n\t"
+        "s_branch 23f\n"
+        "22:\n\t"
+        "v_add_f32 %[c1], %[c1], %[t1]\n"
+        "23:\n\t"
+        "s_bitcmp1_b32 %[wt], 2\n\t"
+        "s_cbranch_scc0 24f\n\t"
+        "v_fma_f32 %[w2], %[a2], %[B], 1.0\n\t"
+        "v_rcp_f32 %[w2], %[w2]\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32 %[c2], %[w2], %[t2]\n\t"
+        "s_branch 25f\n"
+        "24:\n\t"
+        "v_add_f32 %[c2], %[c2], %[t2]\n"
+        "25:\n\t"
+        "s_bitcmp1_b32 %[wt], 3\n\t"
+        "s_cbranch_scc0 26f\n\t"
+        "v_fma_f32 %[w3], %[a3], %[B], 1.0\n\t"
+        "v_rcp_f32 %[w3], %[w3]\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32 %[c3], %[w3], %[t3]\n\t"
+        "s_branch 14f\n"
+        "26:\n\t"
+        "v_add_f32 %[c3], %[c3], %[t3]\n\t"
+        "s_branch 14f\n"
+        // every active slice weighted
+        "12:\n\t"
+        "v_fma_f32 %[w0], %[a0], %[B], 1.0\n\t"
+        "v_fma_f32 %[w1], %[a1], %[B], 1.0\n\t"
+        "v_fma_f32 %[w2], %[a2], %[B], 1.0\n\t"
+        "v_fma_f32 %[w3], %[a3], %[B], 1.0\n\t"
+        "v_rcp_f32 %[w0], %[w0]\n\t"
+        "v_rcp_f32 %[w1], %[w1]\n\t"
+        "v_rcp_f32 %[w2], %[w2]\n\t"
+        "v_rcp_f32 %[w3], %[w3]\n\t"
+        "v_fmac_f32 %[c0], %[w0], %[t0]\n\t"
+        "v_fmac_f32 %[c1], %[w1], %[t1]\n\t"
+        "v_fmac_f32 %[c2], %[w2], %[t2]\n\t"
+        "v_fmac_f32 %[c3], %[w3], %[t3]\n\t"
+        "s_branch 14f\n"
+        // no weighted slice
+        "13:\n\t"
+        "v_add_f32 %[c0], %[c0], %[t0]\n\t"
+        "v_add_f32 %[c1], %[c1], %[t1]\n\t"
+        "v_add_f32 %[c2], %[c2], %[t2]\n\t"
+        "v_add_f32 %[c3], %[c3], %[t3]\n"
+        "14:"
+        : [c0] "+v"(acc0), [c1] "+v"(acc1), [c2] "+v"(acc2), [c3] "+v"(acc3),
+          [w0] "=&v"(w0), [w1] "=&v"(w1), [w2] "=&v"(w2), [w3] "=&v"(w3)
+        : [t0] "v"(t0), [t1] "v"(t1), [t2] "v"(t2), [t3] "v"(t3), [wt] "s"(wt), [act] "s"(act),
+          [a0] "s"(a0), [a1] "s"(a1), [a2] "s"(a2), [a3] "s"(a3), [B] "v"(B)
         : "scc");
 }
 
@@ -868,7 +960,7 @@ __device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsig
     const int rA = max((int)floorf(vlo) - 1, -1);
     const int rB = min((int)floorf(vhi) + 1, zlm1);
     if (rB <= -1 || rA >= zlm1) { v.mode = ZS_SKIP; return v; }   // only zero rows
-    // rows [rA, rB] in 4-row loads, then one zero row after them
+    // rows [rA, rB] in 4-row loads, fewer than RMAX: the slot's last row stays zero
     if (4 * ((rB - rA + 4) / 4) >= RMAX) return v;
     v.mode = ZS_LDS; v.rA = rA; v.rB = rB; v.c0 = c0;
     return v;
@@ -892,7 +984,7 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     static_assert(Gather<T, LAYOUT>::kElem == 8, "LDS staging takes 8-byte records");
     static_assert(ZB <= 16 && RMAX % 4 == 0, "bad block shape");
     constexpr int kSlot = RMAX * kZsRowBytes;
-    __shared__ __attribute__((aligned(16))) char stage[4 * kSlot + 16];
+    __shared__ __attribute__((aligned(16))) char stage0[4 * kSlot];
     __shared__ __attribute__((aligned(16))) unsigned fpt[kZsMaxViews];
     typedef Gather<T, LAYOUT> Gt;
     typedef LdsGather<T, LAYOUT> Lg;
@@ -916,7 +1008,9 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     const float ex = 7.5f * fabsf(A.dx), ey = 7.5f * fabsf(A.dy);
 
     // Pass 1: every view's footprint into the table.  A workgroup with a view
-    // that does not fit runs the z-column kernel's body instead.
+    // that does not fit runs the z-column kernel's body instead.  (Computing the
+    // footprints where the views are staged instead frees the table's LDS for a
+    // sixth workgroup per CU, but costs about two VALU per update: slower.)
     int fits = pmax - pmin + 1 <= kZsMaxViews;
     if (fits)
         for (int p = pmin + (int)threadIdx.x; p <= pmax; p += 256) {
@@ -925,9 +1019,16 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             if (v.mode == ZS_DIRECT) fits = 0;
             fpt[p - pmin] = zs_pack(v);
         }
-    if (threadIdx.x < 4)
+    // the last row of every slot is zero and stays zero (a view fits only when
+    // its loaded rows, a multiple of four, are fewer than RMAX): rows past the
+    // view's last staged row (slices outside the view's window, temp2 = 1e30,
+    // and rows off the detector's end) clamp onto it and add exactly zero, and
+    // lanes off the detector read it
+    for (int i = threadIdx.x; i < 4 * kZsRowBytes / 4; i += 256) {
+        const int slot = i / (kZsRowBytes / 4), w = i - slot * (kZsRowBytes / 4);
         reinterpret_cast<__attribute__((address_space(3))) unsigned*>(
-            (lds_char*)stage)[(4 * kSlot) / 4 + threadIdx.x] = 0u;      // zero record
+            (lds_char*)stage0 + slot * kSlot + (RMAX - 1) * kZsRowBytes)[w] = 0u;
+    }
     const int staged = __syncthreads_and(fits);
     if (threadIdx.x == 0 && A.stats) atomicAdd(A.stats + (staged ? 0 : 1), 1ull);
     if (!staged) {
@@ -947,46 +1048,46 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     const int z0 = zs * ZB;
     constexpr unsigned kElem = Gt::kElem;
     const int rowb = (LAYOUT == ARECON_FP_QUADS ? A.YL - 1 : A.YL) * (int)kElem;
-    lds_char* const lstage = (lds_char*)stage;
+    lds_char* const lstage = (lds_char*)stage0;
     float zlc, zoff;
     asm("v_mov_b32 %0, %1" : "=v"(zlc) : "s"(A.ZLC));
-    // a lane off the detector gets row -1e9, clamped to rA, and reads the zero
-    // record after the slots
+    // a lane off the detector gets row -1e9, clamped to rA, and reads the slot's
+    // zero row
     asm("v_mov_b32 %0, %1" : "=v"(zoff) : "s"(-1e9f));
     float acc[ZB];
 #pragma unroll
     for (int j = 0; j < ZB; ++j) acc[j] = 0.f;
 
-    for (int p0 = pmin; p0 <= pmax; p0 += 4) {
-        // stage: wave w copies the footprint of view p0 + w into slot w
-        {
-            const int p = p0 + wave;
-            if (p <= pmax) {
-                const unsigned e = __builtin_amdgcn_readfirstlane(fpt[p - pmin]);
-                const int rA = (int)((e >> 16) & 0xffu) - 1, rB = (int)(e >> 24) - 1;
-                const int c0 = (int)(short)(e & 0xffffu);
-                if (rB >= rA) {
-                    const __amdgpu_buffer_rsrc_t rs =
-                        view_rsrc(fp + (long long)p * view_stride, view_bytes);
-                    // lane: row rA + i + lane / 16, records c0 + 2 (lane % 16) + {0, 1};
-                    // rows -1 and ZL-1 and records outside the row are never read
-                    // by an update or lie outside the view (and load as zeros)
-                    const int off = (rA + (lane >> 4)) * rowb + (c0 + 2 * (lane & 15)) * (int)kElem;
-                    lds_char* dst = lstage + wave * kSlot;
-                    for (int i = 0; i <= rB - rA; i += 4)
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                            rs, (lds_void*)(dst + i * kZsRowBytes), 16,
-                            (unsigned)(off + i * rowb), 0, 0, 0);
-                    // the zero row after the loaded ones: slices outside the view's
-                    // window (temp2 = 1e30) clamp onto it and add exactly zero
-                    reinterpret_cast<__attribute__((address_space(3))) unsigned*>(
-                        dst + 4 * ((rB - rA + 4) / 4) * kZsRowBytes)[lane] = 0u;
-                }
+    // stage: wave w copies the footprint of view p0 + w into slot w
+    auto stage_chunk = [&](int p0) {
+        const int p = p0 + wave;
+        if (p <= pmax) {
+            const unsigned e = __builtin_amdgcn_readfirstlane(fpt[p - pmin]);
+            const int rA = (int)((e >> 16) & 0xffu) - 1, rB = (int)(e >> 24) - 1;
+            const int c0 = (int)(short)(e & 0xffffu);
+            if (rB >= rA) {
+                const __amdgpu_buffer_rsrc_t rs =
+                    view_rsrc(fp + (long long)p * view_stride, view_bytes);
+                // nl rows [rB + 1 - nl, rB] (nl = rB - rA + 1 rounded up to a
+                // multiple of four) go to slot rows [RMAX - 1 - nl, RMAX - 2], so row
+                // rB + 1 lands on the slot's zero last row.  Lane: row
+                // rB + 1 - nl + i + lane / 16, records c0 + 2 (lane % 16) + {0, 1};
+                // rows before the view's first or after its last record (-1, ZL-1
+                // and beyond) load as zeros
+                const int nl = 4 * ((rB - rA + 4) / 4);
+                const int off = (rB + 1 - nl + (lane >> 4)) * rowb +
+                                (c0 + 2 * (lane & 15)) * (int)kElem;
+                lds_char* dst = lstage + wave * kSlot + (RMAX - 1 - nl) * kZsRowBytes;
+                for (int i = 0; i < nl; i += 4)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        rs, (lds_void*)(dst + i * kZsRowBytes), 16,
+                        (unsigned)(off + i * rowb), 0, 0, 0);
             }
         }
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        // consume the chunk's views; their four table entries in one LDS read
+    };
+    // consume the chunk of views p0 .. p0+3; their four table entries in one LDS
+    // read
+    auto consume_chunk = [&](int p0) {
         const uint4 e4 = *reinterpret_cast<const uint4*>(fpt + (p0 - pmin));
         const unsigned ech[4] = {(unsigned)__builtin_amdgcn_readfirstlane(e4.x),
                                  (unsigned)__builtin_amdgcn_readfirstlane(e4.y),
@@ -1009,16 +1110,27 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             float2 ent[ZB];
 #pragma unroll
             for (int j = 0; j < ZB; ++j) ent[j] = cld_f2(erow, j);
-            // byte offset in `stage` = row * kZsRowBytes + colterm, row clamped to
-            // [rA, rB]; a lane off the detector gets row -1e9 (clamped to rA) and
-            // the zero record after the slots
+            // byte offset in the slot = row * kZsRowBytes + colterm, row clamped to
+            // [rA, rBv]; a lane off the detector gets row -1e9 (clamped to rA) and
+            // the slot's zero row
             const float zl = uok ? zlc : zoff;
-            const int cterm = s * kSlot - (c0 + 1) * 8 - rA * kZsRowBytes;
-            int zterm;
-            asm("v_mov_b32 %0, %1" : "=v"(zterm) : "s"(4 * kSlot - rA * kZsRowBytes));
-            int rBv;   // clamp bound: the zero row after the loaded rows
-            asm("v_mov_b32 %0, %1" : "=v"(rBv) : "s"(rA + 4 * ((rB - rA + 4) / 4)));
-            const int colterm = uok ? ct.ub * 8 + 8 + cterm : zterm;
+            // slot s's offset s * kSlot goes in the loads' immediate offset; the
+            // rest of the address is one per-view VGPR, kept whole (opaque to the
+            // compiler, which otherwise splits its scalar part off and adds it
+            // back at every update), so an update's address is one v_lshl_add_u32
+            static_assert(kZsRowBytes == 256, "row stride is a shift by 8");
+            const int cterm = -c0 * 8 + (RMAX - 2 - rB) * kZsRowBytes;   // row rB at RMAX - 2
+            int colterm;
+            if constexpr (INSIDE) {
+                asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(colterm) : "v"(ct.ub), "s"(cterm));
+            } else {
+                // the slot's zero row (row rBv): r clamps to rA off the detector
+                const int zterm = (RMAX - 1 - rA) * kZsRowBytes;
+                const int c = uok ? ct.ub * 8 + cterm : zterm;
+                asm("v_mov_b32 %0, %1" : "=v"(colterm) : "v"(c));
+            }
+            int rBv;   // clamp bound: row rB + 1, the slot's last row, kept zero
+            asm("v_mov_b32 %0, %1" : "=v"(rBv) : "s"(rB + 1));
             // groups of four slices, each skipped when none of its slices has the
             // view in its window, and branch-free inside: a slice outside the
             // window has temp2 = 1e30, so its row clamps onto the zero row (and a
@@ -1036,14 +1148,26 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
                     fr[i] = __builtin_amdgcn_fractf(V1);
                     int r;   // one VGPR bound: a VOP3 reads at most one SGPR here
                     asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(vf), "s"(rA), "v"(rBv));
-                    q[i] = Lg::load(lstage + (r * kZsRowBytes + colterm));
+                    int a;
+                    asm("v_lshl_add_u32 %0, %1, 8, %2" : "=v"(a) : "v"(r), "v"(colterm));
+                    q[i] = Lg::load(lstage + s * kSlot + a);
                 }
+                // all four reads in flight before the first blend waits on one
+                __builtin_amdgcn_sched_barrier(0);
+                float t[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    acc_weighted_dyn(acc[g + i], Gt::lerp(q[i], fr[i], Dey, wu),
-                                     (m.y >> (g + i)) & 1u, ent[g + i].y, B);
+                for (int i = 0; i < 4; ++i) t[i] = Gt::lerp(q[i], fr[i], Dey, wu);
+                acc_weighted_4(acc[g], acc[g + 1], acc[g + 2], acc[g + 3], t[0], t[1], t[2], t[3],
+                               (m.y >> g) & 0xfu, (m.x >> g) & 0xfu, ent[g].y, ent[g + 1].y,
+                               ent[g + 2].y, ent[g + 3].y, B);
             }
         }
+    };
+    for (int p0 = pmin; p0 <= pmax; p0 += 4) {
+        stage_chunk(p0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        consume_chunk(p0);
         __syncthreads();
     }
     if (xi < A.XN && yi < A.YN) {

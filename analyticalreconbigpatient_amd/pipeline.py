@@ -56,16 +56,18 @@ def fdk_weights(N_2pi: int, HSCoef: float, delta: int) -> np.ndarray:
 def choose_bp_kernel(params: BPParams, dz: float | None = None, storage: str = "f32") -> dict:
     """Backprojector launch shape for a geometry.  Consecutive slices' windows
     are N_2pi dz / h views apart; when a view lies in the windows of at least
-    four consecutive slices, the z-column kernel over blocks of 8 slices shares
-    the per-(column, view) work among them, with one gather per voxel-update:
-    row pairs for FP32 storage, difference quads (same bytes, three FMAs per
-    blend) for 16-bit storage.  Otherwise (thick slices, few shared views) the
+    four consecutive slices, the z-column kernels over blocks of 8 slices share
+    the per-(column, view) work among them: row pairs for FP32 storage,
+    difference quads (same bytes, three FMAs per blend) for 16-bit storage.  The
+    LDS-staged one (variant 3) copies each view's footprint to LDS and reads its
+    samples there; a workgroup whose footprints do not fit runs the gathering
+    z-column body (variant 2).  Otherwise (thick slices, few shared views) the
     view-table kernel runs on the rows layout."""
     dzv = params.dz if dz is None else dz
     L = 2 * window_half_width(params.N_2pi, params.HSCoef) + 1
     shift = abs(params.N_2pi * dzv / params.h)
     if shift * 4 <= L:
-        return dict(variant=2, nz=8, group=8, layout="rowpairs" if storage == "f32" else "quads")
+        return dict(variant=3, nz=8, group=8, layout="rowpairs" if storage == "f32" else "quads")
     return dict(variant=0, nz=0, group=0, layout="rows")
 
 
@@ -222,6 +224,8 @@ class HelicalFDK:
             k = choose_bp_kernel(self.params, dz, storage)
             variant, nz, group = k["variant"], nz or k["nz"], group or k["group"]
             layout = layout or k["layout"]
+        elif variant == 3 and layout is None:   # the LDS-staged kernel's only layouts
+            layout = "rowpairs" if storage == "f32" else "quads"
         layout = layout or "rows"
         self.plan = BPPlan(self.params, dz=dz, nz=nz, order=order, variant=variant, group=group)
         self.variant, self.group = variant, group

@@ -44,7 +44,7 @@ def small_workload():
 
 @pytest.mark.parametrize("storage,tol", [("f32", 1e-4), ("f16", 2e-3), ("bf16", 1e-2)])
 @pytest.mark.parametrize("variant,layout", [(0, "rows"), (2, "rows"), (2, "rowpairs"),
-                                            (2, "quads")])
+                                            (2, "quads"), (3, None), (None, None)])
 def test_pipeline_matches_oracle_chain(gpu, nthreads, storage, tol, variant, layout):
     print(f"pipeline storage={storage} variant={variant} layout={layout}", flush=True)
     wl, s, dz, E = small_workload()
@@ -59,7 +59,8 @@ def test_pipeline_matches_oracle_chain(gpu, nthreads, storage, tol, variant, lay
     assert err < tol
 
 
-@pytest.mark.parametrize("variant,layout", [(0, "rows"), (2, "rowpairs"), (2, "quads")])
+@pytest.mark.parametrize("variant,layout", [(0, "rows"), (2, "rowpairs"), (2, "quads"),
+                                            (3, None)])
 def test_pipeline_trimmed_segment(gpu, nthreads, variant, layout):
     """Backprojecting a trimmed view range (segment 2 of 3) by pointer offset."""
     wl, s, dz, E = small_workload()
