@@ -15,7 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libarecon.so"
 ARECON_OK = 0
 ARECON_F32, ARECON_F16, ARECON_BF16 = 0, 1, 2
 ARECON_LAYOUT_VYZ, ARECON_LAYOUT_VZY = 0, 1
-ARECON_FP_ROWS, ARECON_FP_ROWPAIRS, ARECON_FP_QUADS = 0, 1, 2
+ARECON_FP_ROWS, ARECON_FP_ROWPAIRS, ARECON_FP_QUADS, ARECON_FP_ROWDIFF = 0, 1, 2, 3
 
 _f = C.c_float
 _d = C.c_double

@@ -459,6 +459,23 @@ template <class T> struct Gather<T, ARECON_FP_ROWPAIRS> {
     }
 };
 
+// Row differences: (f, f(r+1) - f) for channels U-1 and U.  The differences are
+// the ones the row-pair blend forms, in the same FP32 rounding, so the sample is
+// bit-identical to it in four instructions instead of six.
+template <class T> struct Gather<T, ARECON_FP_ROWDIFF> {
+    static constexpr unsigned kElem = 2 * sizeof(T);
+    typedef typename Quad<T>::Raw Raw;
+    __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t rs, unsigned ob, unsigned) {
+        return Quad<T>::load(rs, ob);
+    }
+    __device__ static __forceinline__ float lerp(const Raw& raw, float fr, float Dey, float) {
+        const float4 q = Quad<T>::get(raw);
+        const float c0 = fmaf(fr, q.y, q.x);              // channel U-1, rows blended
+        const float c1 = fmaf(fr, q.w, q.z);              // channel U
+        return fmaf(Dey, c0 - c1, c1);
+    }
+};
+
 // Difference quads: (a, dr, dc, drc) with the sample at (row V-1 + fr, channel
 // U-1 + wu) equal to a + fr dr + wu (dc + fr drc), wu = 1 - Dey.
 template <class T> struct Gather<T, ARECON_FP_QUADS> {
@@ -542,26 +559,6 @@ __device__ __forceinline__ void acc_weighted(float& acc, float t, unsigned wm, f
         "2:"
         : [acc] "+v"(acc), [w] "=&v"(w)
         : [t] "v"(t), [wm] "s"(wm), [j] "i"(J), [a] "s"(a), [B] "v"(B)
-        : "scc");
-}
-
-// acc_weighted with the wavefront-uniform weighted flag as a value (0 or 1).
-__device__ __forceinline__ void acc_weighted_dyn(float& acc, float t, unsigned weighted, float a,
-                                                 float B) {
-    float w;
-    asm volatile(
-        "s_cmp_eq_u32 %[wt], 0\n\t"
-        "s_cbranch_scc1 1f\n\t"
-        "v_fma_f32 %[w], %[a], %[B], 1.0\n\t"
-        "v_rcp_f32 %[w], %[w]\n\t"
-        "s_nop 0\n\t"
-        "v_fmac_f32 %[acc], %[w], %[t]\n\t"
-        "s_branch 2f\n"
-        "1:\n\t"
-        "v_add_f32 %[acc], %[acc], %[t]\n"
-        "2:"
-        : [acc] "+v"(acc), [w] "=&v"(w)
-        : [t] "v"(t), [wt] "s"(weighted), [a] "s"(a), [B] "v"(B)
         : "scc");
 }
 
@@ -916,6 +913,7 @@ template <> struct LdsGather<float, ARECON_FP_ROWPAIRS> {   // records c, c+1 (1
         return Raw{lo.x, lo.y, hi.x, hi.y};
     }
 };
+template <> struct LdsGather<float, ARECON_FP_ROWDIFF> : LdsGather<float, ARECON_FP_ROWPAIRS> {};
 template <class T> struct LdsGather<T, ARECON_FP_QUADS> {         // one quad (8 B)
     typedef typename Gather<T, ARECON_FP_QUADS>::Raw Raw;
     __device__ static __forceinline__ Raw load(const lds_char* a) {
@@ -1654,12 +1652,15 @@ void bp_plan_destroy(BPPlan* P) {
 // layout and whether the volume takes float4 stores.
 static BPArgs launch_args(const BPPlan* P, const float* img, int layout, size_t elem) {
     ARECON_REQUIRE(layout == ARECON_FP_ROWS || P->variant >= 2,
-                   "bp: the row-pair and quad layouts need a z-column kernel (variant 2 or 3)");
+                   "bp: the row-pair, row-difference and quad layouts need a z-column kernel "
+                   "(variant 2 or 3)");
     ARECON_REQUIRE(P->variant != 3 || layout != ARECON_FP_ROWS,
-                   "bp: the LDS-staged z-column kernel (variant 3) reads row pairs or quads");
+                   "bp: the LDS-staged z-column kernel (variant 3) reads row pairs, row "
+                   "differences or quads");
     BPArgs a = P->a;
     a.vec4 = (a.ZN % 4 == 0) && ((reinterpret_cast<uintptr_t>(img) & 15) == 0);
-    if (layout == ARECON_FP_ROWPAIRS) a.view_stride = 2LL * (a.ZL - 1) * a.YL;
+    if (layout == ARECON_FP_ROWPAIRS || layout == ARECON_FP_ROWDIFF)
+        a.view_stride = 2LL * (a.ZL - 1) * a.YL;
     if (layout == ARECON_FP_QUADS) a.view_stride = 4LL * (a.ZL - 1) * (a.YL - 1);
     a.view_bytes = (unsigned)(a.view_stride * (long long)elem);
     return a;
@@ -1686,14 +1687,25 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
         if (P->variant == 3) {
             // rows staged per view: the row span of NZ slices plus margins
             constexpr int RMAX = NZ == 8 ? 24 : 40;
-            constexpr int LY = std::is_same<T, float>::value ? ARECON_FP_ROWPAIRS : ARECON_FP_QUADS;
-            ARECON_REQUIRE(layout == LY, "bp: variant 3 reads FP32 row pairs or 16-bit quads");
-            if (a.inside)
-                hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, true>), dim3(grid), dim3(256),
-                                   0, s, img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a);
-            else
-                hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, false>), dim3(grid), dim3(256),
-                                   0, s, img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a);
+#define ARECON_ZS_(LY)                                                                        \
+    do {                                                                                      \
+        if (a.inside)                                                                         \
+            hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, true>), dim3(grid), dim3(256), \
+                               0, s, img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a); \
+        else                                                                                  \
+            hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, false>), dim3(grid), dim3(256), \
+                               0, s, img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a); \
+    } while (0)
+            if constexpr (std::is_same<T, float>::value) {
+                ARECON_REQUIRE(layout == ARECON_FP_ROWPAIRS || layout == ARECON_FP_ROWDIFF,
+                               "bp: variant 3 reads FP32 row pairs or row differences");
+                if (layout == ARECON_FP_ROWDIFF) ARECON_ZS_(ARECON_FP_ROWDIFF);
+                else ARECON_ZS_(ARECON_FP_ROWPAIRS);
+            } else {
+                ARECON_REQUIRE(layout == ARECON_FP_QUADS, "bp: variant 3 reads 16-bit quads");
+                ARECON_ZS_(ARECON_FP_QUADS);
+            }
+#undef ARECON_ZS_
             return;
         }
     }
@@ -1722,6 +1734,13 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
                 else if (P->group == 2) ARECON_ZC(2, ARECON_FP_ROWPAIRS);
                 else if (P->group == 4) ARECON_ZC(4, ARECON_FP_ROWPAIRS);
                 else if constexpr (NZ <= 16) ARECON_ZC(0, ARECON_FP_ROWPAIRS);
+            } else if (layout == ARECON_FP_ROWDIFF) {
+                if constexpr (std::is_same<T, float>::value) {   // FP32 only
+                    if (P->group == 1) ARECON_ZC(1, ARECON_FP_ROWDIFF);
+                    else if (P->group == 2) ARECON_ZC(2, ARECON_FP_ROWDIFF);
+                    else if (P->group == 4) ARECON_ZC(4, ARECON_FP_ROWDIFF);
+                    else if constexpr (NZ <= 16) ARECON_ZC(0, ARECON_FP_ROWDIFF);
+                }
             } else {
                 if (P->group == 1) ARECON_ZC(1, ARECON_FP_QUADS);
                 else if (P->group == 2) ARECON_ZC(2, ARECON_FP_QUADS);
@@ -1754,8 +1773,10 @@ void bp_plan_launch(const BPPlan* P, float* img, const void* fp, int fp_dtype, i
     ARECON_REQUIRE(P != nullptr, "bp: null plan");
     ARECON_REQUIRE(img != nullptr && fp != nullptr, "bp: null buffer");
     ARECON_REQUIRE(fp_layout == ARECON_FP_ROWS || fp_layout == ARECON_FP_ROWPAIRS ||
-                       fp_layout == ARECON_FP_QUADS,
+                       fp_layout == ARECON_FP_QUADS || fp_layout == ARECON_FP_ROWDIFF,
                    "bp: unknown fp layout");
+    ARECON_REQUIRE(fp_layout != ARECON_FP_ROWDIFF || fp_dtype == ARECON_F32,
+                   "bp: the row-difference layout is FP32 only");
     if (block_end < 0) block_end = P->a.nzsets;
     ARECON_REQUIRE(block_begin >= 0 && block_begin < block_end && block_end <= P->a.nzsets,
                    "bp: slice-block range outside the plan");
@@ -1796,8 +1817,10 @@ BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const v
     ARECON_REQUIRE(fp_dtype == ARECON_F32 || fp_dtype == ARECON_F16 || fp_dtype == ARECON_BF16,
                    "bp batch: unknown fp dtype");
     ARECON_REQUIRE(fp_layout == ARECON_FP_ROWS || fp_layout == ARECON_FP_ROWPAIRS ||
-                       fp_layout == ARECON_FP_QUADS,
+                       fp_layout == ARECON_FP_QUADS || fp_layout == ARECON_FP_ROWDIFF,
                    "bp batch: unknown fp layout");
+    ARECON_REQUIRE(fp_layout != ARECON_FP_ROWDIFF || fp_dtype == ARECON_F32,
+                   "bp batch: the row-difference layout is FP32 only");
     std::vector<SlabDesc> slabs(n);
     long long wg = 0;
     for (int i = 0; i < n; ++i) {
@@ -1865,6 +1888,11 @@ static void batch_launch_t(const BPBatch* B, hipStream_t s) {
     case ARECON_FP_ROWPAIRS:
         hipLaunchKernelGGL((bp_fdk3dw_zc_batch<NZ, G, T, ARECON_FP_ROWPAIRS>), grid, block, 0,
                            s, B->d_slabs, B->nslabs);
+        break;
+    case ARECON_FP_ROWDIFF:
+        if constexpr (std::is_same<T, float>::value)
+            hipLaunchKernelGGL((bp_fdk3dw_zc_batch<NZ, G, T, ARECON_FP_ROWDIFF>), grid, block, 0,
+                               s, B->d_slabs, B->nslabs);
         break;
     default:
         hipLaunchKernelGGL((bp_fdk3dw_zc_batch<NZ, G, T, ARECON_FP_QUADS>), grid, block, 0, s,

@@ -613,13 +613,14 @@ __global__ __launch_bounds__(256) void expand_quads(OutT* __restrict__ out,
     }
 }
 
-// Filter and difference quads in one pass (N = 256 .. 2048).  A workgroup
-// takes whole views and walks their rows in pairs (k, k+1): the inverse
-// transform leaves both filtered rows in LDS, where they give quad rows k-1
-// (with row k-1, kept from the previous pair) and k.  The quads are the same
-// FP32 differences as expand_quads forms from stored rows, rounded once to
+// Filter and difference quads (OUTL = ARECON_FP_QUADS) or row differences
+// (ARECON_FP_ROWDIFF, FP32) in one pass (N = 256 .. 2048).  A workgroup takes
+// whole views and walks their rows in pairs (k, k+1): the inverse transform
+// leaves both filtered rows in LDS, where they give output rows k-1 (with row
+// k-1, kept from the previous pair) and k.  The differences are the same FP32
+// ones that expand_quads / expand_rowdiff form from stored rows, rounded once to
 // OutT, so the output is identical; the FP32 rows never reach HBM.
-template <int N, class OutT, int S>
+template <int N, class OutT, int S, int OUTL>
 __global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
     OutT* __restrict__ out, const float* in, const float2* __restrict__ twg,
     const float* __restrict__ Hg, const float* __restrict__ Wg, int YL, int ZL, int ViewN,
@@ -632,7 +633,7 @@ __global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
     const int nc = YL - 1;
     RowOut<OutT> o{};                            // the rows stay in LDS
     for (int v = blockIdx.x; v < ViewN; v += gridDim.x) {
-        OutT* ov = out + (long long)v * (ZL - 1) * nc * 4;
+        OutT* ov = out + (long long)v * (ZL - 1) * (OUTL == ARECON_FP_QUADS ? nc * 4 : YL * 2);
         for (int k = 0; k < ZL; k += 2) {
             RowCtx c;
             c.has_b = k + 1 < ZL;
@@ -652,6 +653,15 @@ __global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
                 const float a0 = la[ch];
                 const float b0 = c.has_b ? lb[ch] : 0.f;
                 if (c.has_b) pn[ch] = b0;
+                if constexpr (OUTL == ARECON_FP_ROWDIFF) {
+                    if (k > 0)
+                        *reinterpret_cast<float2*>(ov + ((long long)(k - 1) * YL + ch) * 2) =
+                            make_float2(pr[ch], a0 - pr[ch]);
+                    if (c.has_b)
+                        *reinterpret_cast<float2*>(ov + ((long long)k * YL + ch) * 2) =
+                            make_float2(a0, b0 - a0);
+                    continue;
+                }
                 if (ch >= nc) continue;
                 const float a1 = la[ch + 1];
                 if (k > 0) {
@@ -681,67 +691,107 @@ static int filter_schedule() {
     return (e && e[0] == '0') ? 0 : 1;
 }
 
+// Row differences from filtered FP32 rows: element (v, r, c) of
+// [ViewN][ZL-1][YL][2] holds f(r,c) and f(r+1,c)-f(r,c).
+__global__ __launch_bounds__(256) void expand_rowdiff(float2* __restrict__ out,
+                                                      const float* __restrict__ rows, int YL,
+                                                      int ZL, long long n) {
+    const int nr = ZL - 1;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long t = i / YL;
+        const int c = (int)(i - t * YL);
+        const long long v = t / nr;
+        const int r = (int)(t - v * nr);
+        const float* p = rows + (v * ZL + r) * YL + c;
+        const float a = p[0];
+        out[i] = make_float2(a, p[YL] - a);
+    }
+}
+
 static void launch_filter_rows(void* out, int out_dtype, int out_layout, const float* in,
                                int YL, int ZL, int ViewN, float PLC, float ZLC, float dYL,
                                float dZL, float SO, hipStream_t s);
 
+// Launch of the fused per-view filter for layout OUTL.
+template <int OUTL, class OT>
+static void launch_filter_fused(OT* outp, const float* in, const FilterTables& t, const float* W,
+                                int N, int YL, int ZL, int ViewN, float PLC, float dYL, float SO,
+                                hipStream_t s) {
+    const dim3 grid((unsigned)ViewN);
+    switch (N) {
+    case 256: hipLaunchKernelGGL((filter_quads_wave<256, OT, 0, OUTL>), grid, dim3(tpt<256>()), 0, s,
+                                 outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
+    case 512: hipLaunchKernelGGL((filter_quads_wave<512, OT, 0, OUTL>), grid, dim3(tpt<512>()), 0, s,
+                                 outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
+    case 1024: hipLaunchKernelGGL((filter_quads_wave<1024, OT, 0, OUTL>), grid, dim3(tpt<1024>()), 0, s,
+                                  outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
+    default:
+        if (filter_schedule() == 1)
+            hipLaunchKernelGGL((filter_quads_wave<2048, OT, 1, OUTL>), grid, dim3(tpt<2048, 1>()), 0, s,
+                               outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO);
+        else
+            hipLaunchKernelGGL((filter_quads_wave<2048, OT, 0, OUTL>), grid, dim3(tpt<2048, 0>()), 0, s,
+                               outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO);
+        break;
+    }
+}
+
 void launch_filter(void* out, int out_dtype, int out_layout, const float* in, int YL, int ZL,
                    int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
                    hipStream_t s) {
-    if (out_layout != ARECON_FP_QUADS) {
+    if (out_layout != ARECON_FP_QUADS && out_layout != ARECON_FP_ROWDIFF) {
         launch_filter_rows(out, out_dtype, out_layout, in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL,
                            SO, s);
         return;
     }
-    ARECON_REQUIRE(YL >= 2 && ZL >= 2, "filter: the quad layout needs YL >= 2 and ZL >= 2");
+    const bool quads = out_layout == ARECON_FP_QUADS;
+    ARECON_REQUIRE(YL >= 2 && ZL >= 2,
+                   "filter: the quad and row-difference layouts need YL >= 2 and ZL >= 2");
     ARECON_REQUIRE(out != nullptr && out != (const void*)in,
-                   "filter: the quad output cannot alias the input");
-    ARECON_REQUIRE(YL > 0 && ViewN > 0, "filter: YL, ZL and ViewN must be positive");
+                   "filter: the quad or row-difference output cannot alias the input");
+    ARECON_REQUIRE(ViewN > 0, "filter: YL, ZL and ViewN must be positive");
+    ARECON_REQUIRE(YL <= 2048, "filter: YL above 2048 is not supported");
     ARECON_REQUIRE(dYL != 0.f && SO > 0.f, "filter: dYL must be nonzero and SO positive");
     ARECON_REQUIRE(out_dtype == ARECON_F32 || out_dtype == ARECON_F16 || out_dtype == ARECON_BF16,
                    "filter: unknown output dtype");
+    ARECON_REQUIRE(quads || out_dtype == ARECON_F32, "filter: the row-difference layout is FP32 only");
     int dev = 0;
     ARECON_HIP(hipGetDevice(&dev));
     const int N = filter_length(YL);
     if (N >= 256 && N <= 2048) {      // fused: filtered rows stay in LDS
         const FilterTables& t = filter_tables(dev, YL, dYL);
         const float* W = preweight_table(dev, YL, ZL, PLC, ZLC, dYL, dZL, SO);
-        const unsigned grid = (unsigned)ViewN;
-        auto go = [&](auto outp) {
-            using OT = std::remove_pointer_t<decltype(outp)>;
-            switch (N) {
-            case 256: hipLaunchKernelGGL((filter_quads_wave<256, OT, 0>), dim3(grid), dim3(tpt<256>()), 0, s,
-                                         outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
-            case 512: hipLaunchKernelGGL((filter_quads_wave<512, OT, 0>), dim3(grid), dim3(tpt<512>()), 0, s,
-                                         outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
-            case 1024: hipLaunchKernelGGL((filter_quads_wave<1024, OT, 0>), dim3(grid), dim3(tpt<1024>()), 0, s,
-                                          outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
-            default:
-                if (filter_schedule() == 1)
-                    hipLaunchKernelGGL((filter_quads_wave<2048, OT, 1>), dim3(grid), dim3(tpt<2048, 1>()), 0, s,
-                                       outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO);
-                else
-                    hipLaunchKernelGGL((filter_quads_wave<2048, OT, 0>), dim3(grid), dim3(tpt<2048, 0>()), 0, s,
-                                       outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO);
-                break;
-            }
-        };
-        switch (out_dtype) {
-        case ARECON_F32: go(static_cast<float*>(out)); break;
-        case ARECON_F16: go(static_cast<__half*>(out)); break;
-        default: go(static_cast<__hip_bfloat16*>(out)); break;
-        }
+        if (!quads)
+            launch_filter_fused<ARECON_FP_ROWDIFF>(static_cast<float*>(out), in, t, W, N, YL, ZL,
+                                                   ViewN, PLC, dYL, SO, s);
+        else if (out_dtype == ARECON_F32)
+            launch_filter_fused<ARECON_FP_QUADS>(static_cast<float*>(out), in, t, W, N, YL, ZL,
+                                                 ViewN, PLC, dYL, SO, s);
+        else if (out_dtype == ARECON_F16)
+            launch_filter_fused<ARECON_FP_QUADS>(static_cast<__half*>(out), in, t, W, N, YL, ZL,
+                                                 ViewN, PLC, dYL, SO, s);
+        else
+            launch_filter_fused<ARECON_FP_QUADS>(static_cast<__hip_bfloat16*>(out), in, t, W, N,
+                                                 YL, ZL, ViewN, PLC, dYL, SO, s);
         ARECON_HIP(hipGetLastError());
         return;
     }
+    // other transform lengths: filter in place, then derive the layout
     float* rows = const_cast<float*>(in);
     launch_filter_rows(rows, ARECON_F32, ARECON_FP_ROWS, in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL,
                        SO, s);
-    const long long n = (long long)ViewN * (ZL - 1) * (YL - 1);
+    const long long n = (long long)ViewN * (ZL - 1) * (quads ? YL - 1 : YL);
     int cus = 0;
     ARECON_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const long long grid = std::max<long long>(1, std::min<long long>((n + 255) / 256,
                                                                       (long long)cus * 64));
+    if (!quads) {
+        hipLaunchKernelGGL(expand_rowdiff, dim3((unsigned)grid), dim3(256), 0, s,
+                           static_cast<float2*>(out), rows, YL, ZL, n);
+        ARECON_HIP(hipGetLastError());
+        return;
+    }
     switch (out_dtype) {
     case ARECON_F32:
         hipLaunchKernelGGL(expand_quads<float>, dim3((unsigned)grid), dim3(256), 0, s,

@@ -28,13 +28,13 @@ STORAGE = {"f32": (_lib.ARECON_F32, torch.float32), "f16": (_lib.ARECON_F16, tor
 # [ViewN][ZL-1][YL][2] (rows r and r+1 of each channel side by side); difference
 # quads [ViewN][ZL-1][YL-1][4] (sample, row, channel and cross differences).
 LAYOUT = {"rows": _lib.ARECON_FP_ROWS, "rowpairs": _lib.ARECON_FP_ROWPAIRS,
-          "quads": _lib.ARECON_FP_QUADS}
+          "quads": _lib.ARECON_FP_QUADS, "rowdiff": _lib.ARECON_FP_ROWDIFF}
 
 
 def filtered_shape(ViewN: int, ZL: int, YL: int, layout: str) -> tuple:
     if layout == "rows":
         return (ViewN, ZL, YL)
-    if layout == "rowpairs":
+    if layout in ("rowpairs", "rowdiff"):
         return (ViewN, ZL - 1, YL, 2)
     return (ViewN, ZL - 1, YL - 1, 4)
 
@@ -57,7 +57,8 @@ def choose_bp_kernel(params: BPParams, dz: float | None = None, storage: str = "
     """Backprojector launch shape for a geometry.  Consecutive slices' windows
     are N_2pi dz / h views apart; when a view lies in the windows of at least
     four consecutive slices, the z-column kernels over blocks of 8 slices share
-    the per-(column, view) work among them: row pairs for FP32 storage,
+    the per-(column, view) work among them: row differences for FP32 storage
+    (row pairs with the row difference stored: four instructions per blend),
     difference quads (same bytes, three FMAs per blend) for 16-bit storage.  The
     LDS-staged one (variant 3) copies each view's footprint to LDS and reads its
     samples there; a workgroup whose footprints do not fit runs the gathering
@@ -67,7 +68,7 @@ def choose_bp_kernel(params: BPParams, dz: float | None = None, storage: str = "
     L = 2 * window_half_width(params.N_2pi, params.HSCoef) + 1
     shift = abs(params.N_2pi * dzv / params.h)
     if shift * 4 <= L:
-        return dict(variant=3, nz=8, group=8, layout="rowpairs" if storage == "f32" else "quads")
+        return dict(variant=3, nz=8, group=8, layout="rowdiff" if storage == "f32" else "quads")
     return dict(variant=0, nz=0, group=0, layout="rows")
 
 
@@ -202,7 +203,8 @@ class HelicalFDK:
     sinogram (the segment trimming of src/analyticalReconBigPatient.m:80-93);
     ``bp_geom`` is the ScanGeom after trimming (BetaS shifted accordingly).
     ``storage`` is the element type of the filtered sinogram and ``layout`` its
-    layout ("rows", or "rowpairs" for the z-column backprojector, variant 2).
+    layout ("rows", or "rowpairs", "rowdiff" or "quads" for the z-column
+    backprojectors, variants 2 and 3).
     With ``variant=None`` the backprojector's launch shape and the layout come
     from ``choose_bp_kernel``.
     """
@@ -225,7 +227,7 @@ class HelicalFDK:
             variant, nz, group = k["variant"], nz or k["nz"], group or k["group"]
             layout = layout or k["layout"]
         elif variant == 3 and layout is None:   # the LDS-staged kernel's only layouts
-            layout = "rowpairs" if storage == "f32" else "quads"
+            layout = "rowdiff" if storage == "f32" else "quads"
         layout = layout or "rows"
         self.plan = BPPlan(self.params, dz=dz, nz=nz, order=order, variant=variant, group=group)
         self.variant, self.group = variant, group

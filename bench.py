@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--group", type=int, default=0, help="z-column kernel slice grouping")
     ap.add_argument("--overlap", type=int, default=1,
                     help="view chunks / slice-block groups of the overlapped pass (1: off)")
-    ap.add_argument("--layout", default=None, help="filtered layout: rows or rowpairs")
+    ap.add_argument("--layout", default=None,
+                    help="filtered layout: rows, rowpairs, rowdiff (FP32) or quads")
     ap.add_argument("--storages", default="f32,f16",
                     help="filtered-sinogram storages to time (plain f32 first)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -63,7 +64,7 @@ def parse():
 
 def kernel_name(fdk) -> str:
     """The backprojector's kernel as rocprofv3 names it (FP32 storage)."""
-    lay = {"rows": 0, "rowpairs": 1, "quads": 2}[fdk.layout]
+    lay = {"rows": 0, "rowpairs": 1, "quads": 2, "rowdiff": 3}[fdk.layout]
     inside = "true" if fdk.plan.inside else "false"
     if fdk.variant == 2:
         g = 0 if fdk.group == 8 else fdk.group
@@ -198,7 +199,7 @@ def main():
     # Issue bound of the gathers: the texture addresser takes four lane addresses
     # per clock, so one 64-lane gather instruction occupies it for 16 clocks; the
     # row-pair layout needs one gather per voxel-update, the rows layout two.
-    gathers = 1 if fdk.layout == "rowpairs" else 2
+    gathers = 2 if fdk.layout == "rows" else 1
     gather_peak = N_CU * CLOCK_HZ / 16.0 * 64 / gathers / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -129,7 +129,8 @@ def test_projector_sphere_known_answer(gpu):
     np.testing.assert_allclose(got, want, rtol=0, atol=2e-5 * (2 * mu * r))
 
 
-@pytest.mark.parametrize("storage,layout", [("f32", "rowpairs"), ("f16", "quads")])
+@pytest.mark.parametrize("storage,layout", [("f32", "rowpairs"), ("f32", "rowdiff"),
+                                            ("f16", "quads")])
 def test_overlapped_pass_equals_sequential_pass(gpu, storage, layout):
     """Rebin + filter in view chunks overlapped with block-range
     backprojections on a second stream: bit-identical to the plain pass, for

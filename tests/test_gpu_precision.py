@@ -39,7 +39,8 @@ def rel_l2(a, b):
 
 def stored_rows(fdk):
     """The filtered samples as stored, as FP32 rows [ViewN][ZL][YL] (quads keep
-    f(r, c) for r < ZL-1, c < YL-1 only; the rest is NaN)."""
+    f(r, c) for r < ZL-1, c < YL-1 only; the rest is NaN; the last row of the
+    row differences is their sum, f(ZL-2) + (f(ZL-1) - f(ZL-2)))."""
     f = fdk.filtered.float()
     if fdk.layout == "rows":
         return f.cpu().numpy()
@@ -47,6 +48,9 @@ def stored_rows(fdk):
     if fdk.layout == "rowpairs":
         out[:, :-1, :] = f[..., 0]
         out[:, -1, :] = f[:, -1, :, 1]
+    elif fdk.layout == "rowdiff":
+        out[:, :-1, :] = f[..., 0]
+        out[:, -1, :] = f[:, -1, :, 0] + f[:, -1, :, 1]
     else:
         out[:, :-1, :-1] = f[..., 0]
     return out.cpu().numpy()

@@ -85,7 +85,8 @@ def run_filter(P, PLC0, ZLC0, dYL, dZL, dtype=_lib.ARECON_F32, inplace=False,
     tdt = {_lib.ARECON_F32: torch.float32, _lib.ARECON_F16: torch.float16,
            _lib.ARECON_BF16: torch.bfloat16}[dtype]
     shape = {_lib.ARECON_FP_ROWS: din.shape, _lib.ARECON_FP_ROWPAIRS: (ViewN, ZL - 1, YL, 2),
-             _lib.ARECON_FP_QUADS: (ViewN, ZL - 1, YL - 1, 4)}[layout]
+             _lib.ARECON_FP_QUADS: (ViewN, ZL - 1, YL - 1, 4),
+             _lib.ARECON_FP_ROWDIFF: (ViewN, ZL - 1, YL, 2)}[layout]
     dout = din if inplace else torch.full(shape, float("nan"), dtype=tdt, device=din.device)
     _lib.check(_lib.lib().arecon_filter_dev(
         C.c_void_p(dout.data_ptr()), dtype, layout, C.c_void_p(din.data_ptr()), YL, ZL, ViewN,
@@ -149,6 +150,30 @@ def test_filter_quad_layout(gpu, ViewN, ZL, YL, dtype):
         want64 = quads(oracle.filter_f64(P.astype(np.float64),
                                          *[float(np.float32(v)) for v in args], SO))
         assert rel_l2(got, want64) < max(tol, 1e-5)
+
+
+@pytest.mark.parametrize("ViewN,ZL,YL", [(9, 64, 888), (5, 5, 130), (3, 2, 1024), (6, 4, 300),
+                                         (7, 3, 20), (4, 2, 2048)])
+def test_filter_row_difference_layout(gpu, ViewN, ZL, YL):
+    """Row differences (f(r), f(r+1) - f(r)) in FP32: bit-identical to those formed
+    from the rows output for even ZL (and for the in-place path, YL = 20 and
+    2048); to FP32 rounding of the filter for odd ZL (see the quad test)."""
+    print(f"filter rowdiff ViewN={ViewN} ZL={ZL} YL={YL}", flush=True)
+    P = np.random.default_rng(YL + 3 * ZL).normal(size=(ViewN, ZL, YL)).astype(np.float32)
+    args = ((YL - 1) / 2.0, (ZL - 1) / 2.0, 0.9493, 1.9231)
+    rows = run_filter(P, *args)
+    got = run_filter(P, *args, layout=_lib.ARECON_FP_ROWDIFF)
+    want = row_diffs(rows)
+    if ZL % 2 == 0 or YL > 1024 or YL < 128:
+        assert np.array_equal(got, want)
+    else:
+        assert rel_l2(got, want) < 1e-6
+
+
+def test_filter_row_difference_is_fp32_only(gpu):
+    P = np.zeros((2, 4, 100), np.float32)
+    with pytest.raises(RuntimeError, match="FP32 only"):
+        run_filter(P, 49.5, 1.5, 0.9, 1.9, dtype=_lib.ARECON_F16, layout=_lib.ARECON_FP_ROWDIFF)
 
 
 def test_filter_quads_leave_input_unchanged(gpu):
@@ -222,6 +247,12 @@ def row_pairs(fp):
     return np.stack([fp[:, :-1, :], fp[:, 1:, :]], axis=-1)
 
 
+def row_diffs(fp):
+    """[PN][ZL][YL] -> the row-difference layout [PN][ZL-1][YL][2], in FP32."""
+    f = np.asarray(fp, np.float32)
+    return np.stack([f[:, :-1, :], f[:, 1:, :] - f[:, :-1, :]], axis=-1)
+
+
 def quads(fp):
     """[PN][ZL][YL] -> the difference-quad layout [PN][ZL-1][YL-1][4], in FP32."""
     f = np.asarray(fp, np.float32)
@@ -235,7 +266,8 @@ def gpu_bp(fp, g, nz=0, order=0, storage=torch.float32, variant=0, group=0, layo
     p = P_(g, YL, ZL, PN)
     w = oracle.fdk_weights(g["N_2pi"], g["HSCoef"], 8)
     plan = BPPlan(p, nz=nz, order=order, w=w, variant=variant, group=group)
-    host = {"rows": lambda f: f, "rowpairs": row_pairs, "quads": quads}[layout](fp)
+    host = {"rows": lambda f: f, "rowpairs": row_pairs, "quads": quads,
+            "rowdiff": row_diffs}[layout](fp)
     dfp = torch.from_numpy(np.ascontiguousarray(host, dtype=np.float32)).cuda().to(storage)
     img = torch.full((g["YN"], g["XN"], g["ZN"]), float("nan"), device="cuda")
     plan.execute(img, dfp, layout=layout)
@@ -277,7 +309,7 @@ def test_bp_matches_fp64_oracle(gpu, nz, order, variant):
 @pytest.mark.parametrize("nz,group", [(8, 1), (8, 2), (8, 4), (8, 8), (16, 2), (16, 4), (16, 8),
                                       (32, 1), (32, 2), (32, 4)])
 @pytest.mark.parametrize("ZNC", [-15.37, -68.41])
-@pytest.mark.parametrize("layout", ["rows", "rowpairs", "quads"])
+@pytest.mark.parametrize("layout", ["rows", "rowpairs", "quads", "rowdiff"])
 def test_bp_z_column_matches_fp64_oracle(gpu, nz, group, ZNC, layout):
     """Z-column kernel: slice blocks whose windows overlap heavily (thin slices),
     a ragged last block, windows cut by the first (ZNC=-15.37) or the last
@@ -292,6 +324,30 @@ def test_bp_z_column_matches_fp64_oracle(gpu, nz, group, ZNC, layout):
     assert not np.isnan(got).any()
     assert np.abs(want).max() > 0
     assert rel_l2(got, want) < 1e-4
+
+
+@pytest.mark.parametrize("variant,nz", [(2, 8), (2, 16), (3, 8), (3, 16)])
+@pytest.mark.parametrize("ZNC", [-15.37, -68.41])
+def test_bp_row_differences_bit_identical_to_row_pairs(gpu, variant, nz, ZNC):
+    """FP32 row differences hold the very differences the row-pair blend forms,
+    so both z-column kernels give bit-identical volumes from either layout."""
+    print(f"bp rowdiff variant={variant} nz={nz} ZNC={ZNC}", flush=True)
+    g = realistic(ZN=37, dz=0.45, ZNC=ZNC)
+    fp = np.random.default_rng(nz + 3).normal(size=(200, 16, 128)).astype(np.float32)
+    a, w = gpu_bp(fp, g, nz=nz, variant=variant, group=8, layout="rowpairs")
+    b, _ = gpu_bp(fp, g, nz=nz, variant=variant, group=8, layout="rowdiff")
+    assert not np.isnan(b).any()
+    assert np.array_equal(a, b)
+    want = oracle.fdk_core(fp.astype(np.float64), w,
+                           **{k: v for k, v in g.items() if k not in ("YL", "ZL")})
+    assert rel_l2(b, want) < 1e-4
+
+
+def test_bp_row_differences_need_fp32(gpu):
+    g = realistic(ZN=37, dz=0.45, ZNC=-15.37)
+    fp = np.zeros((200, 16, 128), np.float32)
+    with pytest.raises(RuntimeError, match="FP32 only"):
+        gpu_bp(fp, g, nz=8, variant=2, group=8, layout="rowdiff", storage=torch.float16)
 
 
 @pytest.mark.parametrize("k1", [0.0, 2.85])
