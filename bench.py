@@ -187,30 +187,50 @@ def main():
     alg_bytes = 16.0 * updates + 4.0 * nvox
     achieved = alg_bytes / (bp_ms * 1e-3) / 1e9
     kname = kernel_name(fdk)
-    traffic = None
+    traffic, entry = None, None
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists():
         try:
             entry = json.loads(tfile.read_text()).get("kernels", {}).get(kname)
             if entry and entry.get("workload") == wl.name:
                 traffic = entry["fetch_bytes"] + entry["write_bytes"]
+            else:
+                entry = None
         except (ValueError, OSError, KeyError, TypeError):
-            traffic = None
-    # Issue bound of the gathers: the texture addresser takes four lane addresses
-    # per clock, so one 64-lane gather instruction occupies it for 16 clocks; the
-    # row-pair layout needs one gather per voxel-update, the rows layout two.
-    gathers = 2 if fdk.layout == "rows" else 1
-    gather_peak = N_CU * CLOCK_HZ / 16.0 * 64 / gathers / 1e9
+            traffic, entry = None, None
+    gvu = updates / (bp_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kname, "kernel_ms": round(bp_ms, 3), "bytes_per_update": 16,
-                "note": "gathers are served from L1/L2; compulsory HBM bytes are the filtered "
-                        "sinogram once and the volume once, so the gather issue rate, not HBM, "
-                        "bounds this kernel",
-                "gather_issue": {"achieved": round(updates / (bp_ms * 1e-3) / 1e9, 1),
-                                 "peak": round(gather_peak, 1), "unit": "GVoxel-updates/s",
-                                 "frac": round(updates / (bp_ms * 1e-3) / 1e9 / gather_peak, 4),
-                                 "gathers_per_update": gathers}}
+                "kernel": kname, "kernel_ms": round(bp_ms, 3), "bytes_per_update": 16}
+    if fdk.variant == 3:
+        # Samples come from LDS: each view's detector footprint is copied there once
+        # per workgroup.  The bound is VALU issue: one wavefront instruction per CU
+        # per clock (transcendentals issue on their own pipeline), so at V
+        # instructions per voxel-update the roof is N_CU * clock * 64 / V.
+        roofline["note"] = ("samples are read from LDS, where each view's detector footprint is "
+                            "staged once per workgroup; compulsory HBM bytes are the filtered "
+                            "sinogram and the volume; VALU issue bounds this kernel")
+        vpu = entry.get("valu_per_update") if entry else None
+        if vpu:
+            peak = N_CU * CLOCK_HZ * 64 / vpu / 1e9
+            roofline["valu_issue"] = {
+                "achieved": round(gvu, 1), "peak": round(peak, 1), "unit": "GVoxel-updates/s",
+                "frac": round(gvu / peak, 4), "valu_per_update": vpu,
+                "trans_per_update": entry.get("trans_per_update"),
+                "source": "profiles/sq_counters_bp_fdk3dw_zs_clinical.md"}
+    else:
+        # Issue bound of the gathers: the texture addresser takes four lane
+        # addresses per clock, so one 64-lane gather instruction occupies it for 16
+        # clocks; the row-pair, row-difference and quad layouts need one gather per
+        # voxel-update, the rows layout two.
+        gathers = 2 if fdk.layout == "rows" else 1
+        gather_peak = N_CU * CLOCK_HZ / 16.0 * 64 / gathers / 1e9
+        roofline["note"] = ("gathers are served from L1/L2; compulsory HBM bytes are the filtered "
+                            "sinogram once and the volume once, so the gather issue rate, not HBM, "
+                            "bounds this kernel")
+        roofline["gather_issue"] = {"achieved": round(gvu, 1), "peak": round(gather_peak, 1),
+                                    "unit": "GVoxel-updates/s", "frac": round(gvu / gather_peak, 4),
+                                    "gathers_per_update": gathers}
 
     stage_roof = {
         "rebin_GBs": round(8.0 * p.YL * p.ZL * fdk.ViewN / (plain["stages"][0] * 1e-3) / 1e9, 1),
