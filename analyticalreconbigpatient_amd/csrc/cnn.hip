@@ -29,6 +29,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <mutex>
 #include <vector>
 
 namespace arecon {
@@ -135,6 +137,119 @@ __global__ __launch_bounds__(256) void conv3_fwd(u16* __restrict__ y, const u16*
                 if (skip) r += bf2f(skip[n * cout + co0 + q]);
                 if (relu) r = fmaxf(r, 0.f);
                 o[q] = f2bf(r);
+            }
+            *reinterpret_cast<uint2*>(y + n * cout + co0) =
+                make_uint2((unsigned)o[0] | ((unsigned)o[1] << 16),
+                           (unsigned)o[2] | ((unsigned)o[3] << 16));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled forward convolution (also the data gradient) for Cin <= 64.  The
+// untiled kernel above fetches every operand through the texture addresser:
+// each activation is read again for each of the 27 taps, and that, not the
+// matrix cores, bounds it.  Here a workgroup (4 wavefronts) computes an output
+// tile of 4 (z) x NV (y) x 16 (x) voxels of one volume: wavefront w the NV rows
+// of 16 voxels at z0 + w, i.e. NV B tiles, times all MT output-channel tiles.
+// The input tile with its one-voxel halo, 6 x (NV+2) x 18 voxels x Cin BF16
+// channels, is staged in LDS once (zeros outside the volume), and every B
+// operand is one 16-byte LDS read.  A (weights) comes from global memory, the
+// same 16-byte pieces for every wavefront (L1 hits), once per K chunk and reused
+// over the NV tiles.  The K loop is the untiled kernel's (k = tap Cin + ci in
+// chunks of 32), so every output sees the same MFMA sequence and the result is
+// identical.
+// ---------------------------------------------------------------------------
+template <int MT, int NV, int CIN>
+__global__ __launch_bounds__(256) void conv3_fwd_tiled(u16* __restrict__ y,
+                                                       const u16* __restrict__ x,
+                                                       const u16* __restrict__ w,
+                                                       const float* __restrict__ bias,
+                                                       const u16* __restrict__ skip, Vol v,
+                                                       int cout, int relu,
+                                                       int tilesx, int tilesy, int tilesz) {
+    constexpr int cin = CIN, kp = (27 * CIN + 31) / 32 * 32, nchunks = kp / 32;
+    extern __shared__ uint4 tile[];          // [6][NV+2][18][cin/8] 16-byte pieces
+    constexpr int TY = NV + 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int t = blockIdx.x;
+    const int tx = t % tilesx; t /= tilesx;
+    const int ty = t % tilesy; t /= tilesy;
+    const int tz = t % tilesz;
+    const int b = t / tilesz;
+    const int x0 = tx * 16, y0 = ty * NV, z0 = tz * 4;
+    constexpr int c8 = CIN / 8;
+    constexpr int npieces = 6 * TY * 18 * c8;
+    const long long zbase = (long long)b * v.D;
+    for (int i = threadIdx.x; i < npieces; i += 256) {
+        const int p = i % c8;
+        int r = i / c8;
+        const int hx = r % 18;
+        r /= 18;
+        const int hy = r % TY, hz = r / TY;
+        const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
+        uint4 val = make_uint4(0u, 0u, 0u, 0u);
+        if (gz >= 0 && gz < v.D && gy >= 0 && gy < v.H && gx >= 0 && gx < v.W)
+            val = *reinterpret_cast<const uint4*>(
+                x + (((zbase + gz) * v.H + gy) * v.W + gx) * cin + 8 * p);
+        tile[i] = val;
+    }
+    __syncthreads();
+    const int col = lane & 15, kg = lane >> 4;
+    f32x4 acc[MT][NV];
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt)
+#pragma unroll
+        for (int r = 0; r < NV; ++r) acc[tt][r] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // A of the next chunk is loaded while this chunk's MFMAs run; all NV B reads
+    // of a chunk are issued before its first MFMA.  The padded K tail (k >= 27
+    // Cin, only for Cin = 8) has zero weights, so its B may be any finite
+    // samples: the tap is clamped to 26 for the address.
+    const u16* wl = w + (long long)col * kp + kg * 8;
+    bf16x8 a[MT], an[MT];
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt) a[tt] = load8(wl + (long long)16 * tt * kp);
+    for (int c = 0; c < nchunks; ++c) {
+        if (c + 1 < nchunks) {
+#pragma unroll
+            for (int tt = 0; tt < MT; ++tt) an[tt] = load8(wl + (long long)16 * tt * kp + 32 * (c + 1));
+        }
+        const int k0 = c * 32 + kg * 8;
+        const int tap_k = k0 / CIN, ci0 = k0 - tap_k * CIN;
+        const int tap = min(tap_k, 26);
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;     // halo-tile offsets
+        const int base = (((wave + dz) * TY + dy) * 18 + col + dx) * c8 + (ci0 >> 3);
+        bf16x8 bv[NV];
+#pragma unroll
+        for (int r = 0; r < NV; ++r) bv[r] = __builtin_bit_cast(bf16x8, tile[base + r * 18 * c8]);
+#pragma unroll
+        for (int r = 0; r < NV; ++r)
+#pragma unroll
+            for (int tt = 0; tt < MT; ++tt)
+                acc[tt][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[tt], bv[r], acc[tt][r], 0, 0, 0);
+        if (c + 1 < nchunks) {
+#pragma unroll
+            for (int tt = 0; tt < MT; ++tt) a[tt] = an[tt];
+        }
+    }
+    const int zz = z0 + wave, xx = x0 + col;
+    if (zz >= v.D || xx >= v.W) return;
+#pragma unroll
+    for (int r = 0; r < NV; ++r) {
+        const int yy = y0 + r;
+        if (yy >= v.H) continue;
+        const long long n = ((zbase + zz) * v.H + yy) * v.W + xx;
+#pragma unroll
+        for (int tt = 0; tt < MT; ++tt) {
+            const int co0 = 16 * tt + 4 * kg;
+            if (co0 >= cout) continue;
+            u16 o[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float rr = acc[tt][r][q] + (bias ? bias[co0 + q] : 0.f);
+                if (skip) rr += bf2f(skip[n * cout + co0 + q]);
+                if (relu) rr = fmaxf(rr, 0.f);
+                o[q] = f2bf(rr);
             }
             *reinterpret_cast<uint2*>(y + n * cout + co0) =
                 make_uint2((unsigned)o[0] | ((unsigned)o[1] << 16),
@@ -255,67 +370,111 @@ __global__ void to_channel_major3(u16* __restrict__ xT3, const u16* __restrict__
 }
 
 // 2x2x2 average pooling and its adjoint; nearest 2x upsampling and its adjoint.
+// One thread per voxel and 8 channels (one 16-byte piece; C is a multiple of 8).
+__device__ __forceinline__ void unpack8(const uint4& q, float* f) {
+    const unsigned u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        f[2 * i] = bf2f((u16)(u[i] & 0xffffu));
+        f[2 * i + 1] = bf2f((u16)(u[i] >> 16));
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float* f) {
+    unsigned u[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) u[i] = (unsigned)f2bf(f[2 * i]) | ((unsigned)f2bf(f[2 * i + 1]) << 16);
+    return make_uint4(u[0], u[1], u[2], u[3]);
+}
+
 __global__ void avgpool2(u16* __restrict__ y, const u16* __restrict__ x, Vol vo, int c) {
-    GRID_STRIDE(i, vo.nvox() * c) {
-        const int ch = (int)(i % c);
-        long long n = i / c;
+    const int c8 = c >> 3;
+    const uint4* x4 = reinterpret_cast<const uint4*>(x);
+    uint4* y4 = reinterpret_cast<uint4*>(y);
+    GRID_STRIDE(i, vo.nvox() * c8) {
+        const int cg = (int)(i % c8);
+        long long n = i / c8;
         const int xo = (int)(n % vo.W); n /= vo.W;
         const int yo = (int)(n % vo.H); n /= vo.H;
         const int zo = (int)(n % vo.D);
         const long long b = n / vo.D;
         const int Wi = 2 * vo.W, Hi = 2 * vo.H, Di = 2 * vo.D;
-        float s = 0.f;
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int d = 0; d < 8; ++d) {
             const long long ni = ((b * Di + 2 * zo + (d >> 2)) * Hi + 2 * yo + ((d >> 1) & 1)) * Wi +
                                  2 * xo + (d & 1);
-            s += bf2f(x[ni * c + ch]);
+            float f[8];
+            unpack8(x4[ni * c8 + cg], f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s[k] += f[k];
         }
-        y[i] = f2bf(0.125f * s);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] *= 0.125f;
+        y4[i] = pack8(s);
     }
 }
 __global__ void avgpool2_bwd(u16* __restrict__ dx, const u16* __restrict__ dy, Vol vi, int c,
                              int accumulate) {
-    GRID_STRIDE(i, vi.nvox() * c) {
-        const int ch = (int)(i % c);
-        long long n = i / c;
+    const int c8 = c >> 3;
+    const uint4* dy4 = reinterpret_cast<const uint4*>(dy);
+    uint4* dx4 = reinterpret_cast<uint4*>(dx);
+    GRID_STRIDE(i, vi.nvox() * c8) {
+        const int cg = (int)(i % c8);
+        long long n = i / c8;
         const int xi = (int)(n % vi.W); n /= vi.W;
         const int yi = (int)(n % vi.H); n /= vi.H;
         const int zi = (int)(n % vi.D);
         const long long b = n / vi.D;
         const long long no = ((b * (vi.D / 2) + zi / 2) * (vi.H / 2) + yi / 2) * (vi.W / 2) + xi / 2;
-        float g = 0.125f * bf2f(dy[no * c + ch]);
-        if (accumulate) g += bf2f(dx[i]);
-        dx[i] = f2bf(g);
+        float g[8];
+        unpack8(dy4[no * c8 + cg], g);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) g[k] *= 0.125f;
+        if (accumulate) {
+            float f[8];
+            unpack8(dx4[i], f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) g[k] += f[k];
+        }
+        dx4[i] = pack8(g);
     }
 }
 __global__ void upsample2(u16* __restrict__ y, const u16* __restrict__ x, Vol vo, int c) {
-    GRID_STRIDE(i, vo.nvox() * c) {
-        const int ch = (int)(i % c);
-        long long n = i / c;
+    const int c8 = c >> 3;
+    const uint4* x4 = reinterpret_cast<const uint4*>(x);
+    uint4* y4 = reinterpret_cast<uint4*>(y);
+    GRID_STRIDE(i, vo.nvox() * c8) {
+        const int cg = (int)(i % c8);
+        long long n = i / c8;
         const int xo = (int)(n % vo.W); n /= vo.W;
         const int yo = (int)(n % vo.H); n /= vo.H;
         const int zo = (int)(n % vo.D);
         const long long b = n / vo.D;
         const long long ni = ((b * (vo.D / 2) + zo / 2) * (vo.H / 2) + yo / 2) * (vo.W / 2) + xo / 2;
-        y[i] = x[ni * c + ch];
+        y4[i] = x4[ni * c8 + cg];
     }
 }
 __global__ void upsample2_bwd(u16* __restrict__ dx, const u16* __restrict__ dy, Vol vi, int c) {
-    GRID_STRIDE(i, vi.nvox() * c) {
-        const int ch = (int)(i % c);
-        long long n = i / c;
+    const int c8 = c >> 3;
+    const uint4* dy4 = reinterpret_cast<const uint4*>(dy);
+    uint4* dx4 = reinterpret_cast<uint4*>(dx);
+    GRID_STRIDE(i, vi.nvox() * c8) {
+        const int cg = (int)(i % c8);
+        long long n = i / c8;
         const int xi = (int)(n % vi.W); n /= vi.W;
         const int yi = (int)(n % vi.H); n /= vi.H;
         const int zi = (int)(n % vi.D);
         const long long b = n / vi.D;
         const int Wo = 2 * vi.W, Ho = 2 * vi.H, Do = 2 * vi.D;
-        float s = 0.f;
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int d = 0; d < 8; ++d) {
             const long long no = ((b * Do + 2 * zi + (d >> 2)) * Ho + 2 * yi + ((d >> 1) & 1)) * Wo +
                                  2 * xi + (d & 1);
-            s += bf2f(dy[no * c + ch]);
+            float f[8];
+            unpack8(dy4[no * c8 + cg], f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s[k] += f[k];
         }
-        dx[i] = f2bf(s);
+        dx4[i] = pack8(s);
     }
 }
 
@@ -414,6 +573,41 @@ void cnn_conv3_fwd(void* y, const void* x, const void* w, const float* bias, con
                    "cnn: output channels must be a multiple of 4 in [4, 128]");
     const int kp = (27 * cin + 31) / 32 * 32;
     const int mt = (cout + 15) / 16;
+    const char* tiled_env = std::getenv("ARECON_CNN_CONV_TILED");
+    if ((cin == 8 || cin == 16 || cin == 32 || cin == 64) && !(tiled_env && tiled_env[0] == '0')) {
+        const int nv = mt == 8 ? 4 : 8;            // rows of 16 voxels per wavefront
+        const int tilesx = (v.W + 15) / 16, tilesy = (v.H + nv - 1) / nv, tilesz = (v.D + 3) / 4;
+        const long long nblk = (long long)v.B * tilesz * tilesy * tilesx;
+        ARECON_REQUIRE(nblk < (1LL << 31), "cnn: volume too large for one launch");
+        const size_t lds = (size_t)6 * (nv + 2) * 18 * cin * 2;
+        const dim3 grid((unsigned)nblk), block(256);
+        auto* Y = static_cast<u16*>(y);
+        auto* X = static_cast<const u16*>(x);
+        auto* Wt = static_cast<const u16*>(w);
+        auto* S = static_cast<const u16*>(skip);
+        auto go = [&](auto kern) {
+            (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      160 * 1024);
+            hipLaunchKernelGGL(kern, grid, block, lds, s, Y, X, Wt, bias, S, v, cout, relu, tilesx,
+                               tilesy, tilesz);
+        };
+#define ARECON_CONV_C(C)                                                                      \
+    switch (mt) {                                                                             \
+    case 1: go(conv3_fwd_tiled<1, 8, C>); break;                                              \
+    case 2: go(conv3_fwd_tiled<2, 8, C>); break;                                              \
+    case 4: go(conv3_fwd_tiled<4, 8, C>); break;                                              \
+    case 8: go(conv3_fwd_tiled<8, 4, C>); break;                                              \
+    default: fail(ARECON_ERR_ARG, "cnn: output channels must be 16, 32, 64 or 128 (or <= 16)"); \
+    }
+        if (cin == 8) { ARECON_CONV_C(8) }
+        else if (cin == 16) { ARECON_CONV_C(16) }
+        else if (cin == 32) { ARECON_CONV_C(32) }
+        else if (cin == 64) { ARECON_CONV_C(64) }
+        else fail(ARECON_ERR_ARG, "cnn: tiled convolution takes 8, 16, 32 or 64 input channels");
+#undef ARECON_CONV_C
+        ARECON_HIP(hipGetLastError());
+        return;
+    }
     const long long waves = (v.nvox() + 31) / 32;
     const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
     auto* Y = static_cast<u16*>(y);
@@ -476,7 +670,8 @@ void cnn_channel_major3(void* xT3, const void* x, const int* dims, int c, hipStr
 
 void cnn_pool(void* y, const void* x, const int* dims_out, int c, hipStream_t s) {
     const Vol v = mkvol(dims_out);
-    hipLaunchKernelGGL(avgpool2, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+    ARECON_REQUIRE(c % 8 == 0, "cnn: pooling and upsampling take a multiple of 8 channels");
+    hipLaunchKernelGGL(avgpool2, dim3(grid_for(v.nvox() * (c / 8))), dim3(256), 0, s,
                        static_cast<u16*>(y), static_cast<const u16*>(x), v, c);
     ARECON_HIP(hipGetLastError());
 }
@@ -484,19 +679,22 @@ void cnn_pool_bwd(void* dx, const void* dy, const int* dims_in, int c, int accum
                   hipStream_t s) {
     const Vol v = mkvol(dims_in);
     ARECON_REQUIRE(v.D % 2 == 0 && v.H % 2 == 0 && v.W % 2 == 0, "cnn: pooling needs even dims");
-    hipLaunchKernelGGL(avgpool2_bwd, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+    ARECON_REQUIRE(c % 8 == 0, "cnn: pooling and upsampling take a multiple of 8 channels");
+    hipLaunchKernelGGL(avgpool2_bwd, dim3(grid_for(v.nvox() * (c / 8))), dim3(256), 0, s,
                        static_cast<u16*>(dx), static_cast<const u16*>(dy), v, c, accumulate);
     ARECON_HIP(hipGetLastError());
 }
 void cnn_upsample(void* y, const void* x, const int* dims_out, int c, hipStream_t s) {
     const Vol v = mkvol(dims_out);
-    hipLaunchKernelGGL(upsample2, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+    ARECON_REQUIRE(c % 8 == 0, "cnn: pooling and upsampling take a multiple of 8 channels");
+    hipLaunchKernelGGL(upsample2, dim3(grid_for(v.nvox() * (c / 8))), dim3(256), 0, s,
                        static_cast<u16*>(y), static_cast<const u16*>(x), v, c);
     ARECON_HIP(hipGetLastError());
 }
 void cnn_upsample_bwd(void* dx, const void* dy, const int* dims_in, int c, hipStream_t s) {
     const Vol v = mkvol(dims_in);
-    hipLaunchKernelGGL(upsample2_bwd, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+    ARECON_REQUIRE(c % 8 == 0, "cnn: pooling and upsampling take a multiple of 8 channels");
+    hipLaunchKernelGGL(upsample2_bwd, dim3(grid_for(v.nvox() * (c / 8))), dim3(256), 0, s,
                        static_cast<u16*>(dx), static_cast<const u16*>(dy), v, c);
     ARECON_HIP(hipGetLastError());
 }

@@ -74,6 +74,38 @@ def test_conv_forward(gpu, cin, cout, relu, skip):
     assert rel(got, want) < 8e-3
 
 
+@pytest.mark.parametrize("cin,cout,dims", [(8, 32, (2, 6, 5, 32)), (32, 32, (1, 9, 11, 40)),
+                                           (64, 32, (1, 5, 7, 20)), (32, 64, (2, 4, 8, 16)),
+                                           (64, 128, (1, 6, 6, 33)), (32, 8, (1, 8, 9, 17)),
+                                           (32, 16, (3, 4, 4, 4))])
+def test_conv_tiled_equals_untiled(gpu, cin, cout, dims, monkeypatch):
+    """The LDS-tiled forward convolution (Cin <= 64) runs the untiled kernel's
+    K loop in the same order, so both give identical BF16 outputs; ragged tiles
+    in every dimension, several volumes per batch."""
+    print(f"conv tiled {cin}->{cout} dims={dims}", flush=True)
+    g = torch.Generator().manual_seed(7 * cin + cout)
+    B, D, H, W = dims
+    x = bf(torch.randn(B, D, H, W, cin, generator=g))
+    w = bf(torch.randn(cout, 27, cin, generator=g) * 0.1)
+    b = (torch.randn(cout, generator=g) * 0.1).cuda()
+    s = to_dev_bf16(bf(torch.randn(B, D, H, W, cout, generator=g)))
+    wf = torch.zeros((cnn._r16(cout), cnn._kp(cin)), dtype=torch.int16, device="cuda")
+    _lib.check(_lib.lib().arecon_cnn_pack_weights(cnn._ptr(wf), cnn._ptr(None),
+                                                   cnn._ptr(w.cuda().contiguous()), cout, cin,
+                                                   st()), "pack")
+    xd = to_dev_bf16(x)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("ARECON_CNN_CONV_TILED", mode)
+        y = torch.full((B * D * H * W, cout), -1, dtype=torch.int16, device="cuda")
+        _lib.check(_lib.lib().arecon_cnn_conv3(cnn._ptr(y), cnn._ptr(xd), cnn._ptr(wf),
+                                                cnn._ptr(b), cnn._ptr(s), cnn._dims(B, D, H, W),
+                                                cin, cout, 1, st()), "conv")
+        torch.cuda.synchronize()
+        out[mode] = y
+    assert torch.equal(out["1"], out["0"])
+
+
 @pytest.mark.parametrize("cin,cout", [(8, 32), (32, 64), (128, 64), (32, 8)])
 def test_conv_gradients(gpu, cin, cout):
     """dX through the flipped/transposed weights and dW through the
