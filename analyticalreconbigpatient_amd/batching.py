@@ -54,7 +54,8 @@ class Shape:
     """Launch shape of the batched backprojector."""
     nz: int = 8          # slices per thread
     group: int = 8       # slices whose gathers are in flight together (8: all)
-    order: int = 0       # 0: slice-block-major workgroups, 1: column-tile-major
+    order: int = 0       # 0: slice-block-major workgroups, 1: column-tile-major,
+                         # 2: slice-block-major, tiles in bands of 6 rows
     slab_order: str = "given"   # or "largest_first"
 
 
@@ -230,7 +231,7 @@ def _slab_updates(p: BPParams, slab: Slab) -> int:
 
 
 DEFAULT_CANDIDATES = [Shape(8, 8, 0, "given"), Shape(8, 8, 0, "largest_first"),
-                      Shape(8, 8, 1, "given"), Shape(8, 4, 0, "given"),
+                      Shape(8, 8, 1, "given"), Shape(8, 8, 2, "given"), Shape(8, 4, 0, "given"),
                       Shape(16, 4, 0, "given"), Shape(8, 2, 0, "given")]
 
 

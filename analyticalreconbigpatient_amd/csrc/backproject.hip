@@ -73,6 +73,32 @@ struct BPArgs {
                                       // the detector (z-column kernels' INSIDE path)
 };
 
+// Logical workgroup lb -> slice block zs and 16x16-column tile (txb, tyb).
+// order 0: slice-major, the tiles of a block row by row; 1: tile-major;
+// 2: slice-major, the tiles of a block column by column within bands of
+// kTileBand tile rows.  The LDS-staged kernel is sensitive to the order (L2
+// reuse of the staged footprints between the ~160 workgroups resident on an
+// XCD): on the 512x512x400 clinical grid, bands of 6 rows take 29.9 ms against
+// 31.2 ms row by row (bands of 3 to 12 rows: 30.2-30.4 ms; tile-major: 42 ms).
+constexpr int kTileBand = 6;
+__device__ __forceinline__ void wg_tile(int lb, const BPArgs& A, int& zs, int& tyb, int& txb) {
+    int tile;
+    if (A.order == 1) { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
+    else { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
+    if (A.order != 2) {
+        tyb = tile / A.tilesx;
+        txb = tile - tyb * A.tilesx;
+        return;
+    }
+    const int tilesy = A.ntiles / A.tilesx;
+    const int band = tile / (kTileBand * A.tilesx);
+    const int y0 = band * kTileBand;
+    const int bh = min(kTileBand, tilesy - y0);           // rows in this band
+    const int idx = tile - band * kTileBand * A.tilesx;
+    txb = idx / bh;
+    tyb = y0 + idx - txb * bh;
+}
+
 // Two adjacent detector samples (channels U-1, U of one row) as floats.
 template <class T> __device__ __forceinline__ float2 load2(const T* p);
 
@@ -161,10 +187,8 @@ __global__ __launch_bounds__(256) void bp_fdk3dw(float* __restrict__ img,
     const int nb = gridDim.x, b = blockIdx.x;
     const int qx = nb >> 3, rx = nb & 7, xcd = b & 7;
     const int lb = xcd * qx + min(xcd, rx) + (b >> 3) + A.lb_offset;
-    int zs, tile;
-    if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
-    else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
-    const int tyb = tile / A.tilesx, txb = tile - tyb * A.tilesx;
+    int zs, tyb, txb;
+    wg_tile(lb, A, zs, tyb, txb);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int xi = txb * 16 + (wave & 1) * 8 + (lane & 7);
     const int yi = tyb * 16 + (wave >> 1) * 8 + (lane >> 3);
@@ -325,10 +349,8 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_vt(float* __restrict__ img,
     const int nb = gridDim.x, b = blockIdx.x;
     const int qx = nb >> 3, rx = nb & 7, xcd = b & 7;
     const int lb = xcd * qx + min(xcd, rx) + (b >> 3) + A.lb_offset;
-    int zs, tile;
-    if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
-    else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
-    const int tyb = tile / A.tilesx, txb = tile - tyb * A.tilesx;
+    int zs, tyb, txb;
+    wg_tile(lb, A, zs, tyb, txb);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int xi = txb * 16 + (wave & 1) * 8 + (lane & 7);
     const int yi = tyb * 16 + (wave >> 1) * 8 + (lane >> 3);
@@ -783,10 +805,8 @@ __device__ __forceinline__ void zc_run(int lb, const BPArgs& A, float* __restric
     table = uniform_ptr(table);
     const long long view_stride = uniform_i64(A.view_stride);
     const unsigned view_bytes = __builtin_amdgcn_readfirstlane(A.view_bytes);
-    int zs, tile;
-    if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
-    else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
-    const int tyb = tile / A.tilesx, txb = tile - tyb * A.tilesx;
+    int zs, tyb, txb;
+    wg_tile(lb, A, zs, tyb, txb);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int xi = txb * 16 + (wave & 1) * 8 + (lane & 7);
     const int yi = tyb * 16 + (wave >> 1) * 8 + (lane >> 3);
@@ -926,11 +946,20 @@ struct ZsView {      // footprint of one staged view (LDS)
 };
 
 // Footprint of view p for the tile centred at (xc, yc) and the active slices m
-// of the block (temp2 of slice j at erow[2 j]).
-template <int RMAX>
+// of the block (temp2 of slice j at erow[2 j]).  nrec: records per detector row
+// of the layout.
+//
+// Staging moves records in 16-byte pairs, and a pair whose first byte lies
+// outside the view's buffer loads as zeros as a whole.  A pair that straddles
+// the start (records -1 and 0) or the end of the buffer would therefore zero
+// the view's first or last record.  An update inside the detector (INSIDE:
+// channels in (1, YL-2)) never reads either; otherwise c0 is made even (row 0's
+// pairs then start at even records and never straddle record 0), and a
+// footprint whose pairs would straddle the last record runs the gathering body.
+template <int RMAX, bool INSIDE>
 __device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsigned m,
                                                cptr<float> erow, float xc, float yc,
-                                               float ex, float ey) {
+                                               float ex, float ey, int nrec) {
     ZsView v{ZS_SKIP, 0, -1, 0};
     if (m == 0u) return v;
     v.mode = ZS_DIRECT;
@@ -939,7 +968,8 @@ __device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsig
     const float eY = ex * fabsf(cs.y) + ey * fabsf(cs.x);
     const float eU = ex * fabsf(cs.x) + ey * fabsf(cs.y);
     const float ua = (Yrc - eY) * A.invdYL + A.YLC, ub = (Yrc + eY) * A.invdYL + A.YLC;
-    const int c0 = (int)floorf(fminf(ua, ub)) - 2;
+    int c0 = (int)floorf(fminf(ua, ub)) - 2;
+    if constexpr (!INSIDE) c0 &= ~1;                    // round down to even
     if ((int)ceilf(fmaxf(ua, ub)) + 2 > c0 + kZsCW - 1) return v;
     const float ya = fmaxf(fabsf(Yrc) - eY, 0.f), yb = fabsf(Yrc) + eY;
     const float t1lo2 = A.SO2 - yb * yb;
@@ -959,7 +989,16 @@ __device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsig
     const int rB = min((int)floorf(vhi) + 1, zlm1);
     if (rB <= -1 || rA >= zlm1) { v.mode = ZS_SKIP; return v; }   // only zero rows
     // rows [rA, rB] in 4-row loads, fewer than RMAX: the slot's last row stays zero
-    if (4 * ((rB - rA + 4) / 4) >= RMAX) return v;
+    const int nl = 4 * ((rB - rA + 4) / 4);
+    if (nl >= RMAX) return v;
+    if constexpr (!INSIDE) {
+        // last record of the view: channel nrec-1 of row ZL-2, staged as the first
+        // of a pair (an even distance from c0) when both lie in the footprint
+        const int last = zlm1 - 1;
+        if (rB + 1 - nl <= last && last <= rB && nrec - 1 <= c0 + kZsCW - 1 &&
+            ((nrec - 1 - c0) & 1) == 0)
+            return v;
+    }
     v.mode = ZS_LDS; v.rA = rA; v.rB = rB; v.c0 = c0;
     return v;
 }
@@ -989,10 +1028,8 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
 
     const int lb0 = lb;
     lb += A.lb_offset;
-    int zs, tile;
-    if (A.order == 0) { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
-    else { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
-    const int tyb = tile / A.tilesx, txb = tile - tyb * A.tilesx;
+    int zs, tyb, txb;
+    wg_tile(lb, A, zs, tyb, txb);
     const cptr<int> hbp = (cptr<int>)const_ptr(uniform_ptr(blocks) + zs);
     const int pmin = hbp[0], pmax = hbp[1];
     const long long hoff = *(cptr<long long>)(hbp + 2);
@@ -1009,11 +1046,13 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     // that does not fit runs the z-column kernel's body instead.  (Computing the
     // footprints where the views are staged instead frees the table's LDS for a
     // sixth workgroup per CU, but costs about two VALU per update: slower.)
+    const int nrec = LAYOUT == ARECON_FP_QUADS ? A.YL - 1 : A.YL;   // records per row
     int fits = pmax - pmin + 1 <= kZsMaxViews;
     if (fits)
         for (int p = pmin + (int)threadIdx.x; p <= pmax; p += 256) {
-            const ZsView v = zs_footprint<RMAX>(A, cld_f2(vt, p), mrow[2LL * p],
-                                                trow + 2LL * p * ZB, xc, yc, ex, ey);
+            const ZsView v = zs_footprint<RMAX, INSIDE>(A, cld_f2(vt, p), mrow[2LL * p],
+                                                        trow + 2LL * p * ZB, xc, yc, ex, ey,
+                                                        nrec);
             if (v.mode == ZS_DIRECT) fits = 0;
             fpt[p - pmin] = zs_pack(v);
         }
@@ -1502,7 +1541,7 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         zt[zi] = e;
     }
     ARECON_REQUIRE(g.variant >= 0 && g.variant <= 3, "bp: variant must be 0, 1, 2 or 3");
-    ARECON_REQUIRE(g.order == 0 || g.order == 1, "bp: order must be 0 or 1");
+    ARECON_REQUIRE(g.order >= 0 && g.order <= 2, "bp: order must be 0, 1 or 2");
     int nz;
     int group = 1;
     if (g.variant == 3) {
@@ -1780,7 +1819,7 @@ void bp_plan_launch(const BPPlan* P, float* img, const void* fp, int fp_dtype, i
     if (block_end < 0) block_end = P->a.nzsets;
     ARECON_REQUIRE(block_begin >= 0 && block_begin < block_end && block_end <= P->a.nzsets,
                    "bp: slice-block range outside the plan");
-    ARECON_REQUIRE(P->a.order == 0 || (block_begin == 0 && block_end == P->a.nzsets),
+    ARECON_REQUIRE(P->a.order != 1 || (block_begin == 0 && block_end == P->a.nzsets),
                    "bp: slice-block ranges need the slice-major workgroup order");
     const int lb_offset = block_begin * P->a.ntiles;
     const int grid = (block_end - block_begin) * P->a.ntiles;

@@ -68,8 +68,9 @@ def choose_bp_kernel(params: BPParams, dz: float | None = None, storage: str = "
     L = 2 * window_half_width(params.N_2pi, params.HSCoef) + 1
     shift = abs(params.N_2pi * dzv / params.h)
     if shift * 4 <= L:
-        return dict(variant=3, nz=8, group=8, layout="rowdiff" if storage == "f32" else "quads")
-    return dict(variant=0, nz=0, group=0, layout="rows")
+        return dict(variant=3, nz=8, group=8, order=2,
+                    layout="rowdiff" if storage == "f32" else "quads")
+    return dict(variant=0, nz=0, group=0, order=0, layout="rows")
 
 
 class BPPlan:
@@ -212,7 +213,8 @@ class HelicalFDK:
     def __init__(self, setup: ReconSetup, *, storage: str = "f32", layout: str | None = None,
                  first_view: int = 0,
                  n_views: int | None = None, bp_geom=None, dz: float | None = None,
-                 nz: int = 0, order: int = 0, variant: int | None = None, group: int = 0,
+                 nz: int = 0, order: int | None = None, variant: int | None = None,
+                 group: int = 0,
                  device: str = "cuda"):
         self.setup = setup
         self.ra = rebin_args(setup.geom)
@@ -226,8 +228,10 @@ class HelicalFDK:
             k = choose_bp_kernel(self.params, dz, storage)
             variant, nz, group = k["variant"], nz or k["nz"], group or k["group"]
             layout = layout or k["layout"]
+            order = k["order"] if order is None else order
         elif variant == 3 and layout is None:   # the LDS-staged kernel's only layouts
             layout = "rowdiff" if storage == "f32" else "quads"
+        order = 0 if order is None else order
         layout = layout or "rows"
         self.plan = BPPlan(self.params, dz=dz, nz=nz, order=order, variant=variant, group=group)
         self.variant, self.group = variant, group

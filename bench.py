@@ -37,7 +37,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and HU RMSE")
     ap.add_argument("--cpu-slices", type=int, default=2, help="slices in the CPU sample")
     ap.add_argument("--nz", type=int, default=0, help="slices per thread (0: chosen)")
-    ap.add_argument("--order", type=int, default=0)
+    ap.add_argument("--order", type=int, default=None,
+                    help="workgroup order (default: chosen with the kernel)")
     ap.add_argument("--variant", type=int, default=None,
                     help="backprojector kernel variant (default: chosen from the geometry)")
     ap.add_argument("--group", type=int, default=0, help="z-column kernel slice grouping")

@@ -183,7 +183,9 @@ typedef struct arecon_bp_geom {
     double SO, SD, k1;
     int nz;                     /* slices per thread: 0 = automatic; variants 0/1:
                                    1/2/4/8/16; variant 2: 8/16/32 */
-    int order;                  /* workgroup order: 0 = slice-major, 1 = column-major */
+    int order;                  /* workgroup order: 0 = slice-major, 1 = column-major,
+                                   2 = slice-major with the tiles of a slice block
+                                   column by column in bands of 6 tile rows */
     int variant;                /* 0 = view-table kernel (default), 1 = per-voxel weight,
                                    2 = z-column kernel (slice blocks with view masks),
                                    3 = z-column kernel reading each view's detector

@@ -38,11 +38,11 @@ def lds_geom(**kw):
     return g
 
 
-def run_bp(fp, g, variant, nz, storage, layout):
+def run_bp(fp, g, variant, nz, storage, layout, order=0):
     PN, ZL, YL = fp.shape
     p = P_(g, YL, ZL, PN)
     w = oracle.fdk_weights(g["N_2pi"], g["HSCoef"], 8)
-    plan = BPPlan(p, nz=nz, w=w, variant=variant, group=8)
+    plan = BPPlan(p, nz=nz, w=w, variant=variant, group=8, order=order)
     host = {"rowpairs": row_pairs, "quads": quads, "rowdiff": row_diffs}[layout](fp)
     dfp = torch.from_numpy(np.ascontiguousarray(host, dtype=np.float32)).cuda().to(storage)
     img = torch.full((g["YN"], g["XN"], g["ZN"]), float("nan"), device="cuda")
@@ -66,6 +66,26 @@ def test_lds_bit_identical_to_z_column(gpu, nz, storage, layout, ZNC):
     want, _, _ = run_bp(fp, g, 2, nz, storage, layout)
     assert not np.isnan(got).any()
     assert staged > 0 and direct == 0, (staged, direct)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("order,storage,layout", [
+    (0, torch.float32, "rowdiff"), (1, torch.float32, "rowdiff"), (2, torch.float32, "rowdiff"),
+    (0, torch.float32, "rowpairs"), (0, torch.float16, "quads"), (2, torch.bfloat16, "quads")])
+def test_lds_edges_and_workgroup_orders(gpu, order, storage, layout):
+    """Columns projecting onto the detector's first and last channels and rows
+    (the geometry is not INSIDE: the grid reaches past the detector's edges),
+    every workgroup order (14 tile rows: one full band of 12 rows and a partial
+    one for order 2; a partial tile column): bit-identical to the gathering
+    kernel.  Staging moves records in 16-byte pairs; a pair straddling the
+    view's first or last record must not zero it."""
+    print(f"bp lds edges order={order} {storage} {layout}", flush=True)
+    g = lds_geom(XN=40, YN=213, YNC=105.6)
+    fp = np.random.default_rng(21).normal(size=(200, 16, 128)).astype(np.float32)
+    got, _, (staged, _) = run_bp(fp, g, 3, 8, storage, layout, order=order)
+    want, _, _ = run_bp(fp, g, 2, 8, storage, layout, order=0)
+    assert staged > 0
+    assert not np.isnan(got).any()
     assert np.array_equal(got, want)
 
 

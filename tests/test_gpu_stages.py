@@ -290,7 +290,7 @@ def realistic(**kw):
 
 
 @pytest.mark.parametrize("nz", [1, 2, 4, 8, 16])
-@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("order", [0, 1, 2])
 @pytest.mark.parametrize("variant", [0, 1])
 def test_bp_matches_fp64_oracle(gpu, nz, order, variant):
     print(f"bp nz={nz} order={order} variant={variant}", flush=True)
