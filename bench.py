@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--cnn-pairs", type=int, default=200, help="training phantoms (cnn)")
     ap.add_argument("--cnn-clinical-pairs", type=int, default=300,
                     help="training crops of the 888x64 scan (cnn)")
-    ap.add_argument("--cnn-steps", type=int, default=5000,
+    ap.add_argument("--cnn-steps", type=int, default=10000,
                     help="training steps of each network (cnn)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch workload)")
     return ap.parse_args()
