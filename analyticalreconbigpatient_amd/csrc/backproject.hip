@@ -416,6 +416,7 @@ template <class T, int LAYOUT> struct Gather;
 
 template <class T> struct Gather<T, ARECON_FP_ROWS> {
     static constexpr unsigned kElem = sizeof(T);            // bytes per channel step
+    static constexpr bool kPair = false;
     struct Raw { float2 r0, r1; };
     __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t rs, unsigned ob,
                                                unsigned rowb) {
@@ -467,17 +468,24 @@ template <> struct Quad<__hip_bfloat16> {
     }
 };
 
+// Pair layouts (row pairs, row differences): chan() gives the two channels'
+// row-blended values (c0 on channel U-1, c1 on channel U); lerp() blends them,
+// and an unweighted update adds them to the accumulator directly (acc_pair).
 template <class T> struct Gather<T, ARECON_FP_ROWPAIRS> {
     static constexpr unsigned kElem = 2 * sizeof(T);
+    static constexpr bool kPair = true;
     typedef typename Quad<T>::Raw Raw;
     __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t rs, unsigned ob, unsigned) {
         return Quad<T>::load(rs, ob);
     }
-    __device__ static __forceinline__ float lerp(const Raw& raw, float fr, float Dey, float) {
+    __device__ static __forceinline__ float2 chan(const Raw& raw, float fr) {
         const float4 q = Quad<T>::get(raw);
-        const float c0 = fmaf(fr, q.y - q.x, q.x);        // channel U-1, rows blended
-        const float c1 = fmaf(fr, q.w - q.z, q.z);        // channel U
-        return fmaf(Dey, c0 - c1, c1);
+        return make_float2(fmaf(fr, q.y - q.x, q.x),      // channel U-1, rows blended
+                           fmaf(fr, q.w - q.z, q.z));     // channel U
+    }
+    __device__ static __forceinline__ float lerp(const Raw& raw, float fr, float Dey, float) {
+        const float2 c = chan(raw, fr);
+        return fmaf(Dey, c.x - c.y, c.y);
     }
 };
 
@@ -486,22 +494,33 @@ template <class T> struct Gather<T, ARECON_FP_ROWPAIRS> {
 // bit-identical to it in four instructions instead of six.
 template <class T> struct Gather<T, ARECON_FP_ROWDIFF> {
     static constexpr unsigned kElem = 2 * sizeof(T);
+    static constexpr bool kPair = true;
     typedef typename Quad<T>::Raw Raw;
     __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t rs, unsigned ob, unsigned) {
         return Quad<T>::load(rs, ob);
     }
-    __device__ static __forceinline__ float lerp(const Raw& raw, float fr, float Dey, float) {
+    __device__ static __forceinline__ float2 chan(const Raw& raw, float fr) {
         const float4 q = Quad<T>::get(raw);
-        const float c0 = fmaf(fr, q.y, q.x);              // channel U-1, rows blended
-        const float c1 = fmaf(fr, q.w, q.z);              // channel U
-        return fmaf(Dey, c0 - c1, c1);
+        return make_float2(fmaf(fr, q.y, q.x),            // channel U-1, rows blended
+                           fmaf(fr, q.w, q.z));           // channel U
+    }
+    __device__ static __forceinline__ float lerp(const Raw& raw, float fr, float Dey, float) {
+        const float2 c = chan(raw, fr);
+        return fmaf(Dey, c.x - c.y, c.y);
     }
 };
+
+// Unweighted update of a pair layout: acc + Dey c0 + wu c1 in two FMAs (the
+// blend and the add would take three).
+__device__ __forceinline__ float acc_pair(float acc, float2 c, float Dey, float wu) {
+    return fmaf(wu, c.y, fmaf(Dey, c.x, acc));
+}
 
 // Difference quads: (a, dr, dc, drc) with the sample at (row V-1 + fr, channel
 // U-1 + wu) equal to a + fr dr + wu (dc + fr drc), wu = 1 - Dey.
 template <class T> struct Gather<T, ARECON_FP_QUADS> {
     static constexpr unsigned kElem = 4 * sizeof(T);
+    static constexpr bool kPair = false;
     typedef typename Quad<T>::Raw Raw;
     __device__ static __forceinline__ Raw load(__amdgpu_buffer_rsrc_t rs, unsigned ob, unsigned) {
         return Quad<T>::load(rs, ob);
@@ -548,11 +567,13 @@ __device__ __forceinline__ void zc_group(cptr<float> e, float K, float zlc,
                         rowb);
 #pragma unroll
     for (int i = 0; i < G; ++i) {
-        const float tou = Gt::lerp(q[i], fr[i], Dey, wu);
-        if (WEIGHTED)   // 1 / (1 + 2^Q B); exactly 1 where a = 0
-            acc[i] = fmaf(fast_rcp(fmaf(e[2 * i + 1], B, 1.f)), tou, acc[i]);
+        if constexpr (WEIGHTED)   // 1 / (1 + 2^Q B); exactly 1 where a = 0
+            acc[i] = fmaf(fast_rcp(fmaf(e[2 * i + 1], B, 1.f)), Gt::lerp(q[i], fr[i], Dey, wu),
+                          acc[i]);
+        else if constexpr (Gt::kPair)
+            acc[i] = acc_pair(acc[i], Gt::chan(q[i], fr[i]), Dey, wu);
         else
-            acc[i] += tou;
+            acc[i] += Gt::lerp(q[i], fr[i], Dey, wu);
     }
 }
 
@@ -581,6 +602,33 @@ __device__ __forceinline__ void acc_weighted(float& acc, float t, unsigned wm, f
         "2:"
         : [acc] "+v"(acc), [w] "=&v"(w)
         : [t] "v"(t), [wm] "s"(wm), [j] "i"(J), [a] "s"(a), [B] "v"(B)
+        : "scc");
+}
+
+// acc_weighted for a pair layout, from the two channel values c = (c0, c1):
+// weighted, t = fma(Dey, c0 - c1, c1) (the layout's lerp) and the same weighted
+// add; unweighted, acc_pair's two FMAs.  c is clobbered (it holds t).
+template <int J>
+__device__ __forceinline__ void acc_weighted_p(float& acc, float2 c, float Dey, float wu,
+                                               unsigned wm, float a, float B) {
+    float w, u = c.x;
+    asm volatile(
+        "s_bitcmp1_b32 %[wm], %[j]\n\t"
+        "s_cbranch_scc0 1f\n\t"
+        "v_sub_f32 %[u], %[u], %[v]\n\t"
+        "v_fma_f32 %[w], %[a], %[B], 1.0\n\t"
+        "v_fma_f32 %[u], %[Dey], %[u], %[v]\n\t"
+        "v_rcp_f32 %[w], %[w]\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32 %[acc], %[w], %[u]\n\t"
+        "s_branch 2f\n"
+        "1:\n\t"
+        "v_fmac_f32 %[acc], %[Dey], %[u]\n\t"
+        "v_fmac_f32 %[acc], %[wu], %[v]\n"
+        "2:"
+        : [acc] "+v"(acc), [w] "=&v"(w), [u] "+v"(u)
+        : [v] "v"(c.y), [Dey] "v"(Dey), [wu] "v"(wu), [wm] "s"(wm), [j] "i"(J), [a] "s"(a),
+          [B] "v"(B)
         : "scc");
 }
 
@@ -671,6 +719,119 @@ __device__ __forceinline__ void acc_weighted_4(float& acc0, float& acc1, float& 
         : "scc");
 }
 
+// acc_weighted_4 for a pair layout, from the channel values u_i = c0, v_i = c1
+// of the four slices: no weighted slice, acc_i = fma(wu, v_i, fma(Dey, u_i,
+// acc_i)); every active slice weighted, t_i = fma(Dey, u_i - v_i, v_i) and
+// acc_i = fma(rcp(1 + a_i B), t_i, acc_i) (an inactive slice has u_i = v_i = 0
+// and a_i = 0 and adds exactly zero); mixed, slice by slice.  The u_i are
+// clobbered (they hold t_i).
+__device__ __forceinline__ void acc_weighted_4p(float& acc0, float& acc1, float& acc2,
+                                                float& acc3, float2 c0, float2 c1, float2 c2,
+                                                float2 c3, float Dey, float wu, unsigned wt,
+                                                unsigned act, float a0, float a1, float a2,
+                                                float a3, float B) {
+    float w0, w1, w2, w3, u0 = c0.x, u1 = c1.x, u2 = c2.x, u3 = c3.x;
+    asm volatile(
+        "s_cmp_eq_u32 %[wt], 0\n\t"
+        "s_cbranch_scc1 13f\n\t"
+        "s_cmp_eq_u32 %[wt], %[act]\n\t"
+        "s_cbranch_scc1 12f\n\t"
+        // mixed
+        "s_bitcmp1_b32 %[wt], 0\n\t"
+        "s_cbranch_scc0 20f\n\t"
+        "v_sub_f32 %[u0], %[u0], %[v0]\n\t"
+        "v_fma_f32 %[w0], %[a0], %[B], 1.0\n\t"
+        "v_fma_f32 %[u0], %[Dey], %[u0], %[v0]\n\t"
+        "v_rcp_f32 %[w0], %[w0]\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32 %[c0], %[w0], %[u0]\n\t"
+        "s_branch 21f\n"
+        "20:\n\t"
+        "v_fmac_f32 %[c0], %[Dey], %[u0]\n\t"
+        "v_fmac_f32 %[c0], %[wu], %[v0]\n"
+        "21:\n\t"
+        "s_bitcmp1_b32 %[wt], 1\n\t"
+        "s_cbranch_scc0 22f\n\t"
+        "v_sub_f32 %[u1], %[u1], %[v1]\n\t"
+        "v_fma_f32 %[w1], %[a1], %[B], 1.0\n\t"
+        "v_fma_f32 %[u1], %[Dey], %[u1], %[v1]\n\t"
+        "v_rcp_f32 %[w1], %[w1]\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32 %[c1], %[w1], %[u1]\n\t"
+        "s_branch 23f\n"
+        "22:\n\t"
+        "v_fmac_f32 %[c1], %[Dey], %[u1]\n\t"
+        "v_fmac_f32 %[c1], %[wu], %[v1]\n"
+        "23:\n\t"
+        "s_bitcmp1_b32 %[wt], 2\n\t"
+        "s_cbranch_scc0 24f\n\t"
+        "v_sub_f32 %[u2], %[u2], %[v2]\n\t"
+        "v_fma_f32 %[w2], %[a2], %[B], 1.0\n\t"
+        "v_fma_f32 %[u2], %[Dey], %[u2], %[v2]\n\t"
+        "v_rcp_f32 %[w2], %[w2]\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32 %[c2], %[w2], %[u2]\n\t"
+        "s_branch 25f\n"
+        "24:\n\t"
+        "v_fmac_f32 %[c2], %[Dey], %[u2]\n\t"
+        "v_fmac_f32 %[c2], %[wu], %[v2]\n"
+        "25:\n\t"
+        "s_bitcmp1_b32 %[wt], 3\n\t"
+        "s_cbranch_scc0 26f\n\t"
+        "v_sub_f32 %[u3], %[u3], %[v3]\n\t"
+        "v_fma_f32 %[w3], %[a3], %[B], 1.0\n\t"
+        "v_fma_f32 %[u3], %[Dey], %[u3], %[v3]\n\t"
+        "v_rcp_f32 %[w3], %[w3]\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32 %[c3], %[w3], %[u3]\n\t"
+        "s_branch 14f\n"
+        "26:\n\t"
+        "v_fmac_f32 %[c3], %[Dey], %[u3]\n\t"
+        "v_fmac_f32 %[c3], %[wu], %[v3]\n"
+        "s_branch 14f\n"
+        // every active slice weighted
+        "12:\n\t"
+        "v_sub_f32 %[u0], %[u0], %[v0]\n\t"
+        "v_sub_f32 %[u1], %[u1], %[v1]\n\t"
+        "v_sub_f32 %[u2], %[u2], %[v2]\n\t"
+        "v_sub_f32 %[u3], %[u3], %[v3]\n\t"
+        "v_fma_f32 %[w0], %[a0], %[B], 1.0\n\t"
+        "v_fma_f32 %[w1], %[a1], %[B], 1.0\n\t"
+        "v_fma_f32 %[w2], %[a2], %[B], 1.0\n\t"
+        "v_fma_f32 %[w3], %[a3], %[B], 1.0\n\t"
+        "v_rcp_f32 %[w0], %[w0]\n\t"
+        "v_rcp_f32 %[w1], %[w1]\n\t"
+        "v_rcp_f32 %[w2], %[w2]\n\t"
+        "v_rcp_f32 %[w3], %[w3]\n\t"
+        "v_fma_f32 %[u0], %[Dey], %[u0], %[v0]\n\t"
+        "v_fma_f32 %[u1], %[Dey], %[u1], %[v1]\n\t"
+        "v_fma_f32 %[u2], %[Dey], %[u2], %[v2]\n\t"
+        "v_fma_f32 %[u3], %[Dey], %[u3], %[v3]\n\t"
+        "v_fmac_f32 %[c0], %[w0], %[u0]\n\t"
+        "v_fmac_f32 %[c1], %[w1], %[u1]\n\t"
+        "v_fmac_f32 %[c2], %[w2], %[u2]\n\t"
+        "v_fmac_f32 %[c3], %[w3], %[u3]\n\t"
+        "s_branch 14f\n"
+        // no weighted slice
+        "13:\n\t"
+        "v_fmac_f32 %[c0], %[Dey], %[u0]\n\t"
+        "v_fmac_f32 %[c1], %[Dey], %[u1]\n\t"
+        "v_fmac_f32 %[c2], %[Dey], %[u2]\n\t"
+        "v_fmac_f32 %[c3], %[Dey], %[u3]\n\t"
+        "v_fmac_f32 %[c0], %[wu], %[v0]\n\t"
+        "v_fmac_f32 %[c1], %[wu], %[v1]\n\t"
+        "v_fmac_f32 %[c2], %[wu], %[v2]\n\t"
+        "v_fmac_f32 %[c3], %[wu], %[v3]\n"
+        "14:"
+        : [c0] "+v"(acc0), [c1] "+v"(acc1), [c2] "+v"(acc2), [c3] "+v"(acc3),
+          [w0] "=&v"(w0), [w1] "=&v"(w1), [w2] "=&v"(w2), [w3] "=&v"(w3),
+          [u0] "+v"(u0), [u1] "+v"(u1), [u2] "+v"(u2), [u3] "+v"(u3)
+        : [v0] "v"(c0.y), [v1] "v"(c1.y), [v2] "v"(c2.y), [v3] "v"(c3.y), [Dey] "v"(Dey),
+          [wu] "v"(wu), [wt] "s"(wt), [act] "s"(act), [a0] "s"(a0), [a1] "s"(a1),
+          [a2] "s"(a2), [a3] "s"(a3), [B] "v"(B)
+        : "scc");
+}
+
 // Gives a sample register an (unspecified) value without an instruction.  The
 // samples of slices outside a view's window are never read, but left undefined
 // the compiler merges them with the previous view's registers and copies every
@@ -685,8 +846,12 @@ __device__ __forceinline__ void zc_blend(const typename Gt::Raw* q, const float*
                                          const float2* ent, uint2 m, float B, float Dey,
                                          float wu, float* acc) {
     if constexpr (J < ZB) {
-        if ((m.x >> J) & 1u)
-            acc_weighted<J>(acc[J], Gt::lerp(q[J], fr[J], Dey, wu), m.y, ent[J].y, B);
+        if ((m.x >> J) & 1u) {
+            if constexpr (Gt::kPair)
+                acc_weighted_p<J>(acc[J], Gt::chan(q[J], fr[J]), Dey, wu, m.y, ent[J].y, B);
+            else
+                acc_weighted<J>(acc[J], Gt::lerp(q[J], fr[J], Dey, wu), m.y, ent[J].y, B);
+        }
         zc_blend<J + 1, ZB, Gt>(q, fr, ent, m, B, Dey, wu, acc);
     }
 }
@@ -728,15 +893,17 @@ struct ColTerms {
     int ub;
     bool uok;
 };
+// wmask: the view's weighted slices (B is computed only when it is nonzero);
+// ylc: YLC in a VGPR, so the channel FMA takes invdYL as its one scalar operand.
 template <bool INSIDE>
 __device__ __forceinline__ ColTerms column_terms(float x, float y, float2 cs, const BPArgs& A,
-                                                 bool weighted) {
+                                                 unsigned wmask, float ylc) {
     ColTerms c;
     const float UU = -x * cs.x - y * cs.y;
     const float Yr = -x * cs.y + y * cs.x;
     const float t1sq = A.SO2 - Yr * Yr;
     const float temp1 = fast_sqrt(t1sq);
-    const float U1 = Yr * A.invdYL + A.YLC;
+    const float U1 = fmaf(Yr, A.invdYL, ylc);
     const float den = fmaf(UU, temp1, t1sq);              // t1 (t1 + UU)
     const float inv = fast_rcp(den);
     c.B = 1.f;
@@ -748,16 +915,36 @@ __device__ __forceinline__ ColTerms column_terms(float x, float y, float2 cs, co
         c.Dey = 1.f - c.wu;
         c.uok = true;
         c.K = A.KSD * inv;
-        if (weighted) c.B = fast_exp2(A.k1 * fast_log2((temp1 - UU) * temp1 * inv));
+        // B = (t1 (t1 - UU) / den)^k1 = exp2(k1 log2(num inv)) behind a uniform
+        // branch on wmask (as C++ the compiler computes it for every view and
+        // selects).  Unset for a view without weighted slices, which never
+        // reads it.  The s_nops are the wait states after v_log / v_exp.
+        const float num = fmaf(-UU, temp1, t1sq);          // t1 (t1 - UU)
+        float B;
+        asm("s_cmp_eq_u32 %[wm], 0\n\t"
+            "s_cbranch_scc1 1f\n\t"
+            "v_mul_f32 %[B], %[num], %[inv]\n\t"
+            "v_log_f32 %[B], %[B]\n\t"
+            "s_nop 0\n\t"
+            "v_mul_f32 %[B], %[k1], %[B]\n\t"
+            "v_exp_f32 %[B], %[B]\n\t"
+            "s_nop 0\n"
+            "1:"
+            : [B] "=&v"(B)
+            : [wm] "s"(wmask), [num] "v"(num), [inv] "v"(inv), [k1] "s"(A.k1)
+            : "scc");
+        c.B = B;
     } else {
+        const bool weighted = wmask != 0u;
         c.uok = (U1 > 0.f) && (U1 <= A.YLm1) && (t1sq >= 0.f) && (den != 0.f);
         const float Uc = ceilf(U1);
         c.Dey = Uc - U1;
         c.wu = 1.f - c.Dey;
         c.ub = (int)Uc - 1;
         c.K = c.uok ? A.KSD * inv : 0.f;
+        // the INSIDE path's num, so both give the same B where it is in range
         if (weighted)   // kept finite and nonzero
-            c.B = fminf(fmaxf(fast_exp2(A.k1 * fast_log2(fabsf((temp1 - UU) * temp1 * inv))),
+            c.B = fminf(fmaxf(fast_exp2(A.k1 * fast_log2(fabsf(fmaf(-UU, temp1, t1sq) * inv))),
                               1e-30f), 1e30f);
     }
     return c;
@@ -827,8 +1014,9 @@ __device__ __forceinline__ void zc_run(int lb, const BPArgs& A, float* __restric
     const unsigned zlm1 = (unsigned)A.ZL - 1u;
     // ZLC as a per-lane value, so the per-slice fma takes the table's temp2 as
     // its one scalar operand.
-    float zlc;
+    float zlc, ylc;
     asm("v_mov_b32 %0, %1" : "=v"(zlc) : "s"(A.ZLC));
+    asm("v_mov_b32 %0, %1" : "=v"(ylc) : "s"(A.YLC));
     constexpr unsigned gmask = (1u << (G == 0 ? 1 : G)) - 1u;
     float acc[ZB];
 #pragma unroll
@@ -838,7 +1026,7 @@ __device__ __forceinline__ void zc_run(int lb, const BPArgs& A, float* __restric
         const uint2 m = cld_u2(mrow, p);               // scalar: active, weighted
         const float2 cs = cld_f2(vt, p);
         const __amdgpu_buffer_rsrc_t rs = view_rsrc(fp + (long long)p * view_stride, view_bytes);
-        const ColTerms ct = column_terms<INSIDE>(x, y, cs, A, m.y != 0u);
+        const ColTerms ct = column_terms<INSIDE>(x, y, cs, A, m.y, ylc);
         const float Dey = ct.Dey, wu = ct.wu, K = ct.K, B = ct.B;
         // byte offset of channel U-1; off the detector, past any view (< 2 GiB) so
         // both gathers read 0 whatever the row
@@ -1019,7 +1207,7 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
                                        const uint2* __restrict__ masks,
                                        const float2* __restrict__ table) {
     static_assert(Gather<T, LAYOUT>::kElem == 8, "LDS staging takes 8-byte records");
-    static_assert(ZB <= 16 && RMAX % 4 == 0, "bad block shape");
+    static_assert(ZB <= 16 && RMAX >= 5, "bad block shape");
     constexpr int kSlot = RMAX * kZsRowBytes;
     __shared__ __attribute__((aligned(16))) char stage0[4 * kSlot];
     __shared__ __attribute__((aligned(16))) unsigned fpt[kZsMaxViews];
@@ -1044,8 +1232,8 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
 
     // Pass 1: every view's footprint into the table.  A workgroup with a view
     // that does not fit runs the z-column kernel's body instead.  (Computing the
-    // footprints where the views are staged instead frees the table's LDS for a
-    // sixth workgroup per CU, but costs about two VALU per update: slower.)
+    // footprints where the views are staged instead frees the table's LDS, but
+    // costs about two VALU per update.)
     const int nrec = LAYOUT == ARECON_FP_QUADS ? A.YL - 1 : A.YL;   // records per row
     int fits = pmax - pmin + 1 <= kZsMaxViews;
     if (fits)
@@ -1086,8 +1274,9 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     constexpr unsigned kElem = Gt::kElem;
     const int rowb = (LAYOUT == ARECON_FP_QUADS ? A.YL - 1 : A.YL) * (int)kElem;
     lds_char* const lstage = (lds_char*)stage0;
-    float zlc, zoff;
+    float zlc, zoff, ylc;
     asm("v_mov_b32 %0, %1" : "=v"(zlc) : "s"(A.ZLC));
+    asm("v_mov_b32 %0, %1" : "=v"(ylc) : "s"(A.YLC));
     // a lane off the detector gets row -1e9, clamped to rA, and reads the slot's
     // zero row
     asm("v_mov_b32 %0, %1" : "=v"(zoff) : "s"(-1e9f));
@@ -1140,7 +1329,7 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             if (rB < rA) continue;                      // every sample would be zero
             const uint2 m = cld_u2(mrow, p);               // scalar: active, weighted
             const float2 cs = cld_f2(vt, p);
-            const ColTerms ct = column_terms<INSIDE>(x, y, cs, A, m.y != 0u);
+            const ColTerms ct = column_terms<INSIDE>(x, y, cs, A, m.y, ylc);
             const float Dey = ct.Dey, wu = ct.wu, K = ct.K, B = ct.B;
             const bool uok = ct.uok;
             const cptr<float> erow = trow + 2LL * p * ZB;
@@ -1168,35 +1357,73 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             }
             int rBv;   // clamp bound: row rB + 1, the slot's last row, kept zero
             asm("v_mov_b32 %0, %1" : "=v"(rBv) : "s"(rB + 1));
+            // Rows need no clamp in a group whose four slices are all active when
+            // the view's row range was not cut at the detector's first or last
+            // row and every lane is on the detector (INSIDE): the footprint then
+            // bounds every active slice's row with a row of margin on each side.
+            // clipped = 1 when the range was cut or the launch is not INSIDE
+            // (integer arithmetic keeps the flag scalar).
+            const unsigned clipped =
+                INSIDE ? ((unsigned)rA | (unsigned)(A.ZL - 2 - rB)) >> 31 : 1u;
             // groups of four slices, each skipped when none of its slices has the
             // view in its window, and branch-free inside: a slice outside the
             // window has temp2 = 1e30, so its row clamps onto the zero row (and a
             // lane off the detector reads the zero record): it adds exactly zero
 #pragma unroll
             for (int g = 0; g < ZB; g += 4) {
-                if (((m.x >> g) & 0xfu) == 0u) continue;
-                typename Gt::Raw q[4];
-                float fr[4];
+                const unsigned act = (m.x >> g) & 0xfu, wt = (m.y >> g) & 0xfu;
+                if (act == 0u) continue;
+                const unsigned clamp = (act ^ 0xfu) | clipped;
+                float V1[4], fr[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const float V1 = fmaf(ent[g + i].x, K, zl);
-                    int vf;
-                    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(vf) : "v"(V1));
-                    fr[i] = __builtin_amdgcn_fractf(V1);
-                    int r;   // one VGPR bound: a VOP3 reads at most one SGPR here
-                    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(vf), "s"(rA), "v"(rBv));
-                    int a;
-                    asm("v_lshl_add_u32 %0, %1, 8, %2" : "=v"(a) : "v"(r), "v"(colterm));
-                    q[i] = Lg::load(lstage + s * kSlot + a);
+                    V1[i] = fmaf(ent[g + i].x, K, zl);
+                    fr[i] = __builtin_amdgcn_fractf(V1[i]);
                 }
+                // rows (floor, clamped to [rA, rBv] unless the group needs no
+                // clamp) and LDS offsets of the four slices in one block: as
+                // separate blocks the clamp's two paths would give every
+                // accumulator a copy at the join
+                int a[4];
+                asm("v_cvt_flr_i32_f32 %[a0], %[v0]\n\t"
+                    "v_cvt_flr_i32_f32 %[a1], %[v1]\n\t"
+                    "v_cvt_flr_i32_f32 %[a2], %[v2]\n\t"
+                    "v_cvt_flr_i32_f32 %[a3], %[v3]\n\t"
+                    "s_cmp_eq_u32 %[cl], 0\n\t"
+                    "s_cbranch_scc1 1f\n\t"
+                    "v_med3_i32 %[a0], %[a0], %[rA], %[rB]\n\t"
+                    "v_med3_i32 %[a1], %[a1], %[rA], %[rB]\n\t"
+                    "v_med3_i32 %[a2], %[a2], %[rA], %[rB]\n\t"
+                    "v_med3_i32 %[a3], %[a3], %[rA], %[rB]\n"
+                    "1:\n\t"
+                    "v_lshl_add_u32 %[a0], %[a0], 8, %[ct]\n\t"
+                    "v_lshl_add_u32 %[a1], %[a1], 8, %[ct]\n\t"
+                    "v_lshl_add_u32 %[a2], %[a2], 8, %[ct]\n\t"
+                    "v_lshl_add_u32 %[a3], %[a3], 8, %[ct]"
+                    : [a0] "=&v"(a[0]), [a1] "=&v"(a[1]), [a2] "=&v"(a[2]), [a3] "=&v"(a[3])
+                    : [v0] "v"(V1[0]), [v1] "v"(V1[1]), [v2] "v"(V1[2]), [v3] "v"(V1[3]),
+                      [cl] "s"(clamp), [rA] "s"(rA), [rB] "v"(rBv), [ct] "v"(colterm)
+                    : "scc");
+                typename Gt::Raw q[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) q[i] = Lg::load(lstage + s * kSlot + a[i]);
                 // all four reads in flight before the first blend waits on one
                 __builtin_amdgcn_sched_barrier(0);
-                float t[4];
+                if constexpr (Gt::kPair) {
+                    float2 c[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) t[i] = Gt::lerp(q[i], fr[i], Dey, wu);
-                acc_weighted_4(acc[g], acc[g + 1], acc[g + 2], acc[g + 3], t[0], t[1], t[2], t[3],
-                               (m.y >> g) & 0xfu, (m.x >> g) & 0xfu, ent[g].y, ent[g + 1].y,
-                               ent[g + 2].y, ent[g + 3].y, B);
+                    for (int i = 0; i < 4; ++i) c[i] = Gt::chan(q[i], fr[i]);
+                    acc_weighted_4p(acc[g], acc[g + 1], acc[g + 2], acc[g + 3], c[0], c[1], c[2],
+                                    c[3], Dey, wu, wt, act, ent[g].y, ent[g + 1].y, ent[g + 2].y,
+                                    ent[g + 3].y, B);
+                } else {
+                    float t[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) t[i] = Gt::lerp(q[i], fr[i], Dey, wu);
+                    acc_weighted_4(acc[g], acc[g + 1], acc[g + 2], acc[g + 3], t[0], t[1], t[2],
+                                   t[3], wt, act, ent[g].y, ent[g + 1].y, ent[g + 2].y,
+                                   ent[g + 3].y, B);
+                }
             }
         }
     };
@@ -1724,8 +1951,11 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
     }
     if constexpr (NZ == 8 || NZ == 16) {
         if (P->variant == 3) {
-            // rows staged per view: the row span of NZ slices plus margins
-            constexpr int RMAX = NZ == 8 ? 24 : 40;
+            // rows staged per view: the row span of NZ slices plus margins.  Rows
+            // load four at a time, so 21 rows hold a 20-row footprint and the
+            // zero row: 25.9 KB of LDS per workgroup, six workgroups per CU (24
+            // rows fit the same footprints but only five workgroups: 9% slower)
+            constexpr int RMAX = NZ == 8 ? 21 : 40;
 #define ARECON_ZS_(LY)                                                                        \
     do {                                                                                      \
         if (a.inside)                                                                         \

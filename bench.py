@@ -71,7 +71,7 @@ def kernel_name(fdk) -> str:
         g = 0 if fdk.group == 8 else fdk.group
         return f"bp_fdk3dw_zc<{fdk.plan_nz}, {g}, float, {lay}, {g == 0 and inside or 'false'}>"
     if fdk.variant == 3:
-        rmax = 24 if fdk.plan_nz == 8 else 40
+        rmax = 21 if fdk.plan_nz == 8 else 40
         return f"bp_fdk3dw_zs<{fdk.plan_nz}, float, {lay}, {rmax}, {inside}>"
     return f"bp_fdk3dw{'_vt' if fdk.variant == 0 else ''}<{fdk.plan_nz}, float>"
 
