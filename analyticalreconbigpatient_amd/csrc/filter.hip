@@ -346,6 +346,14 @@ template <int N, int S = 0> constexpr int tpt() {
     return N >= 2048 ? (S == 1 ? N / 8 : N / 16) : 64;
 }
 
+// LDS slot of element k of a transform: one float2 of padding after every 8.
+// A pass writes its outputs R apart (k = 8 j + r for the first radix-8 pass), so
+// unpadded the 64-bit writes of 16 lanes would hit only two of the 32 banks;
+// padded they cover all 32.  Reads (consecutive k) lose at most a 2-way
+// conflict.  An LDS transform buffer holds N + N / 8 float2.
+__device__ __forceinline__ int lpad(int k) { return k + (k >> 3); }
+template <int N> constexpr int lds_fft_len() { return N + N / 8; }
+
 template <int N, int R, int Ns, int SRC, int DST, class OutT, int S>
 __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict__ twg,
                                           const RowCtx& c, const RowOut<OutT>& o) {
@@ -361,7 +369,10 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
         for (int r = 0; r < R; ++r) {
             const int n = j + r * NB;
             if (SRC == SRC_LDS) {
-                v[b][r] = lds[n];
+                // lpad(n) = lpad(j) + lpad(r NB) (NB a multiple of 8): one base
+                // register and immediate offsets
+                static_assert(NB % 8 == 0, "padded reads need NB % 8 == 0");
+                v[b][r] = lds[lpad(j) + lpad(r * NB)];
             } else if (r < R / 2) {                          // n < N/2; n >= YL reads zero
                 float2 x = make_float2(0.f, 0.f);
                 if (n < c.YL) {
@@ -382,15 +393,21 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
         if (Ns > 1) twiddle<R>(v[b], twg[jm * (N / (Ns * R))]);
         dft<R>(v[b]);
         const int d = (j - jm) * R + jm;
+        // lpad(d + r Ns) = lpad(d) + lpad(r Ns) when the low three bits of d and
+        // r Ns do not carry: Ns a multiple of 8, or Ns = 1 with d a multiple of
+        // R >= 4 (r < R).  Then one base register serves every r.
+        constexpr bool kSplit = Ns % 8 == 0 || (Ns == 1 && R >= 4 && R % 4 == 0);
+        const int dp = lpad(d);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int k = d + r * Ns;
+            const int kp = kSplit ? dp + lpad(r * Ns) : lpad(k);
             if (DST == DST_LDS) {
-                lds[k] = v[b][r];
+                lds[kp] = v[b][r];
             } else if (DST == DST_LDS_SPEC) {
                 const float hs = c.H[k];
-                lds[k] = make_float2(hs * v[b][r].x, -hs * v[b][r].y);
-            } else if (DST == DST_LDS_ROWS) {
+                lds[kp] = make_float2(hs * v[b][r].x, -hs * v[b][r].y);
+            } else if (DST == DST_LDS_ROWS) {   // unpadded rows (2 N floats)
                 if (k < c.YL) {
                     float* rows = reinterpret_cast<float*>(lds);
                     rows[k] = v[b][r].x;
@@ -443,7 +460,7 @@ __global__ __launch_bounds__((tpt<N, S>())) void filter_rows_wave(
     OutT* out, int layout, const float* in, const float2* __restrict__ twg,
     const float* __restrict__ Hg, const float* __restrict__ Wg, int YL, int ZL,
     long long nrows, float PLC, float ZLC, float dYL, float dZL, float SO) {
-    __shared__ float2 lds[N];
+    __shared__ float2 lds[lds_fft_len<N>()];
     const long long npairs = (nrows + 1) >> 1;
     for (long long pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
         const long long ra = 2 * pair;
@@ -626,7 +643,7 @@ __global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
     const float* __restrict__ Hg, const float* __restrict__ Wg, int YL, int ZL, int ViewN,
     float PLC, float dYL, float SO) {
     constexpr int T = tpt<N, S>();
-    __shared__ float2 lds[N];
+    __shared__ float2 lds[lds_fft_len<N>()];
     __shared__ float prev[2][N / 2];             // filtered row k-1, double-buffered
     const float* la = reinterpret_cast<const float*>(lds);
     const float* lb = la + N;
