@@ -99,7 +99,7 @@ class BatchedRecon:
         assert len(setups) == len(raws)
         self.setups, self.raws, self.slabs = setups, raws, list(slabs)
         self.storage = storage
-        self.layout = layout or ("rowdiff" if storage == "f32" else "quads")
+        self.layout = layout or "rowdiff"
         self.dev = torch.device(device)
         code, tdtype = STORAGE[storage]
         self.dtype_code = code

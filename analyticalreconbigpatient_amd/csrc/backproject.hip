@@ -1098,12 +1098,13 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
 //    reaching the source circle) is read straight from global memory as in the
 //    z-column kernel; a view whose rows all lie off the detector is skipped
 //    (every sample it would read is zero).
-// Records are 8 bytes: FP32 row pairs (rows r, r+1 of a channel) or 16-bit
-// difference quads; a staged row is CW records = 256 B.  Each wave stages one
-// view of a chunk of four, then every wave consumes the four views.
+// Records are 8 bytes (FP32 row pairs or row differences, 16-bit difference
+// quads; a staged row of CW records is 256 B) or 4 bytes (16-bit row
+// differences; 128 B rows, half the LDS per view, so more workgroups per CU).
+// Each wave stages one view of a chunk of four, then every wave consumes the
+// four views.
 // ---------------------------------------------------------------------------
 constexpr int kZsCW = 32;                    // records per staged row
-constexpr int kZsRowBytes = kZsCW * 8;       // bytes per staged row
 constexpr int kZsMaxViews = 1024;            // views per block in the footprint table
 enum { ZS_SKIP = 0, ZS_LDS = 1, ZS_DIRECT = 2 };
 
@@ -1122,6 +1123,16 @@ template <> struct LdsGather<float, ARECON_FP_ROWPAIRS> {   // records c, c+1 (1
     }
 };
 template <> struct LdsGather<float, ARECON_FP_ROWDIFF> : LdsGather<float, ARECON_FP_ROWPAIRS> {};
+template <class T> struct LdsGather<T, ARECON_FP_ROWDIFF> {   // 16-bit: records c, c+1 (8 B)
+    typedef typename Gather<T, ARECON_FP_ROWDIFF>::Raw Raw;
+    __device__ static __forceinline__ Raw load(const lds_char* a) {
+        // 4-byte records: the pair is only 4-byte aligned, so two 4-byte reads
+        typedef const __attribute__((address_space(3))) unsigned lds_u32;
+        const unsigned lo = *reinterpret_cast<lds_u32*>(a);
+        const unsigned hi = *reinterpret_cast<lds_u32*>(a + 4);
+        return ((unsigned long long)hi << 32) | lo;
+    }
+};
 template <class T> struct LdsGather<T, ARECON_FP_QUADS> {         // one quad (8 B)
     typedef typename Gather<T, ARECON_FP_QUADS>::Raw Raw;
     __device__ static __forceinline__ Raw load(const lds_char* a) {
@@ -1137,14 +1148,15 @@ struct ZsView {      // footprint of one staged view (LDS)
 // of the block (temp2 of slice j at erow[2 j]).  nrec: records per detector row
 // of the layout.
 //
-// Staging moves records in 16-byte pairs, and a pair whose first byte lies
-// outside the view's buffer loads as zeros as a whole.  A pair that straddles
-// the start (records -1 and 0) or the end of the buffer would therefore zero
-// the view's first or last record.  An update inside the detector (INSIDE:
-// channels in (1, YL-2)) never reads either; otherwise c0 is made even (row 0's
-// pairs then start at even records and never straddle record 0), and a
-// footprint whose pairs would straddle the last record runs the gathering body.
-template <int RMAX, bool INSIDE>
+// PAIRS (8-byte records): staging moves records in 16-byte pairs, and a pair
+// whose first byte lies outside the view's buffer loads as zeros as a whole.  A
+// pair that straddles the start (records -1 and 0) or the end of the buffer
+// would therefore zero the view's first or last record.  An update inside the
+// detector (INSIDE: channels in (1, YL-2)) never reads either; otherwise c0 is
+// made even (row 0's pairs then start at even records and never straddle record
+// 0), and a footprint whose pairs would straddle the last record runs the
+// gathering body.  4-byte records are staged one per lane and need neither.
+template <int RMAX, bool INSIDE, bool PAIRS>
 __device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsigned m,
                                                cptr<float> erow, float xc, float yc,
                                                float ex, float ey, int nrec) {
@@ -1157,7 +1169,7 @@ __device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsig
     const float eU = ex * fabsf(cs.x) + ey * fabsf(cs.y);
     const float ua = (Yrc - eY) * A.invdYL + A.YLC, ub = (Yrc + eY) * A.invdYL + A.YLC;
     int c0 = (int)floorf(fminf(ua, ub)) - 2;
-    if constexpr (!INSIDE) c0 &= ~1;                    // round down to even
+    if constexpr (!INSIDE && PAIRS) c0 &= ~1;           // round down to even
     if ((int)ceilf(fmaxf(ua, ub)) + 2 > c0 + kZsCW - 1) return v;
     const float ya = fmaxf(fabsf(Yrc) - eY, 0.f), yb = fabsf(Yrc) + eY;
     const float t1lo2 = A.SO2 - yb * yb;
@@ -1179,7 +1191,7 @@ __device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsig
     // rows [rA, rB] in 4-row loads, fewer than RMAX: the slot's last row stays zero
     const int nl = 4 * ((rB - rA + 4) / 4);
     if (nl >= RMAX) return v;
-    if constexpr (!INSIDE) {
+    if constexpr (!INSIDE && PAIRS) {
         // last record of the view: channel nrec-1 of row ZL-2, staged as the first
         // of a pair (an even distance from c0) when both lie in the footprint
         const int last = zlm1 - 1;
@@ -1206,9 +1218,13 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
                                        const ZBlock* __restrict__ blocks,
                                        const uint2* __restrict__ masks,
                                        const float2* __restrict__ table) {
-    static_assert(Gather<T, LAYOUT>::kElem == 8, "LDS staging takes 8-byte records");
+    constexpr int kRec = (int)Gather<T, LAYOUT>::kElem;   // bytes per record
+    static_assert(kRec == 8 || kRec == 4, "LDS staging takes 8- or 4-byte records");
     static_assert(ZB <= 16 && RMAX >= 5, "bad block shape");
-    constexpr int kSlot = RMAX * kZsRowBytes;
+    constexpr int kRowB = kZsCW * kRec;                    // bytes per staged row
+    constexpr int kRowShift = kRec == 8 ? 8 : 7, kRecShift = kRec == 8 ? 3 : 2;
+    static_assert(kRowB == 1 << kRowShift && kRec == 1 << kRecShift, "shifts");
+    constexpr int kSlot = RMAX * kRowB;
     __shared__ __attribute__((aligned(16))) char stage0[4 * kSlot];
     __shared__ __attribute__((aligned(16))) unsigned fpt[kZsMaxViews];
     typedef Gather<T, LAYOUT> Gt;
@@ -1238,7 +1254,7 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     int fits = pmax - pmin + 1 <= kZsMaxViews;
     if (fits)
         for (int p = pmin + (int)threadIdx.x; p <= pmax; p += 256) {
-            const ZsView v = zs_footprint<RMAX, INSIDE>(A, cld_f2(vt, p), mrow[2LL * p],
+            const ZsView v = zs_footprint<RMAX, INSIDE, kRec == 8>(A, cld_f2(vt, p), mrow[2LL * p],
                                                         trow + 2LL * p * ZB, xc, yc, ex, ey,
                                                         nrec);
             if (v.mode == ZS_DIRECT) fits = 0;
@@ -1249,10 +1265,10 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     // view's last staged row (slices outside the view's window, temp2 = 1e30,
     // and rows off the detector's end) clamp onto it and add exactly zero, and
     // lanes off the detector read it
-    for (int i = threadIdx.x; i < 4 * kZsRowBytes / 4; i += 256) {
-        const int slot = i / (kZsRowBytes / 4), w = i - slot * (kZsRowBytes / 4);
+    for (int i = threadIdx.x; i < 4 * kRowB / 4; i += 256) {
+        const int slot = i / (kRowB / 4), w = i - slot * (kRowB / 4);
         reinterpret_cast<__attribute__((address_space(3))) unsigned*>(
-            (lds_char*)stage0 + slot * kSlot + (RMAX - 1) * kZsRowBytes)[w] = 0u;
+            (lds_char*)stage0 + slot * kSlot + (RMAX - 1) * kRowB)[w] = 0u;
     }
     const int staged = __syncthreads_and(fits);
     if (threadIdx.x == 0 && A.stats) atomicAdd(A.stats + (staged ? 0 : 1), 1ull);
@@ -1296,18 +1312,28 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
                     view_rsrc(fp + (long long)p * view_stride, view_bytes);
                 // nl rows [rB + 1 - nl, rB] (nl = rB - rA + 1 rounded up to a
                 // multiple of four) go to slot rows [RMAX - 1 - nl, RMAX - 2], so row
-                // rB + 1 lands on the slot's zero last row.  Lane: row
-                // rB + 1 - nl + i + lane / 16, records c0 + 2 (lane % 16) + {0, 1};
-                // rows before the view's first or after its last record (-1, ZL-1
-                // and beyond) load as zeros
+                // rB + 1 lands on the slot's zero last row.  Rows before the view's
+                // first or after its last record (-1, ZL-1 and beyond) load as zeros.
                 const int nl = 4 * ((rB - rA + 4) / 4);
-                const int off = (rB + 1 - nl + (lane >> 4)) * rowb +
-                                (c0 + 2 * (lane & 15)) * (int)kElem;
-                lds_char* dst = lstage + wave * kSlot + (RMAX - 1 - nl) * kZsRowBytes;
-                for (int i = 0; i < nl; i += 4)
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                        rs, (lds_void*)(dst + i * kZsRowBytes), 16,
-                        (unsigned)(off + i * rowb), 0, 0, 0);
+                lds_char* dst = lstage + wave * kSlot + (RMAX - 1 - nl) * kRowB;
+                if constexpr (kRec == 8) {
+                    // 16-byte loads, four rows each.  Lane: row
+                    // rB + 1 - nl + i + lane / 16, records c0 + 2 (lane % 16) + {0, 1}
+                    const int off = (rB + 1 - nl + (lane >> 4)) * rowb +
+                                    (c0 + 2 * (lane & 15)) * kRec;
+                    for (int i = 0; i < nl; i += 4)
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                            rs, (lds_void*)(dst + i * kRowB), 16, (unsigned)(off + i * rowb),
+                            0, 0, 0);
+                } else {
+                    // 4-byte loads (one record per lane), two rows each.  Lane: row
+                    // rB + 1 - nl + i + lane / 32, record c0 + lane % 32
+                    const int off = (rB + 1 - nl + (lane >> 5)) * rowb + (c0 + (lane & 31)) * kRec;
+                    for (int i = 0; i < nl; i += 2)
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                            rs, (lds_void*)(dst + i * kRowB), 4, (unsigned)(off + i * rowb), 0,
+                            0, 0);
+                }
             }
         }
     };
@@ -1336,7 +1362,7 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             float2 ent[ZB];
 #pragma unroll
             for (int j = 0; j < ZB; ++j) ent[j] = cld_f2(erow, j);
-            // byte offset in the slot = row * kZsRowBytes + colterm, row clamped to
+            // byte offset in the slot = row * kRowB + colterm, row clamped to
             // [rA, rBv]; a lane off the detector gets row -1e9 (clamped to rA) and
             // the slot's zero row
             const float zl = uok ? zlc : zoff;
@@ -1344,15 +1370,15 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             // rest of the address is one per-view VGPR, kept whole (opaque to the
             // compiler, which otherwise splits its scalar part off and adds it
             // back at every update), so an update's address is one v_lshl_add_u32
-            static_assert(kZsRowBytes == 256, "row stride is a shift by 8");
-            const int cterm = -c0 * 8 + (RMAX - 2 - rB) * kZsRowBytes;   // row rB at RMAX - 2
+            const int cterm = -c0 * kRec + (RMAX - 2 - rB) * kRowB;   // row rB at RMAX - 2
             int colterm;
             if constexpr (INSIDE) {
-                asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(colterm) : "v"(ct.ub), "s"(cterm));
+                asm("v_lshl_add_u32 %0, %1, %2, %3"
+                    : "=v"(colterm) : "v"(ct.ub), "i"(kRecShift), "s"(cterm));
             } else {
                 // the slot's zero row (row rBv): r clamps to rA off the detector
-                const int zterm = (RMAX - 1 - rA) * kZsRowBytes;
-                const int c = uok ? ct.ub * 8 + cterm : zterm;
+                const int zterm = (RMAX - 1 - rA) * kRowB;
+                const int c = uok ? ct.ub * kRec + cterm : zterm;
                 asm("v_mov_b32 %0, %1" : "=v"(colterm) : "v"(c));
             }
             int rBv;   // clamp bound: row rB + 1, the slot's last row, kept zero
@@ -1396,13 +1422,14 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
                     "v_med3_i32 %[a2], %[a2], %[rA], %[rB]\n\t"
                     "v_med3_i32 %[a3], %[a3], %[rA], %[rB]\n"
                     "1:\n\t"
-                    "v_lshl_add_u32 %[a0], %[a0], 8, %[ct]\n\t"
-                    "v_lshl_add_u32 %[a1], %[a1], 8, %[ct]\n\t"
-                    "v_lshl_add_u32 %[a2], %[a2], 8, %[ct]\n\t"
-                    "v_lshl_add_u32 %[a3], %[a3], 8, %[ct]"
+                    "v_lshl_add_u32 %[a0], %[a0], %[sh], %[ct]\n\t"
+                    "v_lshl_add_u32 %[a1], %[a1], %[sh], %[ct]\n\t"
+                    "v_lshl_add_u32 %[a2], %[a2], %[sh], %[ct]\n\t"
+                    "v_lshl_add_u32 %[a3], %[a3], %[sh], %[ct]"
                     : [a0] "=&v"(a[0]), [a1] "=&v"(a[1]), [a2] "=&v"(a[2]), [a3] "=&v"(a[3])
                     : [v0] "v"(V1[0]), [v1] "v"(V1[1]), [v2] "v"(V1[2]), [v3] "v"(V1[3]),
-                      [cl] "s"(clamp), [rA] "s"(rA), [rB] "v"(rBv), [ct] "v"(colterm)
+                      [cl] "s"(clamp), [rA] "s"(rA), [rB] "v"(rBv), [ct] "v"(colterm),
+                      [sh] "i"(kRowShift)
                     : "scc");
                 typename Gt::Raw q[4];
 #pragma unroll
@@ -1952,9 +1979,10 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
     if constexpr (NZ == 8 || NZ == 16) {
         if (P->variant == 3) {
             // rows staged per view: the row span of NZ slices plus margins.  Rows
-            // load four at a time, so 21 rows hold a 20-row footprint and the
-            // zero row: 25.9 KB of LDS per workgroup, six workgroups per CU (24
-            // rows fit the same footprints but only five workgroups: 9% slower)
+            // load in multiples of four, so 21 rows hold a 20-row footprint and
+            // the zero row: 25.9 KB of LDS per workgroup with 8-byte records, six
+            // workgroups per CU (24 rows fit the same footprints but only five
+            // workgroups: 9% slower); 15.1 KB with 4-byte records
             constexpr int RMAX = NZ == 8 ? 21 : 40;
 #define ARECON_ZS_(LY)                                                                        \
     do {                                                                                      \
@@ -1971,8 +1999,10 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
                 if (layout == ARECON_FP_ROWDIFF) ARECON_ZS_(ARECON_FP_ROWDIFF);
                 else ARECON_ZS_(ARECON_FP_ROWPAIRS);
             } else {
-                ARECON_REQUIRE(layout == ARECON_FP_QUADS, "bp: variant 3 reads 16-bit quads");
-                ARECON_ZS_(ARECON_FP_QUADS);
+                ARECON_REQUIRE(layout == ARECON_FP_QUADS || layout == ARECON_FP_ROWDIFF,
+                               "bp: variant 3 reads 16-bit quads or row differences");
+                if (layout == ARECON_FP_ROWDIFF) ARECON_ZS_(ARECON_FP_ROWDIFF);
+                else ARECON_ZS_(ARECON_FP_QUADS);
             }
 #undef ARECON_ZS_
             return;
@@ -2004,12 +2034,10 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
                 else if (P->group == 4) ARECON_ZC(4, ARECON_FP_ROWPAIRS);
                 else if constexpr (NZ <= 16) ARECON_ZC(0, ARECON_FP_ROWPAIRS);
             } else if (layout == ARECON_FP_ROWDIFF) {
-                if constexpr (std::is_same<T, float>::value) {   // FP32 only
-                    if (P->group == 1) ARECON_ZC(1, ARECON_FP_ROWDIFF);
-                    else if (P->group == 2) ARECON_ZC(2, ARECON_FP_ROWDIFF);
-                    else if (P->group == 4) ARECON_ZC(4, ARECON_FP_ROWDIFF);
-                    else if constexpr (NZ <= 16) ARECON_ZC(0, ARECON_FP_ROWDIFF);
-                }
+                if (P->group == 1) ARECON_ZC(1, ARECON_FP_ROWDIFF);
+                else if (P->group == 2) ARECON_ZC(2, ARECON_FP_ROWDIFF);
+                else if (P->group == 4) ARECON_ZC(4, ARECON_FP_ROWDIFF);
+                else if constexpr (NZ <= 16) ARECON_ZC(0, ARECON_FP_ROWDIFF);
             } else {
                 if (P->group == 1) ARECON_ZC(1, ARECON_FP_QUADS);
                 else if (P->group == 2) ARECON_ZC(2, ARECON_FP_QUADS);
@@ -2044,8 +2072,6 @@ void bp_plan_launch(const BPPlan* P, float* img, const void* fp, int fp_dtype, i
     ARECON_REQUIRE(fp_layout == ARECON_FP_ROWS || fp_layout == ARECON_FP_ROWPAIRS ||
                        fp_layout == ARECON_FP_QUADS || fp_layout == ARECON_FP_ROWDIFF,
                    "bp: unknown fp layout");
-    ARECON_REQUIRE(fp_layout != ARECON_FP_ROWDIFF || fp_dtype == ARECON_F32,
-                   "bp: the row-difference layout is FP32 only");
     if (block_end < 0) block_end = P->a.nzsets;
     ARECON_REQUIRE(block_begin >= 0 && block_begin < block_end && block_end <= P->a.nzsets,
                    "bp: slice-block range outside the plan");
@@ -2088,8 +2114,6 @@ BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const v
     ARECON_REQUIRE(fp_layout == ARECON_FP_ROWS || fp_layout == ARECON_FP_ROWPAIRS ||
                        fp_layout == ARECON_FP_QUADS || fp_layout == ARECON_FP_ROWDIFF,
                    "bp batch: unknown fp layout");
-    ARECON_REQUIRE(fp_layout != ARECON_FP_ROWDIFF || fp_dtype == ARECON_F32,
-                   "bp batch: the row-difference layout is FP32 only");
     std::vector<SlabDesc> slabs(n);
     long long wg = 0;
     for (int i = 0; i < n; ++i) {
@@ -2159,9 +2183,8 @@ static void batch_launch_t(const BPBatch* B, hipStream_t s) {
                            s, B->d_slabs, B->nslabs);
         break;
     case ARECON_FP_ROWDIFF:
-        if constexpr (std::is_same<T, float>::value)
-            hipLaunchKernelGGL((bp_fdk3dw_zc_batch<NZ, G, T, ARECON_FP_ROWDIFF>), grid, block, 0,
-                               s, B->d_slabs, B->nslabs);
+        hipLaunchKernelGGL((bp_fdk3dw_zc_batch<NZ, G, T, ARECON_FP_ROWDIFF>), grid, block, 0, s,
+                           B->d_slabs, B->nslabs);
         break;
     default:
         hipLaunchKernelGGL((bp_fdk3dw_zc_batch<NZ, G, T, ARECON_FP_QUADS>), grid, block, 0, s,

@@ -612,6 +612,27 @@ template <> struct Quad4<__hip_bfloat16> {
     }
 };
 
+// The two values of a row-difference element, f(r, c) and f(r+1, c) - f(r, c),
+// in one store (8 bytes in FP32, 4 in FP16 / BF16).
+template <class OutT> struct Pair2;
+template <> struct Pair2<float> {
+    __device__ static __forceinline__ void put(float* o, float a, float b) {
+        *reinterpret_cast<float2*>(o) = make_float2(a, b);
+    }
+};
+template <> struct Pair2<__half> {
+    __device__ static __forceinline__ void put(__half* o, float a, float b) {
+        *reinterpret_cast<unsigned*>(o) = (unsigned)__half_as_ushort(__float2half(a)) |
+                                          ((unsigned)__half_as_ushort(__float2half(b)) << 16);
+    }
+};
+template <> struct Pair2<__hip_bfloat16> {
+    __device__ static __forceinline__ void put(__hip_bfloat16* o, float a, float b) {
+        auto bits = [](float v) { return (unsigned)__bfloat16_as_ushort(__float2bfloat16(v)); };
+        *reinterpret_cast<unsigned*>(o) = bits(a) | (bits(b) << 16);
+    }
+};
+
 template <class OutT>
 __global__ __launch_bounds__(256) void expand_quads(OutT* __restrict__ out,
                                                     const float* __restrict__ rows, int YL,
@@ -672,11 +693,10 @@ __global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
                 if (c.has_b) pn[ch] = b0;
                 if constexpr (OUTL == ARECON_FP_ROWDIFF) {
                     if (k > 0)
-                        *reinterpret_cast<float2*>(ov + ((long long)(k - 1) * YL + ch) * 2) =
-                            make_float2(pr[ch], a0 - pr[ch]);
+                        Pair2<OutT>::put(ov + ((long long)(k - 1) * YL + ch) * 2, pr[ch],
+                                         a0 - pr[ch]);
                     if (c.has_b)
-                        *reinterpret_cast<float2*>(ov + ((long long)k * YL + ch) * 2) =
-                            make_float2(a0, b0 - a0);
+                        Pair2<OutT>::put(ov + ((long long)k * YL + ch) * 2, a0, b0 - a0);
                     continue;
                 }
                 if (ch >= nc) continue;
@@ -710,7 +730,8 @@ static int filter_schedule() {
 
 // Row differences from filtered FP32 rows: element (v, r, c) of
 // [ViewN][ZL-1][YL][2] holds f(r,c) and f(r+1,c)-f(r,c).
-__global__ __launch_bounds__(256) void expand_rowdiff(float2* __restrict__ out,
+template <class OutT>
+__global__ __launch_bounds__(256) void expand_rowdiff(OutT* __restrict__ out,
                                                       const float* __restrict__ rows, int YL,
                                                       int ZL, long long n) {
     const int nr = ZL - 1;
@@ -722,7 +743,7 @@ __global__ __launch_bounds__(256) void expand_rowdiff(float2* __restrict__ out,
         const int r = (int)(t - v * nr);
         const float* p = rows + (v * ZL + r) * YL + c;
         const float a = p[0];
-        out[i] = make_float2(a, p[YL] - a);
+        Pair2<OutT>::put(out + 2 * i, a, p[YL] - a);
     }
 }
 
@@ -772,16 +793,21 @@ void launch_filter(void* out, int out_dtype, int out_layout, const float* in, in
     ARECON_REQUIRE(dYL != 0.f && SO > 0.f, "filter: dYL must be nonzero and SO positive");
     ARECON_REQUIRE(out_dtype == ARECON_F32 || out_dtype == ARECON_F16 || out_dtype == ARECON_BF16,
                    "filter: unknown output dtype");
-    ARECON_REQUIRE(quads || out_dtype == ARECON_F32, "filter: the row-difference layout is FP32 only");
     int dev = 0;
     ARECON_HIP(hipGetDevice(&dev));
     const int N = filter_length(YL);
     if (N >= 256 && N <= 2048) {      // fused: filtered rows stay in LDS
         const FilterTables& t = filter_tables(dev, YL, dYL);
         const float* W = preweight_table(dev, YL, ZL, PLC, ZLC, dYL, dZL, SO);
-        if (!quads)
+        if (!quads && out_dtype == ARECON_F32)
             launch_filter_fused<ARECON_FP_ROWDIFF>(static_cast<float*>(out), in, t, W, N, YL, ZL,
                                                    ViewN, PLC, dYL, SO, s);
+        else if (!quads && out_dtype == ARECON_F16)
+            launch_filter_fused<ARECON_FP_ROWDIFF>(static_cast<__half*>(out), in, t, W, N, YL,
+                                                   ZL, ViewN, PLC, dYL, SO, s);
+        else if (!quads)
+            launch_filter_fused<ARECON_FP_ROWDIFF>(static_cast<__hip_bfloat16*>(out), in, t, W,
+                                                   N, YL, ZL, ViewN, PLC, dYL, SO, s);
         else if (out_dtype == ARECON_F32)
             launch_filter_fused<ARECON_FP_QUADS>(static_cast<float*>(out), in, t, W, N, YL, ZL,
                                                  ViewN, PLC, dYL, SO, s);
@@ -804,8 +830,15 @@ void launch_filter(void* out, int out_dtype, int out_layout, const float* in, in
     const long long grid = std::max<long long>(1, std::min<long long>((n + 255) / 256,
                                                                       (long long)cus * 64));
     if (!quads) {
-        hipLaunchKernelGGL(expand_rowdiff, dim3((unsigned)grid), dim3(256), 0, s,
-                           static_cast<float2*>(out), rows, YL, ZL, n);
+        if (out_dtype == ARECON_F32)
+            hipLaunchKernelGGL(expand_rowdiff<float>, dim3((unsigned)grid), dim3(256), 0, s,
+                               static_cast<float*>(out), rows, YL, ZL, n);
+        else if (out_dtype == ARECON_F16)
+            hipLaunchKernelGGL(expand_rowdiff<__half>, dim3((unsigned)grid), dim3(256), 0, s,
+                               static_cast<__half*>(out), rows, YL, ZL, n);
+        else
+            hipLaunchKernelGGL(expand_rowdiff<__hip_bfloat16>, dim3((unsigned)grid), dim3(256),
+                               0, s, static_cast<__hip_bfloat16*>(out), rows, YL, ZL, n);
         ARECON_HIP(hipGetLastError());
         return;
     }

@@ -53,13 +53,14 @@ def fdk_weights(N_2pi: int, HSCoef: float, delta: int) -> np.ndarray:
     return w
 
 
-def choose_bp_kernel(params: BPParams, dz: float | None = None, storage: str = "f32") -> dict:
+def choose_bp_kernel(params: BPParams, dz: float | None = None) -> dict:
     """Backprojector launch shape for a geometry.  Consecutive slices' windows
     are N_2pi dz / h views apart; when a view lies in the windows of at least
     four consecutive slices, the z-column kernels over blocks of 8 slices share
-    the per-(column, view) work among them: row differences for FP32 storage
-    (row pairs with the row difference stored: four instructions per blend),
-    difference quads (same bytes, three FMAs per blend) for 16-bit storage.  The
+    the per-(column, view) work among them, reading row differences (f and
+    f(r+1) - f per channel: four instructions per blend; in FP16 / BF16 half the
+    bytes per channel of difference quads, so the LDS-staged kernel fits eight
+    workgroups per CU instead of six).  The
     LDS-staged one (variant 3) copies each view's footprint to LDS and reads its
     samples there; a workgroup whose footprints do not fit runs the gathering
     z-column body (variant 2).  Otherwise (thick slices, few shared views) the
@@ -68,8 +69,7 @@ def choose_bp_kernel(params: BPParams, dz: float | None = None, storage: str = "
     L = 2 * window_half_width(params.N_2pi, params.HSCoef) + 1
     shift = abs(params.N_2pi * dzv / params.h)
     if shift * 4 <= L:
-        return dict(variant=3, nz=8, group=8, order=2,
-                    layout="rowdiff" if storage == "f32" else "quads")
+        return dict(variant=3, nz=8, group=8, order=2, layout="rowdiff")
     return dict(variant=0, nz=0, group=0, order=0, layout="rows")
 
 
@@ -225,12 +225,12 @@ class HelicalFDK:
         self.params = bp_params(g)
         self.dz = dz
         if variant is None:       # launch shape chosen from the geometry
-            k = choose_bp_kernel(self.params, dz, storage)
+            k = choose_bp_kernel(self.params, dz)
             variant, nz, group = k["variant"], nz or k["nz"], group or k["group"]
             layout = layout or k["layout"]
             order = k["order"] if order is None else order
-        elif variant == 3 and layout is None:   # the LDS-staged kernel's only layouts
-            layout = "rowdiff" if storage == "f32" else "quads"
+        elif variant == 3 and layout is None:   # not rows, which variant 3 cannot read
+            layout = "rowdiff"
         order = 0 if order is None else order
         layout = layout or "rows"
         self.plan = BPPlan(self.params, dz=dz, nz=nz, order=order, variant=variant, group=group)

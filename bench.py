@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--overlap", type=int, default=1,
                     help="view chunks / slice-block groups of the overlapped pass (1: off)")
     ap.add_argument("--layout", default=None,
-                    help="filtered layout: rows, rowpairs, rowdiff (FP32) or quads")
+                    help="filtered layout: rows, rowpairs, rowdiff or quads")
     ap.add_argument("--storages", default="f32,f16",
                     help="filtered-sinogram storages to time (plain f32 first)")
     ap.add_argument("--copy-probe", action="store_true",

@@ -54,11 +54,12 @@ extern "C" {
                                  f(r,c), f(r+1,c)-f(r,c), f(r,c+1)-f(r,c) and
                                  f(r+1,c+1)-f(r+1,c)-f(r,c+1)+f(r,c): the bilinear
                                  sample at (r+fr, c+fc) is then three FMAs */
-#define ARECON_FP_ROWDIFF 3   /* [ViewN][ZL-1][YL][2], FP32 only: element (v, r, c)
-                                 holds f(r,c) and f(r+1,c)-f(r,c), the row pair with
-                                 its difference precomputed: the bilinear sample is
-                                 four instructions instead of six, bit-identical to
-                                 the row-pair blend */
+#define ARECON_FP_ROWDIFF 3   /* [ViewN][ZL-1][YL][2]: element (v, r, c) holds
+                                 f(r,c) and f(r+1,c)-f(r,c), the row pair with its
+                                 difference precomputed: the bilinear sample is
+                                 four instructions instead of six (in FP32
+                                 bit-identical to the row-pair blend); in FP16 /
+                                 BF16 half the bytes per channel of the quads */
 
 /* ========================================================================== */
 /* 1. Reference drop-ins                                                      */
@@ -160,7 +161,7 @@ int arecon_rebin_views_dev(float* d_out, const float* d_in, int YL, int ZL, int 
 
 /* Preweight + ramp filter on device buffers; input layout [ViewN][ZL][YL],
  * output layout out_layout (ARECON_FP_ROWS, ARECON_FP_ROWPAIRS, ARECON_FP_QUADS
- * or, for ARECON_F32 only, ARECON_FP_ROWDIFF).  d_out may equal d_in only for
+ * or ARECON_FP_ROWDIFF).  d_out may equal d_in only for
  * ARECON_F32 rows.  For ARECON_FP_QUADS and ARECON_FP_ROWDIFF with YL <= 1024
  * the output is formed from the filtered rows in LDS and d_in is left unchanged; above that the filter runs in place in
  * d_in (which then holds the FP32 filtered rows) and the output is derived from
@@ -212,8 +213,8 @@ int arecon_bp_plan_nz(const arecon_bp_plan* plan);
 /* Backprojects d_fp (PN views in layout fp_layout, element type fp_dtype) into
  * d_img ([YN][XN][ZN], FP32; every voxel is written).  The row-pair, quad and
  * row-difference layouts need a z-column kernel (variant 2, or the LDS-staged
- * variant 3, which reads FP32 row pairs or row differences, or 16-bit quads);
- * ARECON_FP_ROWDIFF is FP32 only. */
+ * variant 3, which reads FP32 row pairs, row differences of any element type
+ * or 16-bit quads). */
 int arecon_bp_plan_execute(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
                            int fp_dtype, int fp_layout, void* stream);
 

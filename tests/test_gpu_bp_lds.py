@@ -23,7 +23,8 @@ from .test_oracle_fdk import geom as bp_geom_dict
 pytestmark = pytest.mark.gpu
 
 STORAGE_LAYOUT = [(torch.float32, "rowpairs"), (torch.float32, "rowdiff"),
-                  (torch.float16, "quads"), (torch.bfloat16, "quads")]
+                  (torch.float16, "quads"), (torch.bfloat16, "quads"),
+                  (torch.float16, "rowdiff"), (torch.bfloat16, "rowdiff")]
 
 
 def lds_geom(**kw):
@@ -71,14 +72,16 @@ def test_lds_bit_identical_to_z_column(gpu, nz, storage, layout, ZNC):
 
 @pytest.mark.parametrize("order,storage,layout", [
     (0, torch.float32, "rowdiff"), (1, torch.float32, "rowdiff"), (2, torch.float32, "rowdiff"),
-    (0, torch.float32, "rowpairs"), (0, torch.float16, "quads"), (2, torch.bfloat16, "quads")])
+    (0, torch.float32, "rowpairs"), (0, torch.float16, "quads"), (2, torch.bfloat16, "quads"),
+    (0, torch.float16, "rowdiff"), (2, torch.bfloat16, "rowdiff")])
 def test_lds_edges_and_workgroup_orders(gpu, order, storage, layout):
     """Columns projecting onto the detector's first and last channels and rows
     (the geometry is not INSIDE: the grid reaches past the detector's edges),
     every workgroup order (14 tile rows: one full band of 12 rows and a partial
     one for order 2; a partial tile column): bit-identical to the gathering
-    kernel.  Staging moves records in 16-byte pairs; a pair straddling the
-    view's first or last record must not zero it."""
+    kernel.  Staging moves 8-byte records in 16-byte pairs; a pair straddling
+    the view's first or last record must not zero it (4-byte records load one
+    per lane)."""
     print(f"bp lds edges order={order} {storage} {layout}", flush=True)
     g = lds_geom(XN=40, YN=213, YNC=105.6)
     fp = np.random.default_rng(21).normal(size=(200, 16, 128)).astype(np.float32)
@@ -126,7 +129,9 @@ def test_lds_falls_back_when_the_footprint_is_too_wide(gpu):
 @pytest.mark.parametrize("nz,storage,layout", [(8, torch.float32, "rowpairs"),
                                                (8, torch.float32, "rowdiff"),
                                                (8, torch.float16, "quads"),
-                                               (16, torch.float16, "quads")])
+                                               (16, torch.float16, "quads"),
+                                               (8, torch.float16, "rowdiff"),
+                                               (16, torch.bfloat16, "rowdiff")])
 @pytest.mark.parametrize("x0", [0.0, 200.0])
 def test_lds_clinical_geometry(gpu, nz, storage, layout, x0):
     """The clinical scanner (888x64 detector, 984 views/rotation, pitch 0.984,

@@ -154,26 +154,26 @@ def test_filter_quad_layout(gpu, ViewN, ZL, YL, dtype):
 
 @pytest.mark.parametrize("ViewN,ZL,YL", [(9, 64, 888), (5, 5, 130), (3, 2, 1024), (6, 4, 300),
                                          (7, 3, 20), (4, 2, 2048)])
-def test_filter_row_difference_layout(gpu, ViewN, ZL, YL):
-    """Row differences (f(r), f(r+1) - f(r)) in FP32: bit-identical to those formed
-    from the rows output for even ZL (and for the in-place path, YL = 20 and
-    2048); to FP32 rounding of the filter for odd ZL (see the quad test)."""
-    print(f"filter rowdiff ViewN={ViewN} ZL={ZL} YL={YL}", flush=True)
+@pytest.mark.parametrize("dtype", [_lib.ARECON_F32, _lib.ARECON_F16, _lib.ARECON_BF16])
+def test_filter_row_difference_layout(gpu, ViewN, ZL, YL, dtype):
+    """Row differences (f(r), f(r+1) - f(r)), formed in FP32 and rounded once to
+    the output type: bit-identical to those formed from the rows output for
+    even ZL (and for the in-place path, YL = 20 and 2048); to FP32 rounding of
+    the filter for odd ZL (see the quad test)."""
+    print(f"filter rowdiff ViewN={ViewN} ZL={ZL} YL={YL} dtype={dtype}", flush=True)
     P = np.random.default_rng(YL + 3 * ZL).normal(size=(ViewN, ZL, YL)).astype(np.float32)
     args = ((YL - 1) / 2.0, (ZL - 1) / 2.0, 0.9493, 1.9231)
     rows = run_filter(P, *args)
-    got = run_filter(P, *args, layout=_lib.ARECON_FP_ROWDIFF)
+    got = run_filter(P, *args, dtype=dtype, layout=_lib.ARECON_FP_ROWDIFF)
     want = row_diffs(rows)
+    cast = {_lib.ARECON_F32: lambda a: a,
+            _lib.ARECON_F16: lambda a: torch.from_numpy(a).half().float().numpy(),
+            _lib.ARECON_BF16: lambda a: torch.from_numpy(a).bfloat16().float().numpy()}[dtype]
     if ZL % 2 == 0 or YL > 1024 or YL < 128:
-        assert np.array_equal(got, want)
+        assert np.array_equal(got, cast(want))
     else:
-        assert rel_l2(got, want) < 1e-6
-
-
-def test_filter_row_difference_is_fp32_only(gpu):
-    P = np.zeros((2, 4, 100), np.float32)
-    with pytest.raises(RuntimeError, match="FP32 only"):
-        run_filter(P, 49.5, 1.5, 0.9, 1.9, dtype=_lib.ARECON_F16, layout=_lib.ARECON_FP_ROWDIFF)
+        tol = {_lib.ARECON_F32: 1e-6, _lib.ARECON_F16: 1e-3, _lib.ARECON_BF16: 8e-3}[dtype]
+        assert rel_l2(got, want) < tol
 
 
 def test_filter_quads_leave_input_unchanged(gpu):
@@ -343,11 +343,22 @@ def test_bp_row_differences_bit_identical_to_row_pairs(gpu, variant, nz, ZNC):
     assert rel_l2(b, want) < 1e-4
 
 
-def test_bp_row_differences_need_fp32(gpu):
+@pytest.mark.parametrize("storage,tol", [(torch.float16, 1e-3), (torch.bfloat16, 8e-3)])
+@pytest.mark.parametrize("variant,group", [(2, 8), (2, 4), (3, 8)])
+def test_bp_16_bit_row_differences(gpu, storage, tol, variant, group):
+    """16-bit row differences: f and f(r+1) - f each rounded once, so the volume
+    stays within the storage precision of the FP64 oracle, from the gathering
+    kernel (split and grouped) and the LDS-staged one (4-byte records)."""
+    print(f"bp 16-bit rowdiff {storage} variant={variant} group={group}", flush=True)
     g = realistic(ZN=37, dz=0.45, ZNC=-15.37)
-    fp = np.zeros((200, 16, 128), np.float32)
-    with pytest.raises(RuntimeError, match="FP32 only"):
-        gpu_bp(fp, g, nz=8, variant=2, group=8, layout="rowdiff", storage=torch.float16)
+    fp = np.random.default_rng(23).normal(size=(200, 16, 128)).astype(np.float32)
+    got, w = gpu_bp(fp, g, storage=storage, variant=variant, nz=8, group=group, layout="rowdiff")
+    want = oracle.fdk_core(fp.astype(np.float64), w,
+                           **{k: v for k, v in g.items() if k not in ("YL", "ZL")})
+    assert not np.isnan(got).any()
+    err = rel_l2(got, want)
+    print(f"16-bit row differences {storage}: rel L2 {err:.2e}")
+    assert err < tol
 
 
 @pytest.mark.parametrize("k1", [0.0, 2.85])
