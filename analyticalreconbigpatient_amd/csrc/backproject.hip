@@ -44,6 +44,7 @@
 #include <climits>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -71,6 +72,12 @@ struct BPArgs {
     unsigned long long* stats;        // LDS-staged kernel: workgroups staged / direct
     int inside;                       // every column well inside the source circle and
                                       // the detector (z-column kernels' INSIDE path)
+    // LDS-staged kernel: the plan's footprint table (per slice block and tile, one
+    // entry per view of the block, padded to a multiple of four), its per-block
+    // offsets and per-(block, tile) flags (1: every footprint fits the slots)
+    const unsigned* zs_tab;
+    const long long* zs_boff;
+    const int* zs_fits;
 };
 
 // Logical workgroup lb -> slice block zs and 16x16-column tile (txb, tyb).
@@ -1105,7 +1112,6 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
 // four views.
 // ---------------------------------------------------------------------------
 constexpr int kZsCW = 32;                    // records per staged row
-constexpr int kZsMaxViews = 1024;            // views per block in the footprint table
 enum { ZS_SKIP = 0, ZS_LDS = 1, ZS_DIRECT = 2 };
 
 typedef __attribute__((address_space(3))) char lds_char;
@@ -1211,6 +1217,54 @@ __device__ __forceinline__ unsigned zs_pack(const ZsView& v) {
            ((unsigned)(v.rB + 1) << 24);
 }
 
+// Views of slice block zs in the footprint table: pmax - pmin + 1 rounded up to a
+// multiple of four, so every (block, tile) row starts 16-byte aligned.
+__host__ __device__ __forceinline__ int zs_table_views(int pmin, int pmax) {
+    return (pmax - pmin + 4) & ~3;
+}
+
+// The footprint table of the LDS-staged kernel, built once per plan and record
+// layout: workgroup (zs, tile) writes one entry per view of block zs (zs_pack;
+// padding entries are skipped views) and whether every footprint fits the
+// slots.  nrec: records per detector row; PAIRS: 8-byte records staged in
+// 16-byte pairs.
+template <int ZB, int RMAX, bool INSIDE, bool PAIRS>
+__global__ __launch_bounds__(256) void bp_zs_table(BPArgs A, const float2* __restrict__ vtab,
+                                                   const ZBlock* __restrict__ blocks,
+                                                   const uint2* __restrict__ masks,
+                                                   const float2* __restrict__ table,
+                                                   unsigned* __restrict__ tab,
+                                                   const long long* __restrict__ boff,
+                                                   int* __restrict__ fits, int nrec) {
+    const int b = blockIdx.x;
+    const int zs = b / A.ntiles, tile = b - zs * A.ntiles;
+    const int tyb = tile / A.tilesx, txb = tile - tyb * A.tilesx;
+    const ZBlock hb = blocks[zs];
+    const int pmin = hb.pmin, pmax = hb.pmax;
+    const unsigned* mrow = reinterpret_cast<const unsigned*>(masks + (hb.off / ZB - pmin));
+    const float* trow = reinterpret_cast<const float*>(table + (hb.off - (long long)pmin * ZB));
+    const float xc = ((float)(txb * 16) + 7.5f - A.XNC) * A.dx;
+    const float yc = ((float)(tyb * 16) + 7.5f - A.YNC) * A.dy;
+    const float ex = 7.5f * fabsf(A.dx), ey = 7.5f * fabsf(A.dy);
+    const int nvp = zs_table_views(pmin, pmax);
+    unsigned* out = tab + boff[zs] + (long long)tile * nvp;
+    int ok = 1;
+    for (int i = threadIdx.x; i < nvp; i += 256) {
+        const int p = pmin + i;
+        unsigned e = 1u << 16;                                 // a skipped view
+        if (p <= pmax) {
+            const ZsView v = zs_footprint<RMAX, INSIDE, PAIRS>(
+                A, vtab[p], mrow[2LL * p], const_ptr(trow + 2LL * p * ZB), xc, yc, ex, ey,
+                nrec);
+            if (v.mode == ZS_DIRECT) ok = 0;
+            e = zs_pack(v);
+        }
+        out[i] = e;
+    }
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0) fits[b] = ok;
+}
+
 template <int ZB, class T, int LAYOUT, int RMAX, bool INSIDE>
 __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restrict__ img,
                                        const T* __restrict__ fp,
@@ -1226,7 +1280,6 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     static_assert(kRowB == 1 << kRowShift && kRec == 1 << kRecShift, "shifts");
     constexpr int kSlot = RMAX * kRowB;
     __shared__ __attribute__((aligned(16))) char stage0[4 * kSlot];
-    __shared__ __attribute__((aligned(16))) unsigned fpt[kZsMaxViews];
     typedef Gather<T, LAYOUT> Gt;
     typedef LdsGather<T, LAYOUT> Lg;
 
@@ -1242,39 +1295,31 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     const cptr<float> trow =
         (cptr<float>)const_ptr(uniform_ptr(table) + (hoff - (long long)pmin * ZB));
     const cptr<float> vt = (cptr<float>)const_ptr(uniform_ptr(vtab));
-    const float xc = ((float)(txb * 16) + 7.5f - A.XNC) * A.dx;
-    const float yc = ((float)(tyb * 16) + 7.5f - A.YNC) * A.dy;
-    const float ex = 7.5f * fabsf(A.dx), ey = 7.5f * fabsf(A.dy);
 
-    // Pass 1: every view's footprint into the table.  A workgroup with a view
-    // that does not fit runs the z-column kernel's body instead.  (Computing the
-    // footprints where the views are staged instead frees the table's LDS, but
-    // costs about two VALU per update.)
-    const int nrec = LAYOUT == ARECON_FP_QUADS ? A.YL - 1 : A.YL;   // records per row
-    int fits = pmax - pmin + 1 <= kZsMaxViews;
-    if (fits)
-        for (int p = pmin + (int)threadIdx.x; p <= pmax; p += 256) {
-            const ZsView v = zs_footprint<RMAX, INSIDE, kRec == 8>(A, cld_f2(vt, p), mrow[2LL * p],
-                                                        trow + 2LL * p * ZB, xc, yc, ex, ey,
-                                                        nrec);
-            if (v.mode == ZS_DIRECT) fits = 0;
-            fpt[p - pmin] = zs_pack(v);
-        }
-    // the last row of every slot is zero and stays zero (a view fits only when
-    // its loaded rows, a multiple of four, are fewer than RMAX): rows past the
-    // view's last staged row (slices outside the view's window, temp2 = 1e30,
-    // and rows off the detector's end) clamp onto it and add exactly zero, and
-    // lanes off the detector read it
-    for (int i = threadIdx.x; i < 4 * kRowB / 4; i += 256) {
-        const int slot = i / (kRowB / 4), w = i - slot * (kRowB / 4);
-        reinterpret_cast<__attribute__((address_space(3))) unsigned*>(
-            (lds_char*)stage0 + slot * kSlot + (RMAX - 1) * kRowB)[w] = 0u;
-    }
-    const int staged = __syncthreads_and(fits);
+    // Every view's footprint comes from the plan's table (bp_zs_table, built once
+    // per plan and layout; kept in LDS it cost the sixth or seventh workgroup per
+    // CU, and recomputed here about two VALU per update).  A workgroup with a
+    // view that does not fit runs the z-column kernel's body instead.
+    const int tile = tyb * A.tilesx + txb;
+    const cptr<unsigned> ftab = (cptr<unsigned>)const_ptr(
+        uniform_ptr(A.zs_tab) + *(cptr<long long>)const_ptr(uniform_ptr(A.zs_boff) + zs) +
+        (long long)tile * zs_table_views(pmin, pmax));
+    const int staged = *(cptr<int>)const_ptr(uniform_ptr(A.zs_fits) + (zs * A.ntiles + tile));
     if (threadIdx.x == 0 && A.stats) atomicAdd(A.stats + (staged ? 0 : 1), 1ull);
     if (!staged) {
         zc_run<ZB, 0, T, LAYOUT, INSIDE>(lb0, A, img, fp, vtab, blocks, masks, table);
         return;
+    }
+    // the last row of every slot is zero and stays zero (a view fits only when
+    // its loaded rows, a multiple of four, are fewer than RMAX): rows past the
+    // view's last staged row (slices outside the view's window, temp2 = 1e30,
+    // and rows off the detector's end) clamp onto it and add exactly zero, and
+    // lanes off the detector read it.  The first chunk's barrier orders these
+    // writes before any read.
+    for (int i = threadIdx.x; i < 4 * kRowB / 4; i += 256) {
+        const int slot = i / (kRowB / 4), w = i - slot * (kRowB / 4);
+        reinterpret_cast<__attribute__((address_space(3))) unsigned*>(
+            (lds_char*)stage0 + slot * kSlot + (RMAX - 1) * kRowB)[w] = 0u;
     }
 
     fp = uniform_ptr(fp);
@@ -1304,7 +1349,7 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     auto stage_chunk = [&](int p0) {
         const int p = p0 + wave;
         if (p <= pmax) {
-            const unsigned e = __builtin_amdgcn_readfirstlane(fpt[p - pmin]);
+            const unsigned e = ftab[p - pmin];
             const int rA = (int)((e >> 16) & 0xffu) - 1, rB = (int)(e >> 24) - 1;
             const int c0 = (int)(short)(e & 0xffffu);
             if (rB >= rA) {
@@ -1337,14 +1382,12 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             }
         }
     };
-    // consume the chunk of views p0 .. p0+3; their four table entries in one LDS
-    // read
+    // consume the chunk of views p0 .. p0+3; their four table entries in one
+    // scalar load
     auto consume_chunk = [&](int p0) {
-        const uint4 e4 = *reinterpret_cast<const uint4*>(fpt + (p0 - pmin));
-        const unsigned ech[4] = {(unsigned)__builtin_amdgcn_readfirstlane(e4.x),
-                                 (unsigned)__builtin_amdgcn_readfirstlane(e4.y),
-                                 (unsigned)__builtin_amdgcn_readfirstlane(e4.z),
-                                 (unsigned)__builtin_amdgcn_readfirstlane(e4.w)};
+        typedef unsigned u32x4_v __attribute__((ext_vector_type(4)));
+        const u32x4_v e4 = *(cptr<u32x4_v>)(ftab + (p0 - pmin));
+        const unsigned ech[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int p = p0 + s;
@@ -1633,6 +1676,16 @@ struct BPPlan {
     int group = 1;      // z-column kernel: slices per branch-free group
     int grid = 0;
     std::vector<int> blk_pmin, blk_pmax;   // per slice set: union of the windows
+    // LDS-staged kernel: footprint tables, built at the first launch of each
+    // record kind (8-byte records over YL channels, quads over YL-1, 4-byte
+    // records over YL)
+    struct ZsTable {
+        unsigned* tab = nullptr;
+        long long* boff = nullptr;
+        int* fits = nullptr;
+    };
+    ZsTable zs_tables[3];
+    std::mutex zs_mu;
 };
 
 // One slice's entry for one view, FP64 on the host: temp2 = z - h V / 2pi (:79) and
@@ -1937,6 +1990,11 @@ void bp_plan_destroy(BPPlan* P) {
     if (P->zblocks) (void)hipFree(P->zblocks);
     if (P->zmasks) (void)hipFree(P->zmasks);
     if (P->stats) (void)hipFree(P->stats);
+    for (BPPlan::ZsTable& z : P->zs_tables) {
+        if (z.tab) (void)hipFree(z.tab);
+        if (z.boff) (void)hipFree(z.boff);
+        if (z.fits) (void)hipFree(z.fits);
+    }
     (void)hipSetDevice(cur);
     delete P;
 }
@@ -1957,6 +2015,69 @@ static BPArgs launch_args(const BPPlan* P, const float* img, int layout, size_t 
     if (layout == ARECON_FP_QUADS) a.view_stride = 4LL * (a.ZL - 1) * (a.YL - 1);
     a.view_bytes = (unsigned)(a.view_stride * (long long)elem);
     return a;
+}
+
+// The plan's footprint table for the LDS-staged kernel reading layout LY, built
+// on stream s at its first use (allocates, then synchronizes s once, so that a
+// launch on another stream never reads a table still being written).
+template <int NZ, class T, int LY, int RMAX>
+static const BPPlan::ZsTable& zs_table(const BPPlan* P, hipStream_t s) {
+    constexpr int kRec = (int)Gather<T, LY>::kElem;
+    const int kind = kRec == 4 ? 2 : (LY == ARECON_FP_QUADS ? 1 : 0);
+    BPPlan* M = const_cast<BPPlan*>(P);
+    std::lock_guard<std::mutex> lock(M->zs_mu);
+    BPPlan::ZsTable& z = M->zs_tables[kind];
+    if (z.tab) return z;
+    const BPArgs& A = P->a;
+    std::vector<long long> boff(A.nzsets);
+    long long total = 0;
+    for (int zs = 0; zs < A.nzsets; ++zs) {
+        boff[zs] = total;
+        total += (long long)A.ntiles * zs_table_views(P->blk_pmin[zs], P->blk_pmax[zs]);
+    }
+    BPPlan::ZsTable t;
+    try {
+        ARECON_HIP(hipMalloc(&t.boff, sizeof(long long) * A.nzsets));
+        ARECON_HIP(hipMalloc(&t.fits, sizeof(int) * (size_t)A.nzsets * A.ntiles));
+        ARECON_HIP(hipMalloc(&t.tab, sizeof(unsigned) * (size_t)std::max<long long>(total, 4)));
+        ARECON_HIP(hipMemcpy(t.boff, boff.data(), sizeof(long long) * A.nzsets,
+                             hipMemcpyHostToDevice));
+        const int nrec = LY == ARECON_FP_QUADS ? A.YL - 1 : A.YL;
+        const dim3 grid((unsigned)(A.nzsets * A.ntiles)), block(256);
+        if (A.inside)
+            hipLaunchKernelGGL((bp_zs_table<NZ, RMAX, true, kRec == 8>), grid, block, 0, s, A,
+                               P->vtab, P->zblocks, P->zmasks, P->vt_table, t.tab, t.boff,
+                               t.fits, nrec);
+        else
+            hipLaunchKernelGGL((bp_zs_table<NZ, RMAX, false, kRec == 8>), grid, block, 0, s, A,
+                               P->vtab, P->zblocks, P->zmasks, P->vt_table, t.tab, t.boff,
+                               t.fits, nrec);
+        ARECON_HIP(hipGetLastError());
+        ARECON_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        if (t.tab) (void)hipFree(t.tab);
+        if (t.boff) (void)hipFree(t.boff);
+        if (t.fits) (void)hipFree(t.fits);
+        throw;
+    }
+    z = t;
+    return z;
+}
+
+// One launch of the LDS-staged kernel for layout LY.
+template <int NZ, class T, int LY, int RMAX>
+static void launch_zs(const BPPlan* P, BPArgs a, float* img, const T* fp, hipStream_t s,
+                      int grid) {
+    const BPPlan::ZsTable& t = zs_table<NZ, T, LY, RMAX>(P, s);
+    a.zs_tab = t.tab;
+    a.zs_boff = t.boff;
+    a.zs_fits = t.fits;
+    if (a.inside)
+        hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, true>), dim3(grid), dim3(256), 0, s,
+                           img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a);
+    else
+        hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, false>), dim3(grid), dim3(256), 0, s,
+                           img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a);
 }
 
 template <int NZ, class T>
@@ -1984,15 +2105,7 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
             // workgroups per CU (24 rows fit the same footprints but only five
             // workgroups: 9% slower); 15.1 KB with 4-byte records
             constexpr int RMAX = NZ == 8 ? 21 : 40;
-#define ARECON_ZS_(LY)                                                                        \
-    do {                                                                                      \
-        if (a.inside)                                                                         \
-            hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, true>), dim3(grid), dim3(256), \
-                               0, s, img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a); \
-        else                                                                                  \
-            hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, false>), dim3(grid), dim3(256), \
-                               0, s, img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a); \
-    } while (0)
+#define ARECON_ZS_(LY) launch_zs<NZ, T, LY, RMAX>(P, a, img, fp, s, grid)
             if constexpr (std::is_same<T, float>::value) {
                 ARECON_REQUIRE(layout == ARECON_FP_ROWPAIRS || layout == ARECON_FP_ROWDIFF,
                                "bp: variant 3 reads FP32 row pairs or row differences");
