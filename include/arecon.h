@@ -214,7 +214,10 @@ int arecon_bp_plan_nz(const arecon_bp_plan* plan);
  * d_img ([YN][XN][ZN], FP32; every voxel is written).  The row-pair, quad and
  * row-difference layouts need a z-column kernel (variant 2, or the LDS-staged
  * variant 3, which reads FP32 row pairs, row differences of any element type
- * or 16-bit quads). */
+ * or 16-bit quads).  Variant 3 builds its footprint table (device memory owned
+ * by the plan, about 4 bytes per view of each slice block and 16x16-column
+ * tile) at the first execute with each record layout, and synchronizes the
+ * stream once then: run one execute before capturing it in a graph. */
 int arecon_bp_plan_execute(const arecon_bp_plan* plan, float* d_img, const void* d_fp,
                            int fp_dtype, int fp_layout, void* stream);
 
