@@ -53,23 +53,24 @@ def fdk_weights(N_2pi: int, HSCoef: float, delta: int) -> np.ndarray:
     return w
 
 
-def choose_bp_kernel(params: BPParams, dz: float | None = None) -> dict:
+def choose_bp_kernel(params: BPParams, dz: float | None = None, storage: str = "f32") -> dict:
     """Backprojector launch shape for a geometry.  Consecutive slices' windows
     are N_2pi dz / h views apart; when a view lies in the windows of at least
-    four consecutive slices, the z-column kernels over blocks of 8 slices share
-    the per-(column, view) work among them, reading row differences (f and
-    f(r+1) - f per channel: four instructions per blend; in FP16 / BF16 half the
-    bytes per channel of difference quads, so the LDS-staged kernel fits eight
-    workgroups per CU instead of six).  The
-    LDS-staged one (variant 3) copies each view's footprint to LDS and reads its
-    samples there; a workgroup whose footprints do not fit runs the gathering
-    z-column body (variant 2).  Otherwise (thick slices, few shared views) the
-    view-table kernel runs on the rows layout."""
+    four consecutive slices, the LDS-staged z-column kernel (variant 3) shares
+    the per-(column, view) work among a block of slices and reads each view's
+    footprint from LDS, as row differences (f and f(r+1) - f per channel: four
+    instructions per blend); a workgroup whose footprints do not fit runs the
+    gathering z-column body (variant 2).  Blocks of 8 slices for FP32; in FP16 /
+    BF16 a row difference takes 4 bytes, so 16-slice blocks (twice the slices
+    per view's work) still leave seven workgroups per CU, where FP32's would
+    leave three.  Otherwise (thick slices, few shared views) the view-table
+    kernel runs on the rows layout."""
     dzv = params.dz if dz is None else dz
     L = 2 * window_half_width(params.N_2pi, params.HSCoef) + 1
     shift = abs(params.N_2pi * dzv / params.h)
     if shift * 4 <= L:
-        return dict(variant=3, nz=8, group=8, order=2, layout="rowdiff")
+        return dict(variant=3, nz=8 if storage == "f32" else 16, group=8, order=2,
+                    layout="rowdiff")
     return dict(variant=0, nz=0, group=0, order=0, layout="rows")
 
 
@@ -225,7 +226,7 @@ class HelicalFDK:
         self.params = bp_params(g)
         self.dz = dz
         if variant is None:       # launch shape chosen from the geometry
-            k = choose_bp_kernel(self.params, dz)
+            k = choose_bp_kernel(self.params, dz, storage)
             variant, nz, group = k["variant"], nz or k["nz"], group or k["group"]
             layout = layout or k["layout"]
             order = k["order"] if order is None else order
