@@ -1101,10 +1101,12 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
 //    first and last active slices' temp2 bound temp2 (affine in the slice).
 //    Record rows [rA, rB] are staged, clipped to [-1, ZL-1]; rows -1 and ZL-1
 //    are off the detector and load as zeros (out-of-range buffer reads).
-//  * a view whose footprint does not fit (rows > RMAX, channels > CW, or a tile
-//    reaching the source circle) is read straight from global memory as in the
-//    z-column kernel; a view whose rows all lie off the detector is skipped
-//    (every sample it would read is zero).
+//  * a workgroup with a view whose footprint does not fit (rows > RMAX,
+//    channels > CW, or a tile reaching the source circle) runs the z-column
+//    kernel's body, reading its samples straight from global memory; a view
+//    whose rows all lie off the detector is skipped (every sample it would
+//    read is zero).  The footprints come from a table the plan builds once
+//    (bp_zs_table).
 // Records are 8 bytes (FP32 row pairs or row differences, 16-bit difference
 // quads; a staged row of CW records is 256 B) or 4 bytes (16-bit row
 // differences; 128 B rows, half the LDS per view, so more workgroups per CU).
