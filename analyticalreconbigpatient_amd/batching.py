@@ -57,6 +57,7 @@ class Shape:
     order: int = 0       # 0: slice-block-major workgroups, 1: column-tile-major,
                          # 2: slice-block-major, tiles in bands of 6 rows
     slab_order: str = "given"   # or "largest_first"
+    variant: int = 2     # 2: gathering z-column kernel; 3: LDS-staged (nz 8 or 16)
 
 
 def full_window_range(p: BPParams) -> tuple[float, float]:
@@ -130,7 +131,7 @@ class BatchedRecon:
             pl.close()
         self.shape = shape
         self.plans = [BPPlan(slab_params(self.params[sl.scan], sl), dz=sl.dz, nz=shape.nz,
-                             order=shape.order, variant=2, group=shape.group)
+                             order=shape.order, variant=shape.variant, group=shape.group)
                       for sl in self.slabs]
         idx = list(range(len(self.slabs)))
         if shape.slab_order == "largest_first":
@@ -232,7 +233,8 @@ def _slab_updates(p: BPParams, slab: Slab) -> int:
 
 DEFAULT_CANDIDATES = [Shape(8, 8, 0, "given"), Shape(8, 8, 0, "largest_first"),
                       Shape(8, 8, 1, "given"), Shape(8, 8, 2, "given"), Shape(8, 4, 0, "given"),
-                      Shape(16, 4, 0, "given"), Shape(8, 2, 0, "given")]
+                      Shape(16, 4, 0, "given"), Shape(8, 2, 0, "given"),
+                      Shape(8, 8, 2, "given", 3), Shape(16, 8, 2, "given", 3)]
 
 
 def autotune(recon: BatchedRecon, candidates=DEFAULT_CANDIDATES, cache: str | os.PathLike | None =

@@ -1559,6 +1559,21 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc_batch(const SlabDesc* __rest
                              d.blocks, d.masks, d.table);
 }
 
+// The LDS-staged kernel over a batch of slabs: each slab's descriptor carries
+// its plan's footprint table (built when the batch was created).  INSIDE when
+// every plan of the batch is: the slabs then run exactly the code of their
+// single launches (bit-identical volumes).
+template <int ZB, class T, int LAYOUT, int RMAX, bool INSIDE>
+__global__ __launch_bounds__(256) void bp_fdk3dw_zs_batch(const SlabDesc* __restrict__ slabs,
+                                                          int nslabs) {
+    const int lb = xcd_logical_block();
+    int s = 0;
+    while (s + 1 < nslabs && lb >= slabs[s + 1].wg_begin) ++s;
+    const SlabDesc& d = slabs[s];
+    zs_run<ZB, T, LAYOUT, RMAX, INSIDE>(lb - d.wg_begin, d.A, d.img, static_cast<const T*>(d.fp),
+                                        d.vtab, d.blocks, d.masks, d.table);
+}
+
 // FP64 path: the same math as the reference, one thread per voxel, in double.
 __global__ __launch_bounds__(256) void bp_fdk3dw_f64(double* __restrict__ img,
                                                      const double* __restrict__ fp,
@@ -2215,11 +2230,36 @@ void bp_plan_launch(const BPPlan* P, float* img, const void* fp, int fp_dtype, i
 // ---------------------------------------------------------------------------
 struct BPBatch {
     int device = 0;
-    int nz = 0, group = 0, dtype = 0, layout = 0;
+    int variant = 2, nz = 0, group = 0, dtype = 0, layout = 0;
+    int inside = 0;     // every plan INSIDE (the LDS-staged batch kernel's specialisation)
     int nslabs = 0;
     long long grid = 0;
     SlabDesc* d_slabs = nullptr;
 };
+
+// A plan's footprint table for the LDS-staged batch kernel (built on the null
+// stream, which it synchronizes); the row-pair / row-difference / quad layouts
+// of the single launches.
+template <int NZ, class T>
+static const BPPlan::ZsTable& batch_zs_table_t(const BPPlan* P, int layout) {
+    constexpr int RMAX = NZ == 8 ? 21 : 40;
+    if (layout == ARECON_FP_ROWDIFF) return zs_table<NZ, T, ARECON_FP_ROWDIFF, RMAX>(P, nullptr);
+    if constexpr (std::is_same<T, float>::value) {
+        ARECON_REQUIRE(layout == ARECON_FP_ROWPAIRS,
+                       "bp batch: variant 3 reads FP32 row pairs or row differences");
+        return zs_table<NZ, T, ARECON_FP_ROWPAIRS, RMAX>(P, nullptr);
+    } else {
+        ARECON_REQUIRE(layout == ARECON_FP_QUADS,
+                       "bp batch: variant 3 reads 16-bit quads or row differences");
+        return zs_table<NZ, T, ARECON_FP_QUADS, RMAX>(P, nullptr);
+    }
+}
+template <int NZ>
+static const BPPlan::ZsTable& batch_zs_table_nz(const BPPlan* P, int dtype, int layout) {
+    if (dtype == ARECON_F32) return batch_zs_table_t<NZ, float>(P, layout);
+    if (dtype == ARECON_F16) return batch_zs_table_t<NZ, __half>(P, layout);
+    return batch_zs_table_t<NZ, __hip_bfloat16>(P, layout);
+}
 
 BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const void* const* fps,
                          int n, int fp_dtype, int fp_layout) {
@@ -2234,12 +2274,31 @@ BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const v
     for (int i = 0; i < n; ++i) {
         const BPPlan* P = plans[i];
         ARECON_REQUIRE(P && imgs[i] && fps[i], "bp batch: null plan or buffer");
-        ARECON_REQUIRE(P->variant == 2, "bp batch: every plan must use the z-column kernel");
+        ARECON_REQUIRE((P->variant == 2 || P->variant == 3) && P->variant == plans[0]->variant,
+                       "bp batch: every plan must use the same z-column kernel (variant 2 or 3)");
         ARECON_REQUIRE(P->nz == plans[0]->nz && P->group == plans[0]->group &&
                            P->device == plans[0]->device,
                        "bp batch: plans must share nz, group and device");
         SlabDesc d{};
         d.A = launch_args(P, imgs[i], fp_layout, dtype_size(fp_dtype));
+        if (P->variant == 3) {
+            ARECON_REQUIRE(P->nz == 8 || P->nz == 16, "bp batch: variant 3 needs nz 8 or 16");
+            int cur = 0;
+            ARECON_HIP(hipGetDevice(&cur));
+            ARECON_HIP(hipSetDevice(P->device));
+            const BPPlan::ZsTable* t = nullptr;
+            try {
+                t = P->nz == 8 ? &batch_zs_table_nz<8>(P, fp_dtype, fp_layout)
+                               : &batch_zs_table_nz<16>(P, fp_dtype, fp_layout);
+            } catch (...) {
+                (void)hipSetDevice(cur);
+                throw;
+            }
+            ARECON_HIP(hipSetDevice(cur));
+            d.A.zs_tab = t->tab;
+            d.A.zs_boff = t->boff;
+            d.A.zs_fits = t->fits;
+        }
         d.img = imgs[i];
         d.fp = fps[i];
         d.vtab = P->vtab;
@@ -2253,6 +2312,9 @@ BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const v
     }
     BPBatch* B = new BPBatch();
     B->device = plans[0]->device;
+    B->variant = plans[0]->variant;
+    B->inside = 1;
+    for (int i = 0; i < n; ++i) B->inside &= plans[i]->a.inside;
     B->nz = plans[0]->nz;
     B->group = plans[0]->group;
     B->dtype = fp_dtype;
@@ -2330,8 +2392,42 @@ static void batch_launch_nz(const BPBatch* B, hipStream_t s) {
     }
 }
 
+template <int NZ, class T, bool INSIDE>
+static void batch_launch_zs_in(const BPBatch* B, hipStream_t s) {
+    constexpr int RMAX = NZ == 8 ? 21 : 40;
+    const dim3 grid((unsigned)B->grid), block(256);
+    if (B->layout == ARECON_FP_ROWDIFF)
+        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, ARECON_FP_ROWDIFF, RMAX, INSIDE>), grid,
+                           block, 0, s, B->d_slabs, B->nslabs);
+    else if constexpr (std::is_same<T, float>::value)
+        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, ARECON_FP_ROWPAIRS, RMAX, INSIDE>), grid,
+                           block, 0, s, B->d_slabs, B->nslabs);
+    else
+        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, ARECON_FP_QUADS, RMAX, INSIDE>), grid,
+                           block, 0, s, B->d_slabs, B->nslabs);
+}
+template <int NZ, class T>
+static void batch_launch_zs(const BPBatch* B, hipStream_t s) {
+    if (B->inside) batch_launch_zs_in<NZ, T, true>(B, s);
+    else batch_launch_zs_in<NZ, T, false>(B, s);
+}
+template <int NZ>
+static void batch_launch_zs_nz(const BPBatch* B, hipStream_t s) {
+    switch (B->dtype) {
+    case ARECON_F32: batch_launch_zs<NZ, float>(B, s); break;
+    case ARECON_F16: batch_launch_zs<NZ, __half>(B, s); break;
+    default: batch_launch_zs<NZ, __hip_bfloat16>(B, s); break;
+    }
+}
+
 void bp_batch_launch(const BPBatch* B, hipStream_t s) {
     ARECON_REQUIRE(B != nullptr, "bp batch: null batch");
+    if (B->variant == 3) {
+        if (B->nz == 8) batch_launch_zs_nz<8>(B, s);
+        else batch_launch_zs_nz<16>(B, s);
+        ARECON_HIP(hipGetLastError());
+        return;
+    }
     switch (B->nz) {
     case 8: batch_launch_nz<8>(B, s); break;
     case 16: batch_launch_nz<16>(B, s); break;

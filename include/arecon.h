@@ -248,9 +248,11 @@ int arecon_bp_plan_stats(const arecon_bp_plan* plan, unsigned long long* staged,
 
 /* Batched backprojection (no counterpart in the reference, whose MEX
  * backprojects one volume per call): n plans -- z-slabs of one or more scans,
- * all z-column plans (variant 2) with the same nz and group -- bound to their
- * volumes d_imgs[i] and filtered sinograms d_fps[i] (all of fp_dtype and
- * fp_layout).  One execute is one kernel launch over every slab, with no
+ * all z-column plans of one variant (2, gathering, or 3, LDS-staged, nz 8 or
+ * 16) with the same nz and group -- bound to their volumes d_imgs[i] and
+ * filtered sinograms d_fps[i] (all of fp_dtype and fp_layout).  For variant 3
+ * create builds each plan's footprint table for the layout (and synchronizes
+ * the device).  One execute is one kernel launch over every slab, with no
  * allocation or host synchronisation, so it can be captured in a HIP graph.
  * The buffers must outlive the batch. */
 typedef struct arecon_bp_batch arecon_bp_batch;

@@ -35,7 +35,8 @@ def make(storage, shape=B.Shape()):
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 @pytest.mark.parametrize("shape", [B.Shape(), B.Shape(8, 4, 1, "largest_first"),
-                                   B.Shape(16, 4, 0, "given")])
+                                   B.Shape(16, 4, 0, "given"), B.Shape(8, 8, 2, "given", 3),
+                                   B.Shape(16, 8, 2, "largest_first", 3)])
 def test_batch_equals_per_slab_launches(gpu, storage, shape):
     print(f"batch storage={storage} shape={shape}", flush=True)
     r = make(storage, shape)
