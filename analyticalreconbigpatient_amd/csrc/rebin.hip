@@ -193,16 +193,25 @@ __global__ __launch_bounds__(256) void rebin_vyz_s(
 
 // [ViewN][ZL][YL] output.  Block = (view i, channels j0..j0+JB-1).
 // LDS tile[ZL][JB+1] (padded against bank conflicts on the transposed read).
+// One-dimensional grid, channel blocks fastest, remapped so that each XCD (hardware
+// workgroup b runs on XCD b % 8) takes a contiguous run of views: a fan view
+// feeds two neighbouring output views, and with the plain round-robin order
+// those ran on different XCDs, so each L2 fetched it from HBM again.
 template <int JB>
 __global__ __launch_bounds__(256) void rebin_vzy(
     float* __restrict__ out, const float* __restrict__ in,
     const ChanCoef* __restrict__ chan, int YL, int ZL, int ViewN, float DeltaTheta,
-    float DeltaFai, int view_begin, int vec4) {
+    float DeltaFai, int view_begin, int view_count, int vec4) {
     extern __shared__ float tile[];             // ZL * (JB + 1)
     __shared__ Blend sb[JB];
     __shared__ int sbase[JB][4];
-    const int i = view_begin + blockIdx.x;
-    const int j0 = blockIdx.y * JB;
+    const int nb = gridDim.x, hb = blockIdx.x;
+    const int qx = nb >> 3, rx = nb & 7, xcd = hb & 7;
+    const int lb = xcd * qx + min(xcd, rx) + (hb >> 3);
+    const int ncb = (YL + JB - 1) / JB;
+    const int vi = lb / ncb;
+    const int i = view_begin + vi;
+    const int j0 = (lb - vi * ncb) * JB;
     const int nj = min(JB, YL - j0);
     const int tid = threadIdx.x;
     if (tid < nj) {
@@ -294,10 +303,12 @@ void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN, float 
         constexpr int JB = 64;
         const size_t lds = sizeof(float) * (size_t)ZL * (JB + 1);
         ARECON_REQUIRE(lds <= 64 * 1024, "rebin: ZL too large for the transposing kernel");
-        dim3 grid(view_count, (YL + JB - 1) / JB);
+        const long long nblocks = (long long)view_count * ((YL + JB - 1) / JB);
+        ARECON_REQUIRE(nblocks < (1LL << 31), "rebin: too many workgroups");
         const int vec4 = (ZL % 4 == 0) && ((reinterpret_cast<uintptr_t>(in) & 15) == 0);
-        hipLaunchKernelGGL(rebin_vzy<JB>, grid, dim3(256), lds, s, out, in,
-                           chan, YL, ZL, ViewN, DeltaTheta, DeltaFai, view_begin, vec4);
+        hipLaunchKernelGGL(rebin_vzy<JB>, dim3((unsigned)nblocks), dim3(256), lds, s, out, in,
+                           chan, YL, ZL, ViewN, DeltaTheta, DeltaFai, view_begin, view_count,
+                           vec4);
     }
     ARECON_HIP(hipGetLastError());
 }
