@@ -43,7 +43,7 @@ def rebin_args(YL, ViewN, N_2pi=64, YLC0=None):
                 SO=f(SO))
 
 
-@pytest.mark.parametrize("ViewN,YL,ZL", [(120, 70, 8), (50, 33, 5), (2160, 368, 16), (64, 130, 3)])
+@pytest.mark.parametrize("ViewN,YL,ZL", [(120, 70, 8), (50, 33, 5), (2160, 368, 16), (64, 130, 3), (12, 200, 128)])
 @pytest.mark.parametrize("layout", [_lib.ARECON_LAYOUT_VYZ, _lib.ARECON_LAYOUT_VZY])
 def test_rebin_bit_exact(gpu, ViewN, YL, ZL, layout):
     print(f"rebin ViewN={ViewN} YL={YL} ZL={ZL} layout={layout}", flush=True)
