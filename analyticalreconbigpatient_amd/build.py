@@ -135,8 +135,26 @@ def build_mex(verbose: bool = False) -> list[Path]:
     return outs
 
 
+REF_SRC = Path("/root/reference/src")
+REF_RECIPE = ROOT / "oracle" / "build_ref.sh"
+
+
+def build_ref(verbose: bool = False) -> Path | None:
+    """Compile the reference's own CPU functions (test infrastructure; the
+    oracle's bit-for-bit check and the timed CPU baseline) with
+    oracle/build_ref.sh into oracle/_ref/, when the reference sources are
+    present.  Elsewhere the library built beside them is used as it is."""
+    if not REF_SRC.is_dir():
+        return None
+    if verbose:
+        print(f"sh {REF_RECIPE}")
+    _run_parallel([["sh", str(REF_RECIPE)]])
+    return ROOT / "oracle" / "_ref" / "libarecon_ref.so"
+
+
 def build_all(verbose: bool = False) -> None:
     build_oracle(verbose)
+    build_ref(verbose)
     build_library(verbose)
     build_mex(verbose)
 

@@ -21,9 +21,15 @@
  *   oracle_fdk                 src/Parallel_FDK_Helical_3DWeighting.cpp:117-200
  *                              (useGPU == 0 branch)
  *
- * How each function is pinned (the reference ships no tests, golden vectors or
- * fixtures, and its C++ cannot be compiled here because every translation unit
- * includes MATLAB's <mex.h>, which this image lacks):
+ * How each function is pinned:
+ *   reference  bit for bit against the reference's own CPU functions,
+ *              compiled from its source files by build_ref.sh into
+ *              _ref/libarecon_ref.so: ParallelRebinningCBCurve_CPU<float/double>
+ *              and ParallelFDKHelical3DWeightingCPU, on seeded inputs at the
+ *              368x16 and 888x64 geometries, with off-central-ray voxels,
+ *              clipped windows and edge clamps.      tests/test_oracle_ref.py
+ *              (the filter's reference is MATLAB and cuFFT, which cannot run
+ *              here; the weight table lives inside the MEX-side function.)
  *   rebin      closed forms (constant in -> constant out; inputs affine in
  *              (fan view, channel) reproduced at the textbook fan->parallel
  *              coordinates beta = theta + asin(t/SO), gamma = asin(t/SO));
@@ -39,10 +45,9 @@
  *              of unity; exact for any k1 when h = 1 mm because of the +-0.5 mm
  *              conjugate offset); the central-ray (Yr = 0) detector row equals
  *              the textbook magnification SD/(SO+UU).   tests/test_oracle_fdk.py
- *              PARITY UNPINNED: the off-central-ray (Yr != 0) factor
- *              SO/sqrt(SO^2-Yr^2) in the detector row coordinate
- *              (Parallel_FDK_Helical_3DWeighting.cpp:80) has no independent
- *              check; it is restated as written.
+ *              The off-central-ray (Yr != 0) factor SO/sqrt(SO^2-Yr^2) in the
+ *              detector row coordinate (Parallel_FDK_Helical_3DWeighting.cpp:80)
+ *              has no closed form; the reference comparison pins it.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -11,7 +11,9 @@ rejected by at least one of them.
     7  backprojection: G and Gc denominators swapped
     8  backprojection: channel from -Yr
     9  backprojection: row coordinate without the SO/sqrt(SO^2-Yr^2) factor
-       (expected NOT to be caught: that factor is parity-unpinned)
+       (no closed form reaches it; the bit-for-bit comparison with the
+       reference's own compiled CPU function in test_oracle_ref.py rejects it,
+       as it rejects every other mutant)
 """
 import math
 
@@ -78,11 +80,12 @@ def test_channel_mutant_is_rejected(variants):
         assert _fails(F.test_channel_is_parallel_beam_offset, 30.0, -12.0)
 
 
-def test_unpinned_row_factor_is_documented(variants):
+def test_row_factor_mutant_escapes_the_closed_forms(variants):
     """Mutation 9 drops SO/sqrt(SO^2-Yr^2) from the row coordinate.  The
     central-ray and constant-projection pins cannot see it (Yr = 0 there, and a
-    constant projection ignores the row), which is why that factor is listed as
-    parity-unpinned."""
+    constant projection ignores the row); the comparison with the reference's
+    compiled CPU function (test_oracle_ref.py::test_fdk_mutants_differ_from_reference)
+    is what pins that factor."""
     with oracle.using(variants[9]):
         assert not _fails(F.test_central_ray_row_is_textbook_magnification, 120.0, -0.6)
         assert not _fails(F.test_constant_projection_gives_pi_c, 1.0, 5.0)
