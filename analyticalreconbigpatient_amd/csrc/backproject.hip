@@ -1952,6 +1952,8 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         if (P->variant >= 2) build_block_table(P, w, zt);
         if (P->variant == 3) {
             ARECON_REQUIRE(g.ZL <= 250, "bp: variant 3 takes at most 250 detector rows");
+            // the footprint table stores the first channel in 16 signed bits
+            ARECON_REQUIRE(g.YL < 32768, "bp: variant 3 takes at most 32767 detector channels");
             ARECON_HIP(hipMalloc(&P->stats, 2 * sizeof(unsigned long long)));
             ARECON_HIP(hipMemset(P->stats, 0, 2 * sizeof(unsigned long long)));
             A.stats = P->stats;

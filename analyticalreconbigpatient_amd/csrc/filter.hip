@@ -539,7 +539,9 @@ const FilterTables& filter_tables(int device, int YL, float dYL_f) {
 // t = (j - PLC) dYL and b = (k - ZLC) dZL (src/filtering.cu:202-205), for every
 // row k and channel j: [ZL][YL], FP64 on the host, rounded to FP32.  The
 // reference's GPU path evaluates it per sample in FP32; its CPU twin
-// (src/analyticalReconBigPatient.m:109-115) in FP64.
+// (src/analyticalReconBigPatient.m:109-115) in FP64.  Like the twiddle tables,
+// these are kept for the life of the process (captured graphs hold their
+// addresses): 4 bytes per detector sample of each geometry.
 std::mutex g_pw_mu;
 std::map<std::array<uint32_t, 8>, float*> g_pw_cache;
 
@@ -557,15 +559,6 @@ const float* preweight_table(int device, int YL, int ZL, float PLC, float ZLC, f
     std::lock_guard<std::mutex> lk(g_pw_mu);
     auto it = g_pw_cache.find(key);
     if (it != g_pw_cache.end()) return it->second;
-    if (g_pw_cache.size() >= 64) {
-        ARECON_HIP(hipDeviceSynchronize());
-        for (auto& kv : g_pw_cache) {
-            (void)hipSetDevice((int)kv.first[0]);
-            (void)hipFree(kv.second);
-        }
-        g_pw_cache.clear();
-        ARECON_HIP(hipSetDevice(device));
-    }
     std::vector<float> w((size_t)ZL * YL);
     const double so2 = (double)SO * SO;
     for (int k = 0; k < ZL; ++k) {
@@ -747,13 +740,35 @@ __global__ __launch_bounds__(256) void expand_rowdiff(OutT* __restrict__ out,
     }
 }
 
+// The device tables a launch reads: twiddles and spectrum (N entries) and the
+// preweight [ZL][YL] (needed for N <= 2048).  Either the per-device caches of
+// filter_tables() / preweight_table(), or the scratch of
+// arecon_filter_dev_noalloc.
+struct TableSet {
+    const float2* tw = nullptr;
+    const float* H = nullptr;
+    const float* W = nullptr;
+    int N = 0;
+};
+
+static TableSet cached_tables(int dev, int YL, int ZL, float PLC, float ZLC, float dYL,
+                              float dZL, float SO) {
+    const FilterTables& t = filter_tables(dev, YL, dYL);
+    TableSet ts;
+    ts.tw = t.tw;
+    ts.H = t.H;
+    ts.N = t.N;
+    if (t.N >= 256 && t.N <= 2048) ts.W = preweight_table(dev, YL, ZL, PLC, ZLC, dYL, dZL, SO);
+    return ts;
+}
+
 static void launch_filter_rows(void* out, int out_dtype, int out_layout, const float* in,
                                int YL, int ZL, int ViewN, float PLC, float ZLC, float dYL,
-                               float dZL, float SO, hipStream_t s);
+                               float dZL, float SO, hipStream_t s, const TableSet* fixed);
 
 // Launch of the fused per-view filter for layout OUTL.
 template <int OUTL, class OT>
-static void launch_filter_fused(OT* outp, const float* in, const FilterTables& t, const float* W,
+static void launch_filter_fused(OT* outp, const float* in, const TableSet& t, const float* W,
                                 int N, int YL, int ZL, int ViewN, float PLC, float dYL, float SO,
                                 hipStream_t s) {
     const dim3 grid((unsigned)ViewN);
@@ -775,12 +790,14 @@ static void launch_filter_fused(OT* outp, const float* in, const FilterTables& t
     }
 }
 
-void launch_filter(void* out, int out_dtype, int out_layout, const float* in, int YL, int ZL,
-                   int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
-                   hipStream_t s) {
+// fixed: the tables to read (the allocation-free entry point's scratch), or
+// null for the per-device caches.
+static void launch_filter_impl(void* out, int out_dtype, int out_layout, const float* in,
+                               int YL, int ZL, int ViewN, float PLC, float ZLC, float dYL,
+                               float dZL, float SO, hipStream_t s, const TableSet* fixed) {
     if (out_layout != ARECON_FP_QUADS && out_layout != ARECON_FP_ROWDIFF) {
         launch_filter_rows(out, out_dtype, out_layout, in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL,
-                           SO, s);
+                           SO, s, fixed);
         return;
     }
     const bool quads = out_layout == ARECON_FP_QUADS;
@@ -797,8 +814,8 @@ void launch_filter(void* out, int out_dtype, int out_layout, const float* in, in
     ARECON_HIP(hipGetDevice(&dev));
     const int N = filter_length(YL);
     if (N >= 256 && N <= 2048) {      // fused: filtered rows stay in LDS
-        const FilterTables& t = filter_tables(dev, YL, dYL);
-        const float* W = preweight_table(dev, YL, ZL, PLC, ZLC, dYL, dZL, SO);
+        const TableSet t = fixed ? *fixed : cached_tables(dev, YL, ZL, PLC, ZLC, dYL, dZL, SO);
+        const float* W = t.W;
         if (!quads && out_dtype == ARECON_F32)
             launch_filter_fused<ARECON_FP_ROWDIFF>(static_cast<float*>(out), in, t, W, N, YL, ZL,
                                                    ViewN, PLC, dYL, SO, s);
@@ -823,7 +840,7 @@ void launch_filter(void* out, int out_dtype, int out_layout, const float* in, in
     // other transform lengths: filter in place, then derive the layout
     float* rows = const_cast<float*>(in);
     launch_filter_rows(rows, ARECON_F32, ARECON_FP_ROWS, in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL,
-                       SO, s);
+                       SO, s, fixed);
     const long long n = (long long)ViewN * (ZL - 1) * (quads ? YL - 1 : YL);
     int cus = 0;
     ARECON_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -862,7 +879,7 @@ void launch_filter(void* out, int out_dtype, int out_layout, const float* in, in
 
 static void launch_filter_rows(void* out, int out_dtype, int out_layout, const float* in,
                                int YL, int ZL, int ViewN, float PLC, float ZLC, float dYL,
-                               float dZL, float SO, hipStream_t s) {
+                               float dZL, float SO, hipStream_t s, const TableSet* fixed) {
     ARECON_REQUIRE(YL > 0 && ZL > 0 && ViewN > 0, "filter: YL, ZL and ViewN must be positive");
     ARECON_REQUIRE(YL <= 2048, "filter: YL above 2048 is not supported");
     ARECON_REQUIRE(out != nullptr && in != nullptr, "filter: null buffer");
@@ -878,7 +895,7 @@ static void launch_filter_rows(void* out, int out_dtype, int out_layout, const f
     ARECON_REQUIRE(dYL != 0.f && SO > 0.f, "filter: dYL must be nonzero and SO positive");
     int dev = 0;
     ARECON_HIP(hipGetDevice(&dev));
-    const FilterTables& t = filter_tables(dev, YL, dYL);
+    const TableSet t = fixed ? *fixed : cached_tables(dev, YL, ZL, PLC, ZLC, dYL, dZL, SO);
     const int N = t.N;
     const size_t lds = sizeof(float2) * 3 * (size_t)N + sizeof(float) * (size_t)N;
     const long long nrows = (long long)ZL * ViewN;
@@ -899,7 +916,7 @@ static void launch_filter_rows(void* out, int out_dtype, int out_layout, const f
                                   hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
     });
     if (N >= 256 && N <= 2048) {
-        const float* W = preweight_table(dev, YL, ZL, PLC, ZLC, dYL, dZL, SO);
+        const float* W = t.W;
         const long long wgrid = std::min<long long>(npairs, (long long)cus * 16);
         auto go = [&](auto outp) {
             using OT = std::remove_pointer_t<decltype(outp)>;
@@ -946,6 +963,13 @@ static void launch_filter_rows(void* out, int out_dtype, int out_layout, const f
         break;
     }
     ARECON_HIP(hipGetLastError());
+}
+
+void launch_filter(void* out, int out_dtype, int out_layout, const float* in, int YL, int ZL,
+                   int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
+                   hipStream_t s) {
+    launch_filter_impl(out, out_dtype, out_layout, in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL, SO, s,
+                       nullptr);
 }
 
 }  // namespace arecon

@@ -17,9 +17,14 @@
 //   rebin_vyz  output [ViewN][YL][ZL] (reference layout): one thread per
 //              4 consecutive rows of one (view, channel), float4 loads/stores.
 //   rebin_vzy  output [ViewN][ZL][YL] (the filter's layout): one workgroup per
-//              (view, 64-channel block); the ZL-contiguous input vectors are
-//              blended with coalesced loads into an LDS tile and written out
-//              transposed with coalesced stores.
+//              (view, block of JB channels); the ZL-contiguous input vectors
+//              are blended with coalesced loads into an LDS tile [ZL][JB + 1]
+//              and written out transposed with coalesced stores.  JB = 128
+//              (512-byte row stores) while the tile fits 48 KB (ZL <= 95),
+//              else 64 (ZL <= 242, the 64 KB workgroup limit with the static
+//              blend tables).  Workgroups are remapped so each XCD walks a
+//              contiguous run of views (neighbouring output views share fan
+//              views, whose second read then hits the same L2).
 // HBM roofline: 8 B per sample (one read, one write); the 4-tap gather is
 // served by L2 (neighbouring channels share fan views).
 #include "arecon_internal.h"
@@ -74,9 +79,9 @@ std::vector<ChanCoef> channel_table(int YL, float YLC, float dYA, float PLC,
 }
 
 // Device copies of channel tables, keyed by (device, geometry bits).  A table
-// is uploaded once with a blocking copy and kept for the life of the process
-// (the cache is flushed, after a device synchronisation, when it grows past
-// 64 geometries).
+// is uploaded once with a blocking copy and kept for the life of the process:
+// captured HIP graphs and plans hold its address, so no table is ever freed
+// (about 24 bytes per channel and geometry).
 std::mutex g_chan_mu;
 std::map<std::array<uint32_t, 7>, ChanCoef*> g_chan_cache;
 
@@ -95,15 +100,6 @@ const ChanCoef* channel_table_dev(int YL, float YLC, float dYA, float PLC, float
     std::lock_guard<std::mutex> lk(g_chan_mu);
     auto it = g_chan_cache.find(key);
     if (it != g_chan_cache.end()) return it->second;
-    if (g_chan_cache.size() >= 64) {
-        ARECON_HIP(hipDeviceSynchronize());
-        for (auto& kv : g_chan_cache) {
-            (void)hipSetDevice((int)kv.first[0]);
-            (void)hipFree(kv.second);
-        }
-        g_chan_cache.clear();
-        ARECON_HIP(hipSetDevice(dev));
-    }
     const std::vector<ChanCoef> tab = channel_table(YL, YLC, dYA, PLC, DeltaT, SO);
     ChanCoef* d = nullptr;
     ARECON_HIP(hipMalloc(&d, sizeof(ChanCoef) * YL));
@@ -302,9 +298,11 @@ void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN, float 
         }
     } else {
         // 128-channel blocks (512-byte row stores, 1.21 ms against 1.26 ms for
-        // 64 on the 888 x 64 x 11,808 sinogram) while the tile stays within
-        // 48 KB of LDS (ZL <= 95; the blend tables add 6 KB of static LDS),
-        // else 64-channel blocks
+        // 64 on the 888 x 64 x 11,808 sinogram) while the dynamic tile stays
+        // within 48 KB of LDS (ZL <= 95), else 64-channel blocks.  The static
+        // blend tables add JB * 40 bytes (5 KB at 128, 2.5 KB at 64); a
+        // workgroup may hold 64 KB in all, so the 64-channel blocks take
+        // ZL <= 242.
         const int vec4 = (ZL % 4 == 0) && ((reinterpret_cast<uintptr_t>(in) & 15) == 0);
         auto launch = [&](auto jb) {
             constexpr int JB = decltype(jb)::value;
@@ -318,8 +316,10 @@ void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN, float 
         if (sizeof(float) * (size_t)ZL * 129 <= 48 * 1024) {
             launch(std::integral_constant<int, 128>{});
         } else {
-            ARECON_REQUIRE(sizeof(float) * (size_t)ZL * 65 <= 64 * 1024,
-                           "rebin: ZL too large for the transposing kernel");
+            ARECON_REQUIRE(sizeof(float) * (size_t)ZL * 65 + sizeof(Blend) * 64 +
+                                   sizeof(int) * 4 * 64 <= 64 * 1024,
+                           "rebin: ZL above 242 does not fit the transposing kernel's LDS "
+                           "(output layout [ViewN][ZL][YL])");
             launch(std::integral_constant<int, 64>{});
         }
     }

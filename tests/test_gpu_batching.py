@@ -105,3 +105,83 @@ def test_autotune_picks_and_caches(gpu, tmp_path):
     assert best2 == best and timings2 == timings
     assert r.shape == best
     r.close()
+
+
+def test_replay_after_many_table_geometries(gpu):
+    """A captured graph holds the addresses of the rebinning channel table and
+    the filter's twiddle and preweight tables.  Rebinning and filtering 65 other
+    geometries after the capture (more than the old cache bound of 64) must not
+    free them: the replay still equals the eager pass."""
+    import ctypes as C
+    from analyticalreconbigpatient_amd import _lib
+    r = make("f16")
+    r.run()
+    torch.cuda.synchronize()
+    ref = [v.clone() for v in r.volumes]
+    r.capture()
+    lib = _lib.lib()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    YL, ZL, ViewN = 130, 4, 8        # 256-point transforms: the fused filter with both tables
+    src = torch.randn(ViewN, YL, ZL, device="cuda")
+    reb = torch.empty(ViewN, ZL, YL, device="cuda")
+    out = torch.empty(ViewN, ZL - 1, YL, 2, dtype=torch.float16, device="cuda")
+    for g in range(65):
+        f32 = np.float32
+        _lib.check(lib.arecon_rebin_dev(
+            C.c_void_p(reb.data_ptr()), C.c_void_p(src.data_ptr()), YL, ZL, ViewN,
+            f32(64.5), f32(2.2e-3 * (1 + g / 100.0)), f32(0.1), f32(64.5), f32(1.1), f32(0.1),
+            f32(541.0), _lib.ARECON_LAYOUT_VZY, stream), "rebin")
+        _lib.check(lib.arecon_filter_dev(
+            C.c_void_p(out.data_ptr()), _lib.ARECON_F16, _lib.ARECON_FP_ROWDIFF,
+            C.c_void_p(reb.data_ptr()), YL, ZL, ViewN, f32(64.5), f32(1.5),
+            f32(0.9 + g / 1000.0), f32(1.9), f32(541.0), stream), "filter")
+    torch.cuda.synchronize()
+    for v in r.volumes:
+        v.fill_(float("nan"))
+    r.replay()
+    torch.cuda.synchronize()
+    for a, b in zip(r.volumes, ref):
+        assert torch.equal(a, b)
+    r.close()
+
+
+def test_lds_batch_mixing_inside_and_edge_slabs(gpu, nthreads):
+    """A variant-3 batch whose slabs are not all INSIDE runs the general kernel
+    for every slab, also over the INSIDE plans' footprint tables (built with
+    INSIDE = true: first channel not rounded to even, no straddle check).  One
+    scan's grid lies well inside the fan, the other's reaches past the
+    detector's edge channels; every slab must match the oracle chain."""
+    wls = [G.tiny(YL=96, ZL=12, N_2pi=120, rotations=3, xn=32, zn=8, objR=60.0),
+           G.tiny(YL=96, ZL=12, N_2pi=120, rotations=3, xn=32, zn=8, objR=118.0)]
+    setups = [w.setup() for w in wls]
+    params = [G.bp_params(G.bp_scan_geom(s)) for s in setups]
+    slabs = B.stack_slabs(params[0], [8, 5], 1.0, 0) + B.stack_slabs(params[1], [8, 3], 1.0, 1)
+    raws = []
+    for w, s, p in zip(wls, setups, params):
+        lo, hi = B.full_window_range(p)
+        E = phantom.shepp_logan(90.0, 0.45 * (hi - lo), 0.5 * (lo + hi))
+        raws.append(phantom.project(E, s, w.hdr))
+    torch.cuda.synchronize()
+    for nz in (8, 16):
+        r = B.BatchedRecon(setups, raws, slabs, storage="f32",
+                           shape=B.Shape(nz, 8, 2, "given", 3))
+        inside = [pl.inside for pl in r.plans]
+        print(f"mixed INSIDE batch nz={nz}: inside={inside}", flush=True)
+        assert any(inside) and not all(inside)
+        r.run()
+        torch.cuda.synchronize()
+        for sl, vol in zip(r.slabs, r.volumes):
+            s = r.setups[sl.scan]
+            a = G.rebin_args(s.geom)
+            raw = r.raws[sl.scan].cpu().numpy()
+            reb = oracle.rebin_f32(raw, a.YLC0, a.dYA, a.DeltaTheta, a.PLC0, a.DeltaT,
+                                   a.DeltaFai, a.SO)
+            fa = G.filter_args(s)
+            filt = oracle.filter_f64(np.ascontiguousarray(reb.transpose(0, 2, 1)),
+                                     float(fa.PLC0), float(fa.ZLC0), float(fa.dYL),
+                                     float(fa.dZL), float(fa.SO), nthreads=nthreads)
+            want = oracle.fdk_from_params(filt, B.slab_params(r.params[sl.scan], sl), dz=sl.dz,
+                                          nthreads=nthreads)
+            assert np.abs(want).max() > 0
+            assert rel_l2(vol.cpu().numpy(), want) < 1e-4, sl
+        r.close()

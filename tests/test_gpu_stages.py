@@ -43,7 +43,11 @@ def rebin_args(YL, ViewN, N_2pi=64, YLC0=None):
                 SO=f(SO))
 
 
-@pytest.mark.parametrize("ViewN,YL,ZL", [(120, 70, 8), (50, 33, 5), (2160, 368, 16), (64, 130, 3), (12, 200, 128)])
+# ZL 95 / 96: the switch from 128- to 64-channel blocks of the transposing
+# kernel; ZL 242: its largest (64 KB of LDS with the static blend tables)
+@pytest.mark.parametrize("ViewN,YL,ZL", [(120, 70, 8), (50, 33, 5), (2160, 368, 16), (64, 130, 3),
+                                         (12, 200, 128), (10, 150, 95), (10, 150, 96),
+                                         (6, 140, 242)])
 @pytest.mark.parametrize("layout", [_lib.ARECON_LAYOUT_VYZ, _lib.ARECON_LAYOUT_VZY])
 def test_rebin_bit_exact(gpu, ViewN, YL, ZL, layout):
     print(f"rebin ViewN={ViewN} YL={YL} ZL={ZL} layout={layout}", flush=True)
@@ -61,6 +65,20 @@ def test_rebin_bit_exact(gpu, ViewN, YL, ZL, layout):
     if layout == _lib.ARECON_LAYOUT_VZY:
         got = got.reshape(ViewN, ZL, YL).transpose(0, 2, 1)
     assert np.array_equal(got, want), f"max |diff| {np.abs(got - want).max()}"
+
+
+def test_rebin_rejects_rows_beyond_the_lds_tile(gpu):
+    """ZL 243 does not fit the transposing kernel's 64 KB of LDS: refused on the
+    host with an error, before any launch."""
+    ViewN, YL, ZL = 4, 64, 243
+    a = rebin_args(YL, ViewN)
+    din = torch.zeros(ViewN, YL, ZL, device=gpu)
+    dout = torch.empty_like(din)
+    rc = _lib.lib().arecon_rebin_dev(
+        C.c_void_p(dout.data_ptr()), C.c_void_p(din.data_ptr()), YL, ZL, ViewN, a["YLC0"],
+        a["dYA"], a["DeltaTheta"], a["PLC0"], a["DeltaT"], a["DeltaFai"], a["SO"],
+        _lib.ARECON_LAYOUT_VZY, stream())
+    assert rc == 1 and "ZL" in _lib.last_error()[1]
 
 
 def test_rebin_drop_in_gateway(gpu):
