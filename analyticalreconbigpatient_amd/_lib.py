@@ -64,6 +64,7 @@ SIGNATURES = {
     "arecon_device_count": (_i, []),
     "arecon_rebin_dev": (_i, [_p, _p, _i, _i, _i, _f, _f, _f, _f, _f, _f, _f, _i, _p]),
     "arecon_filter_dev": (_i, [_p, _i, _i, _p, _i, _i, _i, _f, _f, _f, _f, _f, _p]),
+    "arecon_filter_dev_noalloc": (_i, [_p, _i, _i, _p, _i, _i, _i, _f, _f, _f, _f, _f, _p]),
     "arecon_rebin_views_dev": (_i, [_p, _p, _i, _i, _i, _f, _f, _f, _f, _f, _f, _f, _i, _i, _p]),
     "arecon_bp_plan_create": (_i, [C.POINTER(BPGeom), _dp, C.POINTER(_p)]),
     "arecon_bp_plan_destroy": (_i, [_p]),

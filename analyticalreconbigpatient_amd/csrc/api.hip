@@ -314,6 +314,15 @@ int arecon_filter_dev(void* d_out, int out_dtype, int out_layout, const float* d
     });
 }
 
+int arecon_filter_dev_noalloc(void* d_out, int out_dtype, int out_layout, const float* d_in,
+                              int YL, int ZL, int ViewN, float PLC, float ZLC, float dYL,
+                              float dZL, float SO, void* stream) {
+    return guarded([&] {
+        launch_filter_noalloc(d_out, out_dtype, out_layout, d_in, YL, ZL, ViewN, PLC, ZLC, dYL,
+                              dZL, SO, static_cast<hipStream_t>(stream));
+    });
+}
+
 int arecon_bp_plan_create(const arecon_bp_geom* g, const double* w, arecon_bp_plan** out) {
     return guarded([&] {
         ARECON_REQUIRE(g && w && out, "arecon_bp_plan_create: null pointer");

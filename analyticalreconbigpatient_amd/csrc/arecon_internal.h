@@ -85,6 +85,11 @@ void launch_rebin(float* out, const float* in, int YL, int ZL, int ViewN,
 void launch_filter(void* out, int out_dtype, int out_layout, const float* in, int YL, int ZL,
                    int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
                    hipStream_t s);
+// Same kernels; the tables are built on the stream in a fixed per-device array
+// (no allocation, no host copy).
+void launch_filter_noalloc(void* out, int out_dtype, int out_layout, const float* in, int YL,
+                           int ZL, int ViewN, float PLC, float ZLC, float dYL, float dZL,
+                           float SO, hipStream_t s);
 
 // Per-view / per-slice / per-offset tables of the backprojector, built on the
 // host in FP64 and cast to FP32 (see backproject.hip).

@@ -972,4 +972,115 @@ void launch_filter(void* out, int out_dtype, int out_layout, const float* in, in
                        nullptr);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant (arecon_filter_dev_noalloc).  The tables that
+// launch_filter reads from its per-device caches (allocated on first use) are
+// built instead, by a kernel on the caller's stream, in a fixed device array:
+//
+//   [0, 32 KB)         twiddles, N float2 (N <= 4096)
+//   [32 KB, 48 KB)     spectrum H/N, N floats
+//   [48 KB, 1 MB)      preweight [ZL][YL] floats: ZL * YL <= 249,856
+//
+// Each device holds one copy for the life of the process.  Every call on a
+// device rewrites it, so calls that overlap on one device (two streams, two
+// threads) race; calls on one stream are ordered.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(1) << 20;
+constexpr size_t kScratchTwOff = 0;
+constexpr size_t kScratchHOff = 32 * 1024;
+constexpr size_t kScratchWOff = 48 * 1024;
+constexpr long long kScratchMaxW = (long long)((kScratchBytes - kScratchWOff) / sizeof(float));
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on each device, looked up once per device with
+// that device current (the symbol's host-side name is not a device address).
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static unsigned char* scratch_ptr(int dev) {
+    ARECON_REQUIRE(dev >= 0 && dev < kMaxDevices, "filter (no-alloc): device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        ARECON_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ARECON_REQUIRE(p != nullptr, "filter (no-alloc): scratch symbol has no device address");
+        g_scratch_ptr[dev] = p;
+    }
+    return static_cast<unsigned char*>(g_scratch_ptr[dev]);
+}
+
+// The same tables filter_tables() and preweight_table() compute on the host,
+// computed here in FP64 with the same expressions in the same order (FMA
+// contraction off) and rounded to FP32.  The spectrum sums the taps against the
+// cosine table kept in LDS, as the host does; only the libm behind cos/sin
+// differs (OCML vs the host's), by at most an ulp of FP64.
+__global__ __launch_bounds__(256) void filter_tables_dev(float2* __restrict__ tw,
+                                                         float* __restrict__ H,
+                                                         float* __restrict__ W, int N, int YL,
+                                                         int ZL, float PLC, float ZLC,
+                                                         float dYL_f, float dZL, float SO) {
+#pragma clang fp contract(off)
+    extern __shared__ double tabs[];     // c[N], then h[YL]
+    double* c = tabs;
+    double* h = tabs + N;
+    const double dYL = (double)dYL_f;
+    for (int m = threadIdx.x; m < N; m += blockDim.x) c[m] = cos(2.0 * M_PI * m / N);
+    for (int n = threadIdx.x; n < YL; n += blockDim.x)
+        h[n] = -2.0 / (9.869604401089358 * (4.0 * n * n - 1.0)) / dYL;
+    __syncthreads();
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < N; f += gridDim.x * blockDim.x) {
+        double acc = h[0];
+        long long idx = 0;
+        for (int n = 1; n < YL; ++n) {
+            idx += f;
+            if (idx >= N) idx -= N;
+            acc += 2.0 * h[n] * c[idx];
+        }
+        H[f] = (float)(acc / N);
+        tw[f] = make_float2((float)c[f], (float)-sin(2.0 * M_PI * f / N));
+    }
+    const double so2 = (double)SO * SO;
+    const long long nw = (long long)ZL * YL;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nw;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(i / YL);
+        const int j = (int)(i - (long long)k * YL);
+        const double b = ((double)k - ZLC) * dZL;
+        const double t = ((double)j - PLC) * dYL;
+        W[i] = (float)(so2 / sqrt(so2 * (so2 + b * b) - b * b * t * t));
+    }
+}
+
+void launch_filter_noalloc(void* out, int out_dtype, int out_layout, const float* in, int YL,
+                           int ZL, int ViewN, float PLC, float ZLC, float dYL, float dZL,
+                           float SO, hipStream_t s) {
+    ARECON_REQUIRE(YL > 0 && ZL > 0 && ViewN > 0, "filter: YL, ZL and ViewN must be positive");
+    ARECON_REQUIRE(YL <= 2048, "filter: YL above 2048 is not supported");
+    ARECON_REQUIRE((long long)ZL * YL <= kScratchMaxW,
+                   "filter (no-alloc): ZL * YL exceeds the 249,856 detector samples per view "
+                   "that its fixed 1 MiB scratch holds; use arecon_filter_dev");
+    ARECON_REQUIRE(dYL != 0.f && SO > 0.f, "filter: dYL must be nonzero and SO positive");
+    int dev = 0;
+    ARECON_HIP(hipGetDevice(&dev));
+    unsigned char* base = scratch_ptr(dev);
+    const int N = filter_length(YL);
+    TableSet t;
+    t.tw = reinterpret_cast<const float2*>(base + kScratchTwOff);
+    t.H = reinterpret_cast<const float*>(base + kScratchHOff);
+    t.W = reinterpret_cast<const float*>(base + kScratchWOff);
+    t.N = N;
+    const size_t lds = sizeof(double) * ((size_t)N + (size_t)YL);
+    const unsigned blocks = (unsigned)std::max(1, N / 256);
+    hipLaunchKernelGGL(filter_tables_dev, dim3(blocks), dim3(256), lds, s,
+                       reinterpret_cast<float2*>(base + kScratchTwOff),
+                       reinterpret_cast<float*>(base + kScratchHOff),
+                       reinterpret_cast<float*>(base + kScratchWOff), N, YL, ZL, PLC, ZLC, dYL,
+                       dZL, SO);
+    ARECON_HIP(hipGetLastError());
+    launch_filter_impl(out, out_dtype, out_layout, in, YL, ZL, ViewN, PLC, ZLC, dYL, dZL, SO, s,
+                       &t);
+}
+
 }  // namespace arecon

@@ -170,6 +170,22 @@ int arecon_filter_dev(void* d_out, int out_dtype, int out_layout, const float* d
                       int ZL, int ViewN, float PLC, float ZLC, float dYL, float dZL, float SO,
                       void* stream);
 
+/* Allocation-free variant of arecon_filter_dev: same arguments, layouts,
+ * kernels and results.  No counterpart in the reference; it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Instead of the per-device table
+ * caches arecon_filter_dev fills on first use, its twiddle, spectrum and
+ * preweight tables are built by a kernel on `stream`, in FP64 rounded to FP32,
+ * in a fixed 1 MiB device array that each device holds until the process ends.
+ * Limit: ZL * YL <= 249,856 (and YL <= 2048 as for arecon_filter_dev); larger
+ * shapes return ARECON_ERR_ARG.  Calls no hipMalloc/hipFree and does not
+ * synchronise.  Every call on a device rewrites that array: calls that overlap
+ * on one device (on different streams or threads) race; calls on one stream
+ * are ordered. */
+int arecon_filter_dev_noalloc(void* d_out, int out_dtype, int out_layout, const float* d_in,
+                              int YL, int ZL, int ViewN, float PLC, float ZLC, float dYL,
+                              float dZL, float SO, void* stream);
+
 /* Backprojection geometry: the derived quantities the reference passes to its
  * kernel (src/ParallelFDKHelical3DWeightingGPU.h:13-22), kept in FP64. */
 typedef struct arecon_bp_geom {
