@@ -96,6 +96,9 @@ SIGNATURES = {
     "arecon_cnn_residual_out": (_i, [_p, _p, _i, _p, C.c_longlong, _p]),
     "arecon_bp_dev_f64": (_i, [_p, _p, C.POINTER(BPGeom), _dp, _p]),
     "arecon_fdk_weights": (_i, [_dp, _i, _d, _i]),
+    "arecon_probe_lds": (_i, [_p, _i, _i, _i, _dp, _p]),
+    "arecon_probe_l2": (_i, [_p, _p, C.c_longlong, _i, _i, _dp, _p]),
+    "arecon_probe_copy": (_i, [_p, _p, C.c_longlong, _p]),
     "arecon_project_ellipsoids_dev": (_i, [_p, _fp, _i, _i, _i, _i, _d, _d, _d, _d, _d, _d,
                                            _d, _d, _d, _p]),
 }

@@ -261,11 +261,14 @@ __global__ __launch_bounds__(256) void conv3_fwd_tiled(u16* __restrict__ y,
 // ---------------------------------------------------------------------------
 // Weight gradient: dW[co][tap][ci] += sum_n dY[co][n] X_dx[ci][nb(n, dz, dy)],
 // with dY (already masked by the ReLU derivative) and X channel-major.  One
-// wavefront: 16 co x 16 ci of one tap over a range of voxels; the K dimension
-// of the MFMA is 32 voxels of one row segment (W is a multiple of 8, so each
-// lane's 8 voxels lie in one row).  Partial sums are added atomically.
+// wavefront: 16 co x 16 ci of one tap over one range (split) of the voxels;
+// the K dimension of the MFMA is 32 voxels of one row segment (W is a multiple
+// of 8, so each lane's 8 voxels lie in one row).  Each wavefront writes its
+// tile's partial sums to part[split][co][tap][ci] (every entry has exactly one
+// writer); wgrad_reduce then adds the splits in order, so the gradient is the
+// same bits on every run.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void conv3_wgrad(float* __restrict__ dw,
+__global__ __launch_bounds__(256) void conv3_wgrad(float* __restrict__ part,
                                                    const u16* __restrict__ dyT,
                                                    const u16* __restrict__ xT3, Vol v, int cin,
                                                    int cout, long long vox_per_split,
@@ -305,11 +308,23 @@ __global__ __launch_bounds__(256) void conv3_wgrad(float* __restrict__ dw,
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
     }
     // D[i = co in tile][j = ci in tile]
+    float* ps = part + (long long)split * cout * 27 * cin;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int c_o = 16 * tc + 4 * kg + q, c_i = 16 * ti + col;
-        if (c_o < cout && c_i < cin)
-            atomicAdd(dw + ((long long)c_o * 27 + tap) * cin + c_i, acc[q]);
+        if (c_o < cout && c_i < cin) ps[((long long)c_o * 27 + tap) * cin + c_i] = acc[q];
+    }
+}
+
+// dst[i] += sum over k = 0 .. nparts-1, in order, of part[k * n + i].
+__global__ __launch_bounds__(256) void sum_parts(float* __restrict__ dst,
+                                                 const float* __restrict__ part, long long n,
+                                                 int nparts) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        float acc = 0.f;
+        for (int k = 0; k < nparts; ++k) acc += part[(long long)k * n + i];
+        dst[i] += acc;
     }
 }
 
@@ -335,9 +350,9 @@ __global__ void relu_bwd(u16* __restrict__ dyp, u16* __restrict__ dypT,
     }
 }
 
-// Bias gradient: db[ch] += sum_n dYT[ch][n]; blockIdx.y = channel, blocks
-// along x split the voxels; one atomic per block.
-__global__ __launch_bounds__(256) void channel_sum(float* __restrict__ db,
+// Bias gradient, first pass: part[b][ch] = sum of dYT[ch][n] over block b's
+// share of the voxels (blockIdx.y = channel); sum_parts adds the blocks in order.
+__global__ __launch_bounds__(256) void channel_sum(float* __restrict__ part,
                                                    const u16* __restrict__ dyT, long long nv) {
     const int ch = blockIdx.y;
     float s = 0.f;
@@ -345,10 +360,11 @@ __global__ __launch_bounds__(256) void channel_sum(float* __restrict__ db,
          n += (long long)gridDim.x * blockDim.x)
         s += bf2f(dyT[(long long)ch * nv + n]);
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
-    __shared__ float part[4];
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __shared__ float red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(db + ch, part[0] + part[1] + part[2] + part[3]);
+    if (threadIdx.x == 0)
+        part[(long long)blockIdx.x * gridDim.y + ch] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // X channels-last -> three channel-major copies shifted by dx = -1, 0, +1
@@ -505,11 +521,15 @@ __global__ void pack_weights(u16* __restrict__ wf, u16* __restrict__ wd,
 }
 
 // Residual output and loss gradient: pred = x0 + out[ch 0] (x0 in FP32);
-// dout[ch 0] = 2 (pred - target) / N (other channels 0); per-wavefront partial
-// sums of the squared error / N go to loss[0] atomically.
-__global__ void mse_residual(u16* __restrict__ dout, float* __restrict__ loss,
-                             const u16* __restrict__ out, int cout, const float* __restrict__ x0,
-                             const float* __restrict__ target, long long nv, float scale) {
+// dout[ch 0] = 2 (pred - target) / N (other channels 0); each block writes its
+// partial sum of the squared error / N to part[blockIdx.x] (sum_parts adds
+// them in order).
+__global__ __launch_bounds__(256) void mse_residual(u16* __restrict__ dout,
+                                                    float* __restrict__ part,
+                                                    const u16* __restrict__ out, int cout,
+                                                    const float* __restrict__ x0,
+                                                    const float* __restrict__ target,
+                                                    long long nv, float scale) {
     float local = 0.f;
     GRID_STRIDE(n, nv) {
         const float pred = x0[n] + bf2f(out[n * cout]);
@@ -519,7 +539,10 @@ __global__ void mse_residual(u16* __restrict__ dout, float* __restrict__ loss,
         for (int ch = 1; ch < cout; ++ch) dout[n * cout + ch] = 0;
     }
     for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, local * scale);
+    __shared__ float red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = local * scale;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // Adam on FP32 master weights.
@@ -639,9 +662,14 @@ void cnn_conv3_wgrad(float* dw, const void* dyT, const void* xT3, const int* dim
     per = (per + 31) / 32 * 32;
     nsplit = (nv + per - 1) / per;
     const long long waves = tiles * nsplit;
-    hipLaunchKernelGGL(conv3_wgrad, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, dw,
-                       static_cast<const u16*>(dyT), static_cast<const u16*>(xT3), v, cin, cout,
-                       per, (int)nsplit);
+    const long long nw = 27LL * cout * cin;
+    DevBuf part(sizeof(float) * (size_t)(nw * nsplit), s);
+    hipLaunchKernelGGL(conv3_wgrad, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s,
+                       part.as<float>(), static_cast<const u16*>(dyT),
+                       static_cast<const u16*>(xT3), v, cin, cout, per, (int)nsplit);
+    ARECON_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sum_parts, dim3(grid_for(nw)), dim3(256), 0, s, dw, part.as<float>(), nw,
+                       (int)nsplit);
     ARECON_HIP(hipGetLastError());
 }
 
@@ -655,8 +683,12 @@ void cnn_relu_bwd(void* dyp, void* dypT, float* db, const void* dy, const void* 
     if (db) {
         const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>(
                                                                  (v.nvox() + 4095) / 4096, 256));
-        hipLaunchKernelGGL(channel_sum, dim3(bx, (unsigned)c), dim3(256), 0, s, db,
-                           static_cast<const u16*>(dypT), v.nvox());
+        DevBuf part(sizeof(float) * (size_t)bx * (size_t)c, s);
+        hipLaunchKernelGGL(channel_sum, dim3(bx, (unsigned)c), dim3(256), 0, s,
+                           part.as<float>(), static_cast<const u16*>(dypT), v.nvox());
+        ARECON_HIP(hipGetLastError());
+        hipLaunchKernelGGL(sum_parts, dim3(1), dim3(256), 0, s, db, part.as<float>(),
+                           (long long)c, (int)bx);
         ARECON_HIP(hipGetLastError());
     }
 }
@@ -714,9 +746,14 @@ void cnn_pack_weights(void* wf, void* wd, const float* w, int cout, int cin, hip
 }
 void cnn_mse_residual(void* dout, float* loss, const void* out, int cout, const float* x0,
                       const float* target, long long nv, hipStream_t s) {
-    hipLaunchKernelGGL(mse_residual, dim3(grid_for(nv)), dim3(256), 0, s,
-                       static_cast<u16*>(dout), loss, static_cast<const u16*>(out), cout, x0,
-                       target, nv, 1.f / (float)nv);
+    const unsigned blocks = std::min<unsigned>(grid_for(nv), 1024u);
+    DevBuf part(sizeof(float) * blocks, s);
+    hipLaunchKernelGGL(mse_residual, dim3(blocks), dim3(256), 0, s, static_cast<u16*>(dout),
+                       part.as<float>(), static_cast<const u16*>(out), cout, x0, target, nv,
+                       1.f / (float)nv);
+    ARECON_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sum_parts, dim3(1), dim3(256), 0, s, loss, part.as<float>(), 1LL,
+                       (int)blocks);
     ARECON_HIP(hipGetLastError());
 }
 void cnn_adam(float* w, float* m, float* v, const float* g, long long n, float lr, float b1,

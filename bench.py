@@ -6,10 +6,19 @@ One step = one pass of the three stages over the device-resident sinogram.
 value = voxel-updates of one pass (512*512*400 voxels x 671-view windows =
 7.036e10) / measured seconds per pass.  Prints one JSON line.
 
-Also reported: per-stage times (HIP events on the launch stream), the FP16
-filtered-storage variant, the dominant kernel's roofline, the HU RMSE against
-the FP64 oracle on sampled slices, and the CPU baseline (the oracle, timed on
-this host on a bounded sample and scaled to the whole workload).
+Also reported: per-stage times (HIP events on the launch stream); the chosen
+form (FP16 filtered storage, the volume as uneven z-slabs in one batched
+launch, the pass replayed from a HIP graph); the pass including the
+host-to-device copy of the raw sinogram; the dominant kernel's roofline, from
+hardware counters collected by rocprofv3 in two child passes of this run (VALU
+instructions per voxel-update, HBM bytes calibrated on a copy) and from
+bandwidth probes (LDS reads, L2 gathers, HBM copy); the HU RMSE against the
+FP64 oracle on sampled slices; and the CPU baseline (the reference's own CPU
+functions timed on this host on a bounded sample, scaled to the workload).
+
+Other workloads: --workload batch (3 scans, 18 uneven slabs), cnn (the learned
+correction), cpu (the reference CPU path on the 368 x 16 scan, in full, on one
+thread and on all threads).
 """
 from __future__ import annotations
 
@@ -42,18 +51,21 @@ def parse():
     ap.add_argument("--variant", type=int, default=None,
                     help="backprojector kernel variant (default: chosen from the geometry)")
     ap.add_argument("--group", type=int, default=0, help="z-column kernel slice grouping")
-    ap.add_argument("--overlap", type=int, default=1,
-                    help="view chunks / slice-block groups of the overlapped pass (1: off)")
     ap.add_argument("--layout", default=None,
                     help="filtered layout: rows, rowpairs, rowdiff or quads")
     ap.add_argument("--storages", default="f32,f16",
                     help="filtered-sinogram storages to time (plain f32 first)")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="only run a 2 GiB device copy (calibrates HBM byte counters)")
-    ap.add_argument("--workload", default="clinical", choices=["clinical", "batch", "cnn"],
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (VALU per update, HBM bytes)")
+    ap.add_argument("--no-slabs", action="store_true",
+                    help="chosen form without slab batching (FP16 storage only)")
+    ap.add_argument("--counter-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--workload", default="clinical", choices=["clinical", "batch", "cnn", "cpu"],
                     help="clinical: the 888x64x11808 -> 512x512x400 pass (default); batch: "
                          "3 scans of 4/9/15 rotations, 18 uneven slabs, batched + HIP graph; "
-                         "cnn: train the learned correction and run it on 512x512x256")
+                         "cnn: train the learned correction and run it on 512x512x256; "
+                         "cpu: the reference CPU path on the 368x16x2160 scan, 1 thread and "
+                         "all threads")
     ap.add_argument("--cnn-pairs", type=int, default=200, help="training phantoms (cnn)")
     ap.add_argument("--cnn-clinical-pairs", type=int, default=300,
                     help="training crops of the 888x64 scan (cnn)")
@@ -64,16 +76,24 @@ def parse():
 
 
 def kernel_name(fdk) -> str:
-    """The backprojector's kernel as rocprofv3 names it (FP32 storage)."""
+    """The backprojector's kernel as rocprofv3 names it."""
     lay = {"rows": 0, "rowpairs": 1, "quads": 2, "rowdiff": 3}[fdk.layout]
     inside = "true" if fdk.plan.inside else "false"
+    t = {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16"}[fdk.storage]
     if fdk.variant == 2:
         g = 0 if fdk.group == 8 else fdk.group
-        return f"bp_fdk3dw_zc<{fdk.plan_nz}, {g}, float, {lay}, {g == 0 and inside or 'false'}>"
+        return f"bp_fdk3dw_zc<{fdk.plan_nz}, {g}, {t}, {lay}, {g == 0 and inside or 'false'}>"
     if fdk.variant == 3:
         rmax = 21 if fdk.plan_nz == 8 else 40
-        return f"bp_fdk3dw_zs<{fdk.plan_nz}, float, {lay}, {rmax}, {inside}>"
-    return f"bp_fdk3dw{'_vt' if fdk.variant == 0 else ''}<{fdk.plan_nz}, float>"
+        return f"bp_fdk3dw_zs<{fdk.plan_nz}, {t}, {lay}, {rmax}, {inside}>"
+    return f"bp_fdk3dw{'_vt' if fdk.variant == 0 else ''}<{fdk.plan_nz}, {t}>"
+
+
+def _kbase(k: str) -> tuple[str, str]:
+    """(base name, template arguments) of a kernel name as rocprofv3 prints it."""
+    import re
+    m = re.search(r"::(\w+)(<[^()]*>)?\(", k) or re.search(r"(\w+)(<[^()]*>)?", k)
+    return (m.group(1), m.group(2) or "") if m else (k, "")
 
 
 def log(msg):
@@ -95,55 +115,62 @@ def main():
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    import torch
-
-    if args.copy_probe:
-        n = 1 << 29                                   # 2 GiB of FP32
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}))
+    if args.counter_child:
+        counter_child(args)
         return
-
     if args.workload == "batch":
         bench_batch(args)
         return
     if args.workload == "cnn":
         bench_cnn(args)
         return
+    if args.workload == "cpu":
+        bench_cpu(args)
+        return
+    bench_clinical(args)
+
+
+def clinical_workload():
+    """The 888x64 helical scan (984 views/rotation, pitch 0.984, 12 rotations)
+    of a random-ellipsoid body (seed 2), exact line integrals, on the device."""
+    import torch
 
     from analyticalreconbigpatient_amd import geometry as G
     from analyticalreconbigpatient_amd import phantom
-    from analyticalreconbigpatient_amd.pipeline import HelicalFDK
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    dev = torch.device("cuda:0")
     wl = G.body_clinical()
     s = wl.setup()
     p0 = G.bp_params(G.bp_scan_geom(s))
     z_mid = ((wl.zn - 1) * 0.5 - p0.ZNC) * p0.dz
-    log(f"workload {wl.name}; projecting phantom")
     E = phantom.random_body(seed=2, fov_radius=250.0, z_centre=z_mid, z_half=250.0)
-    raw = phantom.project(E, s, wl.hdr, device=str(dev))
+    raw = phantom.project(E, s, wl.hdr, device="cuda")
     torch.cuda.synchronize()
+    return wl, s, raw
 
-    results = {}
-    storages = [x for x in args.storages.split(",") if x]
-    assert storages, "no storage to time"
-    for storage in storages:
-        fdk = HelicalFDK(s, storage=storage, nz=args.nz, order=args.order,
-                         variant=args.variant, group=args.group, layout=args.layout)
-        for _ in range(args.warmup):
+
+# Slabs of the 512 x 512 x 400 volume in the chosen form: uneven z-slabs (whole
+# 16-slice blocks) in one batched launch.
+CHOSEN_SLABS = (96, 112, 80, 112)
+
+
+def bench_clinical(args):
+    import numpy as np
+    import torch
+
+    from analyticalreconbigpatient_amd import batching as B
+    from analyticalreconbigpatient_amd.pipeline import HelicalFDK
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    log("clinical workload: projecting the phantom")
+    wl, s, raw = clinical_workload()
+
+    def timed_stages(fdk, steps, warmup):
+        for _ in range(warmup):
             fdk.run(raw)
         torch.cuda.synchronize()
-        log(f"{storage}: warmup done, timing {args.steps} steps")
-        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
-        torch.cuda.synchronize()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
         t0 = time.perf_counter()
-        for k in range(args.steps):
+        for k in range(steps):
             e = ev[k]
             e[0].record()
             fdk.rebin(raw)
@@ -156,87 +183,63 @@ def main():
         wall = time.perf_counter() - t0
         st = np.array([[e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]),
                         e[2].elapsed_time(e[3])] for e in ev])
-        seq_ms = wall * 1e3 / args.steps
-        ovl_ms = None
-        if args.overlap > 1 and fdk.plan.nblocks > 1:
-            for _ in range(args.warmup):
-                fdk.run_overlapped(raw, args.overlap, args.overlap)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                fdk.run_overlapped(raw, args.overlap, args.overlap)
-            torch.cuda.synchronize()
-            ovl_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        ms = min(seq_ms, ovl_ms) if ovl_ms else seq_ms
-        results[storage] = dict(fdk=fdk, wall=wall, stages=st.mean(axis=0), ms=ms,
-                                seq_ms=seq_ms, ovl_ms=ovl_ms)
-        log(f"{storage}: {ms:.2f} ms/step (stages back to back {seq_ms:.2f}, overlapped "
-            f"{ovl_ms}), stages (ms) {np.round(st.mean(axis=0), 3).tolist()}")
+        return wall * 1e3 / steps, st.mean(axis=0)
 
-    plain = results.get("f32", results[storages[0]])
+    results = {}
+    for storage in [x for x in args.storages.split(",") if x]:
+        fdk = HelicalFDK(s, storage=storage, nz=args.nz, order=args.order,
+                         variant=args.variant, group=args.group, layout=args.layout)
+        ms, st = timed_stages(fdk, args.steps, args.warmup)
+        results[storage] = dict(fdk=fdk, ms=ms, stages=st)
+        log(f"{storage}: {ms:.3f} ms/pass, stages (ms) {np.round(st, 3).tolist()}")
+    plain = results.get("f32", next(iter(results.values())))
     fdk = plain["fdk"]
+    p = fdk.params
     updates = fdk.updates
     value = updates / (plain["ms"] * 1e-3) / 1e9
-    chosen = results.get("f16", plain)
+
+    # chosen form: FP16 storage + slab batching, the whole pass replayed from a
+    # HIP graph
+    chosen = None
+    if "f16" in results and not args.no_slabs:
+        z, slabs = 0, []
+        for d in CHOSEN_SLABS:
+            slabs.append(B.Slab(0, (z - p.ZNC) * p.dz, d, p.dz))
+            z += d
+        assert z == p.ZN
+        rb = B.BatchedRecon([s], [raw], slabs, storage="f16",
+                            shape=B.Shape(16, 8, 2, "given", 3))
+        rb.run()
+        torch.cuda.synchronize()
+        single = results["f16"]["fdk"].run(raw)
+        torch.cuda.synchronize()
+        joined = torch.cat(rb.volumes, dim=2)
+        diff = float((joined - single).abs().max())
+        rb.capture()
+        for _ in range(args.warmup):
+            rb.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rb.replay()
+        torch.cuda.synchronize()
+        gms = (time.perf_counter() - t0) * 1e3 / args.steps
+        assert rb.updates == updates, (rb.updates, updates)
+        chosen = dict(ms=gms, recon=rb, max_abs_diff_vs_single=diff,
+                      bp_ms=rb.time_backproject(3))
+        log(f"chosen (FP16, {len(slabs)} slabs {list(CHOSEN_SLABS)}, graph): {gms:.3f} ms/pass; "
+            f"max |slabs - single plan| {diff:.3g}")
+    if chosen is None:
+        f16 = results.get("f16", plain)
+        chosen = dict(ms=f16["ms"], recon=None, max_abs_diff_vs_single=0.0,
+                      bp_ms=float(f16["stages"][2]))
     chosen_value = updates / (chosen["ms"] * 1e-3) / 1e9
 
-    # Roofline of the dominant kernel (backprojection): algorithmic bytes per
-    # launch = 16 B of gathered FP32 samples per voxel-update + 4 B per voxel.
-    bp_ms = float(plain["stages"][2])
-    p = fdk.params
-    nvox = p.XN * p.YN * p.ZN
-    alg_bytes = 16.0 * updates + 4.0 * nvox
-    achieved = alg_bytes / (bp_ms * 1e-3) / 1e9
-    kname = kernel_name(fdk)
-    traffic, entry = None, None
-    tfile = ROOT / "profiles" / "traffic.json"
-    if tfile.exists():
-        try:
-            entry = json.loads(tfile.read_text()).get("kernels", {}).get(kname)
-            if entry and entry.get("workload") == wl.name:
-                traffic = entry["fetch_bytes"] + entry["write_bytes"]
-            else:
-                entry = None
-        except (ValueError, OSError, KeyError, TypeError):
-            traffic, entry = None, None
-    gvu = updates / (bp_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kname, "kernel_ms": round(bp_ms, 3), "bytes_per_update": 16}
-    if fdk.variant == 3:
-        # Samples come from LDS: each view's detector footprint is copied there once
-        # per workgroup.  The bound is VALU issue: one wavefront instruction per CU
-        # per clock (transcendentals issue on their own pipeline), so at V
-        # instructions per voxel-update the roof is N_CU * clock * 64 / V.
-        roofline["note"] = ("samples are read from LDS, where each view's detector footprint is "
-                            "staged once per workgroup; compulsory HBM bytes are the filtered "
-                            "sinogram and the volume; VALU issue bounds this kernel")
-        vpu = entry.get("valu_per_update") if entry else None
-        if vpu:
-            peak = N_CU * CLOCK_HZ * 64 / vpu / 1e9
-            roofline["valu_issue"] = {
-                "achieved": round(gvu, 1), "peak": round(peak, 1), "unit": "GVoxel-updates/s",
-                "frac": round(gvu / peak, 4), "valu_per_update": vpu,
-                "trans_per_update": entry.get("trans_per_update"),
-                "source": "profiles/sq_counters_bp_fdk3dw_zs_clinical.md"}
-    else:
-        # Issue bound of the gathers: the texture addresser takes four lane
-        # addresses per clock, so one 64-lane gather instruction occupies it for 16
-        # clocks; the row-pair, row-difference and quad layouts need one gather per
-        # voxel-update, the rows layout two.
-        gathers = 2 if fdk.layout == "rows" else 1
-        gather_peak = N_CU * CLOCK_HZ / 16.0 * 64 / gathers / 1e9
-        roofline["note"] = ("gathers are served from L1/L2; compulsory HBM bytes are the filtered "
-                            "sinogram once and the volume once, so the gather issue rate, not HBM, "
-                            "bounds this kernel")
-        roofline["gather_issue"] = {"achieved": round(gvu, 1), "peak": round(gather_peak, 1),
-                                    "unit": "GVoxel-updates/s", "frac": round(gvu / gather_peak, 4),
-                                    "gathers_per_update": gathers}
-
-    stage_roof = {
-        "rebin_GBs": round(8.0 * p.YL * p.ZL * fdk.ViewN / (plain["stages"][0] * 1e-3) / 1e9, 1),
-        "filter_GBs": round(8.0 * p.YL * p.ZL * fdk.ViewN / (plain["stages"][1] * 1e-3) / 1e9, 1),
-    }
+    probes = run_probes()
+    log(f"probes: {probes}")
+    counters = None if args.no_counters else measure_counters(args, results)
+    roofline = make_roofline(fdk, plain, results, probes, counters, wl)
+    h2d = h2d_inclusive(raw, results, chosen, args)
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GVoxel-updates/s", "n_gpus": 1,
@@ -248,28 +251,377 @@ def main():
                    "storage": "f32", "path": "plain"},
         "stages_ms": {"rebin": round(float(plain["stages"][0]), 3),
                       "filter": round(float(plain["stages"][1]), 3),
-                      "backproject": round(bp_ms, 3)},
-        "pass_ms": {"stages_back_to_back": round(plain["seq_ms"], 3),
-                    "overlapped": round(plain["ovl_ms"], 3) if plain["ovl_ms"] else None,
-                    "overlap_chunks": args.overlap},
-        "stage_hbm_GBs": stage_roof,
-        "chosen": {"storage": "f16" if "f16" in results else "f32",
-                   "value": round(chosen_value, 2),
-                   "ms_per_step": round(chosen["ms"], 3),
-                   "ms_back_to_back": round(chosen["seq_ms"], 3),
-                   "stages_ms": [round(float(x), 3) for x in chosen["stages"]],
-                   "speedup_vs_plain": round(chosen_value / value, 3)},
+                      "backproject": round(float(plain["stages"][2]), 3)},
+        "stage_hbm_GBs": {
+            "rebin": round(8.0 * p.YL * p.ZL * fdk.ViewN / (plain["stages"][0] * 1e-3) / 1e9, 1),
+            "filter": round(8.0 * p.YL * p.ZL * fdk.ViewN / (plain["stages"][1] * 1e-3) / 1e9,
+                            1)},
+        "chosen": {"form": (f"FP16 filtered storage + {len(CHOSEN_SLABS)} z-slabs "
+                            f"{list(CHOSEN_SLABS)} in one batched launch, pass replayed from a "
+                            f"HIP graph") if chosen["recon"] is not None else "FP16 storage",
+                   "value": round(chosen_value, 2), "ms_per_step": round(chosen["ms"], 3),
+                   "backproject_ms": round(chosen["bp_ms"], 3),
+                   "speedup_vs_plain": round(chosen_value / value, 3),
+                   "max_abs_diff_vs_single_plan": chosen["max_abs_diff_vs_single"]},
+        "per_storage": {k: {"ms_per_step": round(v["ms"], 3),
+                            "value": round(updates / (v["ms"] * 1e-3) / 1e9, 2),
+                            "stages_ms": [round(float(x), 3) for x in v["stages"]]}
+                        for k, v in results.items()},
+        "h2d_inclusive": h2d,
         "roofline": roofline,
+        "probes": probes,
     }
-
     if not args.no_cpu:
-        out["cpu_baseline"], out["hu_rmse"] = cpu_leg(s, fdk, results, raw, args)
+        out["cpu_baseline"], out["hu_rmse"] = cpu_leg(s, fdk, results, chosen, raw, args)
     print(json.dumps(out), flush=True)
 
 
-def cpu_leg(s, fdk, results, raw, args):
-    """Time the oracle on this host on a bounded sample and compare sampled
-    slices of the GPU volumes with the FP64 oracle chain."""
+def run_probes():
+    """Measured roofs of the backprojector's sample reads: LDS read bandwidth
+    (8- and 16-byte lane reads, conflict-free) and L2 gather bandwidth (16-byte
+    scattered reads of a 2 MiB buffer), plus an HBM streaming copy."""
+    import ctypes as C
+
+    import torch
+
+    from analyticalreconbigpatient_amd import _lib
+    lib = _lib.lib()
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    blocks = 256 * 8
+    sink = torch.empty(blocks * 256, device="cuda")
+    out = {}
+
+    def timed(fn, reps=5):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        fn()
+        ts = []
+        for _ in range(reps):
+            ev[0].record()
+            nbytes = fn()
+            ev[1].record()
+            ev[1].synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]))
+        return nbytes / (min(ts) * 1e-3) / 1e12
+
+    for b in (8, 16):
+        def lds(b=b):
+            nb = C.c_double()
+            _lib.check(lib.arecon_probe_lds(C.c_void_p(sink.data_ptr()), b, 4096, blocks,
+                                            C.byref(nb), st), "arecon_probe_lds")
+            return nb.value
+        out[f"lds_read_b{8 * b}_TBs"] = round(timed(lds), 2)
+    src = torch.randn(2 << 18, device="cuda")            # 2 MiB
+    def l2():
+        nb = C.c_double()
+        _lib.check(lib.arecon_probe_l2(C.c_void_p(sink.data_ptr()), C.c_void_p(src.data_ptr()),
+                                       src.numel() * 4, 1024, blocks, C.byref(nb), st),
+                   "arecon_probe_l2")
+        return nb.value
+    out["l2_gather_16B_TBs"] = round(timed(l2), 2)
+    a = torch.ones(1 << 28, device="cuda")                # 1 GiB
+    b2 = torch.empty_like(a)
+    def copy():
+        _lib.check(lib.arecon_probe_copy(C.c_void_p(b2.data_ptr()), C.c_void_p(a.data_ptr()),
+                                         a.numel() * 4, st), "arecon_probe_copy")
+        return 2.0 * a.numel() * 4
+    out["hbm_copy_TBs"] = round(timed(copy), 2)
+    del a, b2, src, sink
+    return out
+
+
+def counter_child(args):
+    """Child of measure_counters (runs under rocprofv3 --pmc): a 1 GiB
+    calibration copy, then one pass of each storage's pipeline (the footprint
+    table is built by a first pass, which the counters skip by kernel name)."""
+    import ctypes as C
+
+    import torch
+
+    from analyticalreconbigpatient_amd import _lib
+    from analyticalreconbigpatient_amd.pipeline import HelicalFDK
+    a = torch.ones(1 << 28, device="cuda")
+    b = torch.empty_like(a)
+    _lib.check(_lib.lib().arecon_probe_copy(C.c_void_p(b.data_ptr()), C.c_void_p(a.data_ptr()),
+                                            a.numel() * 4,
+                                            C.c_void_p(torch.cuda.current_stream().cuda_stream)),
+               "arecon_probe_copy")
+    torch.cuda.synchronize()
+    del a, b
+    wl, s, raw = clinical_workload()
+    for storage in [x for x in args.storages.split(",") if x]:
+        fdk = HelicalFDK(s, storage=storage, nz=args.nz, order=args.order,
+                         variant=args.variant, group=args.group, layout=args.layout)
+        fdk.run(raw)
+        torch.cuda.synchronize()
+        print(f"counter child: {storage} kernel {kernel_name(fdk)}", flush=True)
+
+
+def _read_counters(outdir: Path) -> list[dict]:
+    import csv
+    rows = []
+    for f in outdir.rglob("*counter_collection.csv"):
+        with open(f, newline="") as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def measure_counters(args, results):
+    """Hardware counters of this run's kernels, collected by rocprofv3 in two
+    child passes of the same workload (SQ instruction counts with HBM reads, then
+    HBM writes): VALU instructions per voxel-update of each backprojector and
+    the calibrated HBM bytes of each stage.  Returns None (and says why) when
+    rocprofv3 is unavailable or fails."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not Path(exe).exists():
+        log("counters: rocprofv3 not found")
+        return None
+    passes = {"sq": ["SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "SQ_WAVES", "GRBM_GUI_ACTIVE",
+                     "FETCH_SIZE"],
+              "wr": ["WRITE_SIZE"]}
+    per_kernel: dict[str, dict] = {}
+    for name, ctrs in passes.items():
+        d = Path(tempfile.mkdtemp(prefix=f"arecon_pmc_{name}_"))
+        cmd = [exe, "--pmc", *ctrs, "--output-format", "csv", "-d", str(d), "-o", "run", "--",
+               sys.executable, str(ROOT / "bench.py"), "--counter-child", "--storages",
+               args.storages]
+        for opt, val in (("--nz", args.nz), ("--order", args.order), ("--variant", args.variant),
+                         ("--group", args.group), ("--layout", args.layout)):
+            if val is not None:
+                cmd += [opt, str(val)]
+        log(f"counters ({name}): {' '.join(cmd[:4 + len(ctrs)])} ...")
+        try:
+            r = subprocess.run(cmd, timeout=240, stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, cwd=str(ROOT))
+        except subprocess.TimeoutExpired:
+            log(f"counters ({name}): rocprofv3 timed out")
+            return None
+        rows = _read_counters(d)
+        if not rows:
+            log(f"counters ({name}): rocprofv3 exit {r.returncode}, no counter file; "
+                f"{r.stdout.decode(errors='replace')[-400:]}")
+            return None
+        # the last dispatch of each kernel name (the footprint-table pass and the
+        # pipeline run once per storage)
+        last: dict[str, dict] = {}
+        for row in rows:
+            k = row.get("Kernel_Name", "")
+            did = int(row.get("Dispatch_Id", 0))
+            e = last.setdefault(k, {"id": -1, "c": {}})
+            if did > e["id"]:
+                e["id"], e["c"] = did, {}
+            if did == e["id"]:
+                e["c"][row["Counter_Name"]] = e["c"].get(row["Counter_Name"], 0.0) + \
+                    float(row["Counter_Value"])
+        for k, e in last.items():
+            per_kernel.setdefault(k, {}).update(e["c"])
+        shutil.rmtree(d, ignore_errors=True)
+    # HBM byte counters calibrated on the 1 GiB copy (reads 1 GiB, writes 1 GiB)
+    cal = next((v for k, v in per_kernel.items() if "probe_copy" in k), None)
+    if cal is None or not cal.get("FETCH_SIZE") or not cal.get("WRITE_SIZE"):
+        log("counters: no calibration copy in the counter files")
+        return None
+    fetch_f = float(1 << 30) / (cal["FETCH_SIZE"] * 1024.0)
+    write_f = float(1 << 30) / (cal["WRITE_SIZE"] * 1024.0)
+    out = {"fetch_factor": round(fetch_f, 4), "write_factor": round(write_f, 4), "kernels": {}}
+    for k, c in per_kernel.items():
+        if "probe_copy" in k:
+            continue
+        e = {}
+        if "FETCH_SIZE" in c:
+            e["fetch_bytes"] = c["FETCH_SIZE"] * 1024.0 * fetch_f
+        if "WRITE_SIZE" in c:
+            e["write_bytes"] = c["WRITE_SIZE"] * 1024.0 * write_f
+        for n in ("SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "SQ_WAVES", "GRBM_GUI_ACTIVE"):
+            if n in c:
+                e[n] = c[n]
+        out["kernels"][k] = e
+    log(f"counters: {len(out['kernels'])} kernels, fetch x{fetch_f:.3f}, write x{write_f:.3f}")
+    return out
+
+
+def _match_kernel(counters, name):
+    if not counters:
+        return None
+    for k, v in counters["kernels"].items():
+        if k.split("(")[0].replace(" ", "").endswith(name.replace(" ", "")) or \
+                name.replace(" ", "") in k.replace(" ", ""):
+            return v
+    return None
+
+
+def make_roofline(fdk, plain, results, probes, counters, wl):
+    """Roofline of the dominant kernel, the plain path's backprojector.  The
+    LDS-staged kernel (variant 3) reads its samples from LDS and is bound by VALU
+    issue: one wavefront instruction per CU per clock (transcendentals issue
+    beside them), so at V non-transcendental instructions per voxel-update its
+    roof is N_CU * clock * 64 / V voxel-updates/s.  V is measured in this run
+    (rocprofv3 SQ_INSTS_VALU - SQ_INSTS_VALU_TRANS_F32 over the launch's
+    wavefront-updates).  Beside it: the compulsory HBM bytes (filtered sinogram
+    once, volume once) against the 8 TB/s peak, and the algorithmic sample
+    bytes (16 B per voxel-update) against the LDS read bandwidth the probe
+    measured."""
+    p = fdk.params
+    updates = fdk.updates
+    nvox = p.XN * p.YN * p.ZN
+    bp_ms = float(plain["stages"][2])
+    gvu = updates / (bp_ms * 1e-3) / 1e9
+    kname = kernel_name(fdk)
+    k = _match_kernel(counters, kname)
+    elem = {"f32": 4, "f16": 2, "bf16": 2}[fdk.storage]
+    per_view = fdk.filtered[0].numel() * elem
+    compulsory = per_view * fdk.n_views + 4.0 * nvox
+    traffic = None
+    if k and "fetch_bytes" in k and "write_bytes" in k:
+        traffic = round(k["fetch_bytes"] + k["write_bytes"])
+    roof = {"kernel": kname, "kernel_ms": round(bp_ms, 3)}
+    vpu = None
+    if k and "SQ_INSTS_VALU" in k:
+        vpu = (k["SQ_INSTS_VALU"] - k.get("SQ_INSTS_VALU_TRANS_F32", 0.0)) / (updates / 64.0)
+    if fdk.variant == 3 and vpu:
+        peak = N_CU * CLOCK_HZ * 64 / vpu / 1e9
+        roof.update({"bound": "valu_issue", "achieved": round(gvu, 1), "peak": round(peak, 1),
+                     "unit": "GVoxel-updates/s", "frac": round(gvu / peak, 4),
+                     "traffic": traffic, "valu_per_update": round(vpu, 3),
+                     "valu_source": "rocprofv3 --pmc SQ_INSTS_VALU - SQ_INSTS_VALU_TRANS_F32, "
+                                    "this run",
+                     "peak_basis": f"{N_CU} CUs x {CLOCK_HZ / 1e9} GHz x 64 lanes / "
+                                   f"{vpu:.3f} VALU per voxel-update"})
+        if k.get("GRBM_GUI_ACTIVE"):
+            clk = k["GRBM_GUI_ACTIVE"] / 8.0 / (bp_ms * 1e-3)
+            busy = (k["SQ_INSTS_VALU"] - k.get("SQ_INSTS_VALU_TRANS_F32", 0.0)) / N_CU / \
+                (k["GRBM_GUI_ACTIVE"] / 8.0)
+            roof["measured_clock_GHz"] = round(clk / 1e9, 3)
+            roof["valu_busy_at_measured_clock"] = round(busy, 4)
+    else:
+        # no counters: the HBM roof of the compulsory bytes is what can be stated
+        roof.update({"bound": "hbm", "achieved": round(compulsory / (bp_ms * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(compulsory / (bp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "note": "VALU count not measured in this run (no rocprofv3 counters)"})
+    roof["hbm_compulsory"] = {"bytes": round(compulsory),
+                              "achieved_GBs": round(compulsory / (bp_ms * 1e-3) / 1e9, 1),
+                              "peak_GBs": HBM_PEAK_GBS,
+                              "frac": round(compulsory / (bp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if traffic:
+        roof["hbm_measured"] = {"bytes": traffic,
+                                "achieved_GBs": round(traffic / (bp_ms * 1e-3) / 1e9, 1),
+                                "frac": round(traffic / (bp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                "x_compulsory": round(traffic / compulsory, 2)}
+    lds_peak = probes.get("lds_read_b128_TBs") if elem == 4 else probes.get("lds_read_b64_TBs")
+    samples = 4.0 * elem * updates
+    if lds_peak:
+        ach = samples / (bp_ms * 1e-3) / 1e12
+        roof["lds_gather"] = {"bytes_per_update": 4 * elem, "achieved_TBs": round(ach, 2),
+                              "peak_TBs": lds_peak, "frac": round(ach / lds_peak, 4),
+                              "peak_source": "arecon_probe_lds, this run"}
+    # every stage's HBM bytes from the counters
+    stages = {}
+    for kk, v in (counters or {}).get("kernels", {}).items():
+        base, targs = _kbase(kk)
+        if base in ("rebin_vzy", "filter_quads_wave", "filter_rows_wave", "bp_fdk3dw_zs",
+                    "bp_zs_table") and "fetch_bytes" in v:
+            stages[base + targs] = {"fetch_GB": round(v["fetch_bytes"] / 1e9, 3),
+                                    "write_GB": round(v.get("write_bytes", 0.0) / 1e9, 3)}
+    if stages:
+        roof["stage_hbm_bytes"] = stages
+    # the chosen (FP16) path's backprojector, same measurements
+    f16 = results.get("f16")
+    if f16 is not None and f16["fdk"] is not fdk:
+        cf = f16["fdk"]
+        cms = float(f16["stages"][2])
+        cn = kernel_name(cf)
+        ck = _match_kernel(counters, cn)
+        c = {"kernel": cn, "kernel_ms": round(cms, 3),
+             "achieved": round(cf.updates / (cms * 1e-3) / 1e9, 1)}
+        if ck and "SQ_INSTS_VALU" in ck:
+            cv = (ck["SQ_INSTS_VALU"] - ck.get("SQ_INSTS_VALU_TRANS_F32", 0.0)) / (cf.updates / 64)
+            cpk = N_CU * CLOCK_HZ * 64 / cv / 1e9
+            c.update({"valu_per_update": round(cv, 3), "peak": round(cpk, 1),
+                      "frac": round(c["achieved"] / cpk, 4)})
+        if ck and "fetch_bytes" in ck and "write_bytes" in ck:
+            c["traffic"] = round(ck["fetch_bytes"] + ck["write_bytes"])
+        roof["chosen_kernel"] = c
+    return roof
+
+
+def h2d_inclusive(raw, results, chosen, args):
+    """The pass including the host-to-device copy of the raw sinogram from
+    pinned host memory (the reference copies the sinogram to the device in
+    every stage)."""
+    import torch
+
+    host = raw.cpu().pin_memory()
+    dev = torch.empty_like(raw)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    dev.copy_(host, non_blocking=True)
+    ev[1].record()
+    ev[1].synchronize()
+    copy_ms = ev[0].elapsed_time(ev[1])
+    out = {"h2d_ms": round(copy_ms, 3), "h2d_GBs": round(host.numel() * 4 / copy_ms / 1e6, 1)}
+    steps = max(1, min(args.steps, 5))
+    for name, res in results.items():
+        fdk = res["fdk"]
+        fdk.run(dev)
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(steps):
+            dev.copy_(host, non_blocking=True)
+            fdk.run(dev)
+        ev[1].record()
+        ev[1].synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / steps
+        out[name] = {"ms_per_step": round(ms, 3),
+                     "value": round(fdk.updates / (ms * 1e-3) / 1e9, 2)}
+    rb = chosen.get("recon")
+    if rb is not None:
+        target = rb.raws[0]
+        ev[0].record()
+        for _ in range(steps):
+            target.copy_(host, non_blocking=True)
+            rb.replay()
+        ev[1].record()
+        ev[1].synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / steps
+        out["chosen"] = {"ms_per_step": round(ms, 3),
+                         "value": round(rb.updates / (ms * 1e-3) / 1e9, 2)}
+    del host, dev
+    return out
+
+
+def _ref_rebin_threads(oracle, raw_np, a, nth):
+    """The reference's float rebinning, run on nth threads over runs of views
+    (each thread's run starts its view angle at 0: for timing only)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    V = raw_np.shape[0]
+    cuts = np.linspace(0, V, nth + 1).round().astype(int)
+
+    def part(i):
+        return oracle.ref_rebin_f32(raw_np[cuts[i]:cuts[i + 1]], a.YLC0, a.dYA, a.DeltaTheta,
+                                    a.PLC0, a.DeltaT, a.DeltaFai, a.SO)
+    with ThreadPoolExecutor(nth) as ex:
+        list(ex.map(part, range(nth)))
+
+
+def cpu_leg(s, fdk, results, chosen, raw, args):
+    """CPU baseline and HU RMSE on a bounded sample: the views that 2 slices
+    need and those 2 slices of 512 x 512.
+
+    * Timed (``cpu_baseline``): the reference's own CPU functions (oracle/_ref,
+      compiled from the reference sources) for the rebinning and the
+      backprojection, on this host's threads (views split for the rebinning,
+      runs of y rows for the backprojection), and the oracle's port of the
+      MATLAB CPU filter (the reference has no C/C++ CPU filter), each scaled per
+      view / per voxel-update to the whole workload.  Without oracle/_ref the
+      oracle's port of all three stages is timed instead (kind "port").
+    * HU RMSE: the GPU volumes' sampled slices against the FP64 oracle chain."""
     import numpy as np
     import torch
 
@@ -280,7 +632,6 @@ def cpu_leg(s, fdk, results, raw, args):
     p = fdk.params
     W = G.window_half_width(p.N_2pi, p.HSCoef)
     zs = np.linspace(p.ZN * 0.45, p.ZN * 0.55, args.cpu_slices).astype(int)
-    # views needed: windows of the sampled slices plus the rebinning's fan margin
     s0 = [math.ceil((2 * 3.14159265358979 * (z - p.ZNC) * p.dz / p.h - p.BetaS) / p.DeltaFai - 0.5)
           for z in zs]
     margin = int(math.ceil(math.asin(min(1.0, 1.5 * math.tan(s.DecAngle / 2))) / p.DeltaFai)) + 4
@@ -288,11 +639,18 @@ def cpu_leg(s, fdk, results, raw, args):
     v_hi = min(fdk.ViewN, max(s0) + W + margin + 1)
     a = G.rebin_args(s.geom)
     raw_np = raw[v_lo:v_hi].cpu().numpy()
-    log(f"cpu leg: views [{v_lo}, {v_hi}), slices {zs.tolist()}, {nth} threads")
+    use_ref = oracle.ref_available()
+    log(f"cpu leg: views [{v_lo}, {v_hi}), slices {zs.tolist()}, {nth} threads, "
+        f"{'reference library' if use_ref else 'oracle port'}")
 
-    t = time.perf_counter()
+    # the FP64 oracle chain (HU reference): port, OpenMP
     reb = oracle.rebin_f32(raw_np, a.YLC0, a.dYA, a.DeltaTheta, a.PLC0, a.DeltaT, a.DeltaFai,
                            a.SO)
+    t = time.perf_counter()
+    if use_ref:
+        _ref_rebin_threads(oracle, raw_np, a, nth)
+    else:
+        oracle.rebin_f32(raw_np, a.YLC0, a.dYA, a.DeltaTheta, a.PLC0, a.DeltaT, a.DeltaFai, a.SO)
     t_reb = time.perf_counter() - t
     fa = G.filter_args(s)
     t = time.perf_counter()
@@ -300,13 +658,10 @@ def cpu_leg(s, fdk, results, raw, args):
                              float(fa.ZLC0), float(fa.dYL), float(fa.dZL), float(fa.SO),
                              nthreads=nth)
     t_filt = time.perf_counter() - t
-    # backproject the sampled slices from the view sub-range: shift BetaS and
-    # the slice window by v_lo (views are independent).
     sub = G.BPParams(**{**p.__dict__, "PN": v_hi - v_lo,
                         "BetaS": p.BetaS + v_lo * p.DeltaFai})
     w = oracle.fdk_weights(p.N_2pi, p.HSCoef, p.delta)
     ref = np.zeros((p.YN, p.XN, p.ZN))
-    t = time.perf_counter()
     for z in zs:
         oracle.fdk_core(filt, w, N_2pi=sub.N_2pi, XN=sub.XN, YN=sub.YN, ZN=sub.ZN, XNC=sub.XNC,
                         YNC=sub.YNC, ZNC=sub.ZNC, dx=sub.dx, dy=sub.dy, dz=sub.dz, YLC=sub.YLC,
@@ -314,32 +669,133 @@ def cpu_leg(s, fdk, results, raw, args):
                         DeltaFai=sub.DeltaFai, N_pi=sub.N_pi, HSCoef=sub.HSCoef, SO=sub.SO,
                         SD=sub.SD, k1=sub.k1, zi_range=(int(z), int(z) + 1), nthreads=nth,
                         out=ref)
+    # timed backprojection of the sampled slices
+    ref_check = None
+    t = time.perf_counter()
+    for z in zs:
+        if use_ref:
+            one = oracle.ref_fdk_core(filt, w, N_2pi=sub.N_2pi, XN=sub.XN, YN=sub.YN, ZN=1,
+                                      XNC=sub.XNC, YNC=sub.YNC, ZNC=sub.ZNC - int(z), dx=sub.dx,
+                                      dy=sub.dy, dz=sub.dz, YLC=sub.YLC, ZLC=sub.ZLC,
+                                      dYL=sub.dYL, dZL=sub.dZL, h=sub.h, BetaS=sub.BetaS,
+                                      DeltaFai=sub.DeltaFai, N_pi=sub.N_pi, HSCoef=sub.HSCoef,
+                                      SO=sub.SO, SD=sub.SD, k1=sub.k1, nthreads=nth)
+            d = float(np.abs(one[:, :, 0] - ref[:, :, z]).max() /
+                      max(np.abs(ref[:, :, z]).max(), 1e-300))
+            ref_check = d if ref_check is None else max(ref_check, d)
+        else:
+            oracle.fdk_core(filt, w, N_2pi=sub.N_2pi, XN=sub.XN, YN=sub.YN, ZN=sub.ZN,
+                            XNC=sub.XNC, YNC=sub.YNC, ZNC=sub.ZNC, dx=sub.dx, dy=sub.dy,
+                            dz=sub.dz, YLC=sub.YLC, ZLC=sub.ZLC, dYL=sub.dYL, dZL=sub.dZL,
+                            h=sub.h, BetaS=sub.BetaS, DeltaFai=sub.DeltaFai, N_pi=sub.N_pi,
+                            HSCoef=sub.HSCoef, SO=sub.SO, SD=sub.SD, k1=sub.k1,
+                            zi_range=(int(z), int(z) + 1), nthreads=nth, out=np.zeros_like(ref))
     t_bp = time.perf_counter() - t
     nv = v_hi - v_lo
     upd_sample = len(zs) * p.XN * p.YN * (2 * W + 1)
     t_total = (t_reb / nv * fdk.ViewN + t_filt / nv * fdk.ViewN
                + t_bp / upd_sample * fdk.updates)
+    kind = "reference" if use_ref else "port"
+    stages_desc = ("reference CPU rebinning and backprojection (oracle/_ref, compiled from the "
+                   "reference sources) with the port of the MATLAB CPU filter" if use_ref
+                   else "oracle port of all three stages")
     cpu = {"value": round(fdk.updates / t_total / 1e9, 5), "unit": "GVoxel-updates/s",
-           "cores": nth, "kind": "port",
-           "sample": (f"oracle (rebin FP32, filter FP64, backprojection FP64) on views "
-                      f"[{v_lo},{v_hi}) and {len(zs)} slices of 512x512; measured "
-                      f"{t_reb:.2f}+{t_filt:.2f}+{t_bp:.2f} s, scaled per view / per "
-                      f"voxel-update to the whole workload ({t_total:.0f} s)")}
-    # HU RMSE (1000/mu_w) RMSE(mu) inside the FOV cylinder on the sampled slices
+           "cores": nth, "kind": kind,
+           "sample": (f"{stages_desc} on views [{v_lo},{v_hi}) and {len(zs)} slices of "
+                      f"512x512, {nth} threads; measured {t_reb:.2f}+{t_filt:.2f}+{t_bp:.2f} s, "
+                      f"scaled per view / per voxel-update to the whole workload "
+                      f"({t_total:.0f} s); single-thread figure: python bench.py --workload cpu"),
+           "stage_s_scaled": {"rebin": round(t_reb / nv * fdk.ViewN, 2),
+                              "filter": round(t_filt / nv * fdk.ViewN, 2),
+                              "backproject": round(t_bp / upd_sample * fdk.updates, 2)}}
+    if ref_check is not None:
+        cpu["reference_vs_oracle_max_rel"] = ref_check
     yy, xx = np.meshgrid((np.arange(p.YN) - p.YNC) * p.dy, (np.arange(p.XN) - p.XNC) * p.dx,
                          indexing="ij")
     mask = (xx ** 2 + yy ** 2) <= (0.98 * 250.0) ** 2
     hu = {}
-    for name, res in results.items():
-        vol = res["fdk"].run(raw)
-        torch.cuda.synchronize()
+    vols = {name: res["fdk"].run(raw) for name, res in results.items()}
+    if chosen.get("recon") is not None:
+        chosen["recon"].replay()
+        vols["chosen"] = torch.cat(chosen["recon"].volumes, dim=2)
+    torch.cuda.synchronize()
+    for name, vol in vols.items():
         g = vol[:, :, torch.as_tensor(zs, device=vol.device)].cpu().numpy().astype(np.float64)
         r = ref[:, :, zs]
         diff = (g - r)[mask]
         hu[name] = round(float(1000.0 / 0.02 * np.sqrt(np.mean(diff ** 2))), 4)
+        hu[name + "_max_abs"] = round(float(1000.0 / 0.02 * np.abs(diff).max()), 4)
         hu[name + "_rel_l2"] = float(np.linalg.norm(g - r) / np.linalg.norm(r))
-    log(f"cpu leg done: {cpu['value']} GVU/s, HU RMSE {hu}")
+    log(f"cpu leg done: {cpu['value']} GVU/s ({kind}), HU RMSE {hu}")
     return cpu, hu
+
+
+def bench_cpu(args):
+    """The reference's CPU path on the 368 x 16 Shepp-Logan scan (720
+    views/rotation, pitch 1.0, 3 rotations) into 256 x 256 x 32, in full:
+    single-threaded (as the reference ships) and on this host's threads.
+    Rebinning and backprojection are the reference's own CPU functions
+    (oracle/_ref); the filter is the oracle's port of the MATLAB CPU branch."""
+    import numpy as np
+
+    import oracle
+    from analyticalreconbigpatient_amd import geometry as G
+    from analyticalreconbigpatient_amd import phantom
+
+    wl = G.shepp_logan_small()
+    s = wl.setup()
+    dz = 1.0            # 1 mm slices centred in the helix (the config's 32 slices)
+    s.geom.Obj[6] = G.centred_ZC(wl.hdr, wl.zn, dz) + 0.137
+    p = G.bp_params(G.bp_scan_geom(s))
+    z_mid = ((wl.zn - 1) * 0.5 - p.ZNC) * dz
+    E = phantom.shepp_logan(100.0, 30.0, z_mid)
+    raw = phantom.project(E, s, wl.hdr).cpu().numpy()
+    a = G.rebin_args(s.geom)
+    fa = G.filter_args(s)
+    w = oracle.fdk_weights(p.N_2pi, p.HSCoef, p.delta)
+    use_ref = oracle.ref_available()
+    nth = host_threads()
+    updates = G.voxel_updates(p, dz)
+    out = {"metric": "GVoxel-updates/s of the CPU path (368x16x2160 -> 256x256x32, 1 mm)",
+           "unit": "GVoxel-updates/s", "workload": wl.name, "voxel_updates": updates,
+           "kind": "reference" if use_ref else "port", "runs": {}}
+    for threads in sorted({1, nth}):
+        t = time.perf_counter()
+        if use_ref:
+            if threads == 1:
+                reb = oracle.ref_rebin_f32(raw, a.YLC0, a.dYA, a.DeltaTheta, a.PLC0, a.DeltaT,
+                                           a.DeltaFai, a.SO)
+            else:
+                _ref_rebin_threads(oracle, raw, a, threads)
+                reb = None
+        else:
+            reb = oracle.rebin_f32(raw, a.YLC0, a.dYA, a.DeltaTheta, a.PLC0, a.DeltaT,
+                                   a.DeltaFai, a.SO)
+        t_reb = time.perf_counter() - t
+        if reb is None:
+            reb = oracle.ref_rebin_f32(raw, a.YLC0, a.dYA, a.DeltaTheta, a.PLC0, a.DeltaT,
+                                       a.DeltaFai, a.SO)
+        t = time.perf_counter()
+        filt = oracle.filter_f64(np.ascontiguousarray(reb.transpose(0, 2, 1)), float(fa.PLC0),
+                                 float(fa.ZLC0), float(fa.dYL), float(fa.dZL), float(fa.SO),
+                                 nthreads=threads)
+        t_filt = time.perf_counter() - t
+        t = time.perf_counter()
+        if use_ref:
+            vol = oracle.ref_fdk_from_params(filt, p, dz=dz, w=w, nthreads=threads)
+        else:
+            vol = oracle.fdk_from_params(filt, p, dz=dz, w=w, nthreads=threads)
+        t_bp = time.perf_counter() - t
+        total = t_reb + t_filt + t_bp
+        out["runs"][f"{threads}_threads"] = {
+            "threads": threads, "s": round(total, 2),
+            "stages_s": {"rebin": round(t_reb, 3), "filter": round(t_filt, 3),
+                         "backproject": round(t_bp, 3)},
+            "value": round(updates / total / 1e9, 5), "volume_max": float(np.abs(vol).max())}
+        log(f"cpu {threads} threads: {out['runs'][f'{threads}_threads']}")
+    out["host"] = {"os_cpu_count": os.cpu_count(), "OMP_NUM_THREADS":
+                   os.environ.get("OMP_NUM_THREADS")}
+    print(json.dumps(out), flush=True)
 
 
 def bench_batch(args):

@@ -297,6 +297,22 @@ int arecon_project_ellipsoids_dev(float* d_out, const float* ellipsoids, int nE,
                                   double h, double YLC0, double dYA, double ZLC0, double dZL,
                                   void* stream);
 
+/* Bandwidth probes of the backprojector's roofs (diagnostics, no counterpart in
+ * the reference).  arecon_probe_lds: `blocks` workgroups of 256 lanes, each
+ * lane reading bytes_per_lane (8 or 16) bytes of LDS `iters` times (a multiple
+ * of 4) at conflict-free addresses.  arecon_probe_l2: each lane reads 16 bytes
+ * `iters` times at scattered 16-byte offsets of d_src (span_bytes long; choose
+ * it small enough to stay in L2).  d_sink holds blocks * 256 floats.  Both
+ * only launch (time them with events on `stream`) and report the bytes they
+ * read in *bytes_read. */
+int arecon_probe_lds(float* d_sink, int bytes_per_lane, int iters, int blocks,
+                     double* bytes_read, void* stream);
+int arecon_probe_l2(float* d_sink, const void* d_src, long long span_bytes, int iters,
+                    int blocks, double* bytes_read, void* stream);
+/* Streaming copy of `bytes` (a multiple of 16) from d_src to d_dst: the
+ * reference kernel for calibrating HBM byte counters (reads and writes `bytes`). */
+int arecon_probe_copy(void* d_dst, const void* d_src, long long bytes, void* stream);
+
 /* Builds the reference's weight table w[N_2pi]
  * (src/Parallel_FDK_Helical_3DWeighting.cpp:156-170) on the host. */
 int arecon_fdk_weights(double* w, int N_2pi, double HSCoef, int delta);

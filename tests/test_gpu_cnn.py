@@ -260,3 +260,23 @@ def test_training_lowers_the_loss(gpu):
     losses = [float(net.train_step(x0, target, lr=2e-3)) for _ in range(60)]
     print("loss", losses[0], losses[-1])
     assert losses[-1] < 0.5 * losses[0]
+
+
+def test_training_is_deterministic(gpu):
+    """Weight and bias gradients are summed in a fixed order (per-wavefront and
+    per-block partials, then an ordered reduction; no floating-point atomics),
+    so two training runs from the same seed end with the same bits."""
+    g = torch.Generator().manual_seed(7)
+    B, D, H, W = 2, 16, 16, 32
+    x0 = torch.randn(B, D, H, W, generator=g).cuda() * 0.1
+    target = (x0 + 0.05 * torch.randn(B, D, H, W, generator=g).cuda()).contiguous()
+    runs = []
+    for _ in range(2):
+        net = cnn.Net(seed=3)
+        losses = [float(net.train_step(x0, target, lr=2e-3)) for _ in range(15)]
+        torch.cuda.synchronize()
+        runs.append((losses, {n: (L.w.clone(), L.b.clone()) for n, L in net.layers.items()}))
+    assert runs[0][0] == runs[1][0]
+    for n in runs[0][1]:
+        assert torch.equal(runs[0][1][n][0], runs[1][1][n][0]), n
+        assert torch.equal(runs[0][1][n][1], runs[1][1][n][1]), n
