@@ -215,27 +215,44 @@ def smooth_mask(target: torch.Tensor, tol: float = 0.02) -> torch.Tensor:
     return (target - mean).abs() <= tol
 
 
+def far_mask(target: torch.Tensor, tol: float = 0.02, margin: int = 2) -> torch.Tensor:
+    """Voxels of ``smooth_mask`` more than ``margin`` voxels from every voxel
+    that is not (no phantom edge in the (2 margin + 1)^3 neighbourhood).  The
+    3x3x3 criterion of smooth_mask still admits voxels one or two voxels from an
+    edge, where the reconstruction's error is mostly the blur of its resolution
+    (its point spread reaches 2-3 voxels); past that margin the error is
+    artifact: streaks, cone-beam (windmill) patterns, bias and cupping."""
+    sm = smooth_mask(target, tol)
+    edge = (~sm).float()[None, None]
+    near = torch.nn.functional.max_pool3d(edge, 2 * margin + 1, 1, margin)[0, 0] > 0
+    return sm & ~near
+
+
 def evaluate(net: Net, pairs, spec: PairSpec, bound: float = 0.3):
     """RMSE in HU against the phantom inside the support, for the plain FDK
     and the corrected volume (with the fallback), summed over the pairs; the
     ``artifact`` figures cover only the voxels of ``smooth_mask``."""
     m = fov_mask(spec, pairs[0][0].device)
-    se = {"fdk": 0.0, "cnn": 0.0, "fdk_s": 0.0, "cnn_s": 0.0}
-    n = ns = 0
+    se = {"fdk": 0.0, "cnn": 0.0, "fdk_s": 0.0, "cnn_s": 0.0, "fdk_f": 0.0, "cnn_f": 0.0}
+    n = ns = nf = 0
     fb = []
     for x0, t, *_ in pairs:
         y, frac = net.correct(x0[None], bound=bound)
         y = y[0]
         ms = m & smooth_mask(t)
-        se["fdk"] += float(((x0 - t)[m] ** 2).sum())
-        se["cnn"] += float(((y - t)[m] ** 2).sum())
-        se["fdk_s"] += float(((x0 - t)[ms] ** 2).sum())
-        se["cnn_s"] += float(((y - t)[ms] ** 2).sum())
+        mf = m & far_mask(t)
+        for tag, mk in (("", m), ("_s", ms), ("_f", mf)):
+            se["fdk" + tag] += float(((x0 - t)[mk] ** 2).sum())
+            se["cnn" + tag] += float(((y - t)[mk] ** 2).sum())
         n += int(m.sum())
         ns += int(ms.sum())
+        nf += int(mf.sum())
         fb.append(frac)
-    r = {k: 1000.0 * math.sqrt(v / (ns if k.endswith("_s") else n)) for k, v in se.items()}
+    cnt = {"": n, "_s": ns, "_f": nf}
+    r = {k: 1000.0 * math.sqrt(v / max(cnt[k[3:]], 1)) for k, v in se.items()}
     return {"rmse_fdk_hu": r["fdk"], "rmse_cnn_hu": r["cnn"], "reduction": 1 - r["cnn"] / r["fdk"],
             "artifact_rmse_fdk_hu": r["fdk_s"], "artifact_rmse_cnn_hu": r["cnn_s"],
             "artifact_reduction": 1 - r["cnn_s"] / r["fdk_s"],
+            "artifact_far_rmse_fdk_hu": r["fdk_f"], "artifact_far_rmse_cnn_hu": r["cnn_f"],
+            "artifact_far_reduction": 1 - r["cnn_f"] / r["fdk_f"],
             "fallback_fraction": float(np.mean(fb))}

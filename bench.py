@@ -67,10 +67,12 @@ def parse():
                          "cpu: the reference CPU path on the 368x16x2160 scan, 1 thread and "
                          "all threads")
     ap.add_argument("--cnn-pairs", type=int, default=200, help="training phantoms (cnn)")
-    ap.add_argument("--cnn-clinical-pairs", type=int, default=300,
+    ap.add_argument("--cnn-clinical-pairs", type=int, default=96,
                     help="training crops of the 888x64 scan (cnn)")
-    ap.add_argument("--cnn-steps", type=int, default=10000,
+    ap.add_argument("--cnn-steps", type=int, default=4000,
                     help="training steps of each network (cnn)")
+    ap.add_argument("--cnn-seeds", type=int, default=3,
+                    help="network seeds of the 888x64 correction (cnn)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch workload)")
     return ap.parse_args()
 
@@ -875,16 +877,27 @@ def bench_batch(args):
 
 
 def bench_cnn(args):
-    """The learned correction, one network per scanner, each trained on pairs
-    the method makes for that scanner: (1) seeded random-ellipsoid phantoms on
-    the 368x16 scan (720 views/rotation, pitch U[0.5, 1.5]) reconstructed by
-    the plain pipeline into 192x192x64 at 1 mm; (2) 192x192x64 crops of seeded
-    random bodies on the 888x64 scan (984 views/rotation, pitch U[0.5, 1.5],
-    0.977 x 0.977 x 1.25 mm).  64^3 patches, batch 4, Adam.  Reported: RMSE
-    against the phantom on held-out pairs of each scanner, over all voxels and
-    over the voxels away from phantom edges (artifact RMSE), and inference of
-    the 888x64 network on a 512x512x256 reconstruction of a random body
-    (pitch 0.984) with the same two RMSEs."""
+    """The learned correction: the 3-level 32/64/128 network
+    with its BF16 MFMA kernels, trained on 64^3 patches of pairs the method
+    makes, one network per scanner:
+
+    * 368x16: seeded random-ellipsoid phantoms (720 views/rotation, pitch
+      U[0.5, 1.5]) reconstructed by the plain pipeline into 192x192x64 at 1 mm,
+      evaluated on held-out phantoms;
+    * 888x64: 192x192x64 crops of seeded random bodies (984 views/rotation, pitch
+      U[0.5, 1.5], 0.977 x 0.977 x 1.25 mm), trained from --cnn-seeds network
+      seeds, each applied with its plain-FDK fallback to a 512x512x256
+      reconstruction of another body (pitch 0.984).
+
+    Reported per network: RMSE against the phantom over the field, and the
+    artifact RMSE over the voxels away from the phantom's edges, with two edge
+    margins: phantom within 20 HU of its 3x3x3 neighbourhood mean ("artifact"),
+    and additionally more than 2 voxels from any voxel that fails it
+    ("artifact_far": past the reach of the reconstruction's blur).  The 512 x
+    512 x 256 figures are given per seed and as mean and sample standard
+    deviation over the seeds.  Training is deterministic for a given seed."""
+    import statistics
+
     import torch
 
     from analyticalreconbigpatient_amd import cnn, learned
@@ -906,22 +919,8 @@ def bench_cnn(args):
     t_data = time.perf_counter() - t0
     log(f"{len(pairs)} + {len(cpairs)} training pairs, {len(held)} + {len(cheld)} held out "
         f"in {t_data:.1f} s")
-    nets, hists, t_train = {}, {}, {}
-    for name, train_pairs in (("368x16", pairs), ("888x64", cpairs)):
-        net = cnn.Net(seed=0)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        hists[name] = learned.train(net, train_pairs, steps=args.cnn_steps, batch=4, patch=64,
-                                    lr=1e-3, log=log, log_every=1000)
-        torch.cuda.synchronize()
-        t_train[name] = time.perf_counter() - t0
-        nets[name] = net
-    ev = learned.evaluate(nets["368x16"], held, spec, bound=0.3)
-    cev = learned.evaluate(nets["888x64"], cheld, cspec, bound=0.3)
-    log(f"held-out 368x16: {rnd(ev)}; held-out 888x64 crops: {rnd(cev)}")
 
-    # inference on 512x512x256 (888x64 scan, 984 views/rotation, pitch 0.984)
-    net = nets["888x64"]
+    # the 512 x 512 x 256 volume (888x64 scan, 984 views/rotation, pitch 0.984)
     zn, dz = 256, 1.25
     hdr = G._header(888, 64, 984, 6, 0.984)
     wl = G.Workload("body 888x64x5904 -> 512x512x256", hdr, 512, 512, zn, 250.0,
@@ -939,44 +938,84 @@ def bench_cnn(args):
     ys = (torch.arange(p.YN, device=dev, dtype=torch.float32) - p.YNC) * p.dy
     zs = (torch.arange(p.ZN, device=dev, dtype=torch.float32) - p.ZNC) * dz
     target = learned.voxelize_gpu(E, xs, ys, zs, ss=2) / learned.MU_W - 1.0
-    y, frac = net.correct(x0, bound=0.3)          # first call sizes the workspace
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    y, frac = net.correct(x0, bound=0.3)
-    ev1.record()
-    ev1.synchronize()
-    t_inf = ev0.elapsed_time(ev1)
-    r2 = (xs[None, :] ** 2 + ys[:, None] ** 2 <= (0.95 * 250.0) ** 2)[:, :, None].expand_as(target)
-    sm = r2 & learned.smooth_mask(target)
+    field = (xs[None, :] ** 2 + ys[:, None] ** 2 <= (0.95 * 250.0) ** 2)[:, :, None]
+    field = field.expand_as(target)
+    masks = {"": field, "artifact_": field & learned.smooth_mask(target),
+             "artifact_far_": field & learned.far_mask(target)}
 
     def rmse(a, m):
         return 1000 * float(((a - target)[m] ** 2).mean().sqrt())
 
-    inf = {"ms": round(t_inf, 2), "rmse_fdk_hu": rmse(x0[0], r2), "rmse_cnn_hu": rmse(y[0], r2),
-           "artifact_rmse_fdk_hu": rmse(x0[0], sm), "artifact_rmse_cnn_hu": rmse(y[0], sm)}
-    inf["reduction"] = 1 - inf["rmse_cnn_hu"] / inf["rmse_fdk_hu"]
-    inf["artifact_reduction"] = 1 - inf["artifact_rmse_cnn_hu"] / inf["artifact_rmse_fdk_hu"]
-    inf = rnd(inf)
-    inf.update({"fallback_fraction": round(frac, 4), "bp_voxel_updates": fdk.updates})
-    log(f"512x512x256: {inf}")
-    out = {"metric": "artifact RMSE vs phantom, learned correction vs plain FDK (held-out "
-                     "phantoms)", "value": round(ev["artifact_reduction"], 4),
-           "unit": "fraction", "higher_is_better": True, "n_gpus": 1,
+    def big_volume(net):
+        y, frac = net.correct(x0, bound=0.3)          # first call sizes the workspace
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        y, frac = net.correct(x0, bound=0.3)
+        e1.record()
+        e1.synchronize()
+        out = {"ms": e0.elapsed_time(e1), "fallback_fraction": frac}
+        for tag, m in masks.items():
+            f, c = rmse(x0[0], m), rmse(y[0], m)
+            out.update({f"{tag}rmse_fdk_hu": f, f"{tag}rmse_cnn_hu": c,
+                        f"{tag}reduction": 1 - c / f})
+        return out
+
+    t0 = time.perf_counter()
+    net = cnn.Net(seed=0)
+    hist = learned.train(net, pairs, steps=args.cnn_steps, batch=4, patch=64, lr=1e-3, log=log,
+                         log_every=1000)
+    torch.cuda.synchronize()
+    t368 = time.perf_counter() - t0
+    ev = learned.evaluate(net, held, spec, bound=0.3)
+    log(f"368x16 held out: {rnd(ev)}")
+    runs = []
+    for seed in range(args.cnn_seeds):
+        net = cnn.Net(seed=seed)
+        t0 = time.perf_counter()
+        h = learned.train(net, cpairs, steps=args.cnn_steps, batch=4, patch=64, lr=1e-3,
+                          seed=seed, log=log, log_every=1000)
+        torch.cuda.synchronize()
+        tt = time.perf_counter() - t0
+        cev = learned.evaluate(net, cheld, cspec, bound=0.3)
+        inf = big_volume(net)
+        runs.append({"seed": seed, "train_s": round(tt, 2), "loss_first_last": [h[0][1], h[-1][1]],
+                     "held_out_crops": rnd(cev), "volume_512x512x256": rnd(inf)})
+        log(f"888x64 seed {seed}: crops {rnd(cev)}; 512x512x256 {rnd(inf)}")
+
+    def stat(key):
+        v = [r["volume_512x512x256"][key] for r in runs]
+        return {"mean": round(statistics.mean(v), 4),
+                "std": round(statistics.stdev(v), 4) if len(v) > 1 else 0.0,
+                "min": round(min(v), 4), "max": round(max(v), 4)}
+
+    summary = {k: stat(k) for k in ("reduction", "artifact_reduction", "artifact_far_reduction",
+                                    "rmse_cnn_hu", "artifact_rmse_cnn_hu",
+                                    "artifact_far_rmse_cnn_hu", "ms")}
+    base = runs[0]["volume_512x512x256"]
+    out = {"metric": "artifact RMSE vs phantom on a 512x512x256 reconstruction, learned "
+                     "correction vs plain FDK (mean over network seeds)",
+           "value": summary["artifact_far_reduction"]["mean"], "unit": "fraction",
+           "higher_is_better": True, "n_gpus": 1,
            "dtype": "bf16 weights/activations, f32 accumulate",
-           "data": "synthetic: seeded random-ellipsoid phantoms, exact projections, plain-FDK "
-                   "reconstructions",
-           "config": {"workload": "one network per scanner: 368x16 scans (pitch U[0.5,1.5]) -> "
-                                  "192x192x64; 888x64 crops -> 192x192x64; 64^3 patches; infer "
-                                  "512x512x256 (888x64)",
+           "data": "synthetic: seeded random-ellipsoid phantoms and bodies, exact projections, "
+                   "plain-FDK reconstructions",
+           "config": {"workload": "3-level 32/64/128 network, 64^3 patches, batch 4; 368x16 "
+                                  "phantoms -> 192x192x64; 888x64 body crops -> 192x192x64; "
+                                  "infer 512x512x256 (888x64, pitch 0.984)",
                       "train_pairs_368x16": len(pairs), "train_pairs_888x64": len(cpairs),
-                      "steps": args.cnn_steps, "batch": 4,
-                      "artifact_voxels": "phantom within 20 HU of its 3x3x3 neighbourhood mean"},
-           "held_out_368x16": rnd(ev), "held_out_888x64_crops": rnd(cev),
-           "loss_first_last": {k: [h[0][1], h[-1][1]] for k, h in hists.items()},
-           "train_s": {k: round(v, 2) for k, v in t_train.items()},
-           "ms_per_train_step": round(1e3 * t_train["368x16"] / args.cnn_steps, 2),
-           "data_s": round(t_data, 2), "inference_512x512x256": inf}
+                      "steps": args.cnn_steps, "seeds": args.cnn_seeds,
+                      "artifact_voxels": "phantom within 20 HU of its 3x3x3 neighbourhood mean",
+                      "artifact_far_voxels": "artifact voxels more than 2 voxels from any "
+                                             "voxel that is not one"},
+           "volume_512x512x256_plain_fdk": {k: round(v, 4) for k, v in base.items()
+                                            if "fdk" in k},
+           "volume_512x512x256_over_seeds": summary,
+           "runs_888x64": runs,
+           "held_out_368x16": rnd(ev), "loss_first_last_368x16": [hist[0][1], hist[-1][1]],
+           "train_s_368x16": round(t368, 2),
+           "ms_per_train_step": round(1e3 * t368 / args.cnn_steps, 2),
+           "data_s": round(t_data, 2)}
     print(json.dumps(out), flush=True)
 
 
