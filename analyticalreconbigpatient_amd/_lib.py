@@ -99,6 +99,7 @@ SIGNATURES = {
     "arecon_probe_lds": (_i, [_p, _i, _i, _i, _dp, _p]),
     "arecon_probe_l2": (_i, [_p, _p, C.c_longlong, _i, _i, _dp, _p]),
     "arecon_probe_copy": (_i, [_p, _p, C.c_longlong, _p]),
+    "arecon_probe_valu": (_i, [_p, _i, _i, _dp, _p]),
     "arecon_project_ellipsoids_dev": (_i, [_p, _fp, _i, _i, _i, _i, _d, _d, _d, _d, _d, _d,
                                            _d, _d, _d, _p]),
 }

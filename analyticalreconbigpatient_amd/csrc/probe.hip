@@ -16,6 +16,18 @@
 //                aligned offsets, as the gathering kernel's row-pair loads do.
 //
 // Each returns the bytes it reads; the caller times the launch with events.
+//
+//  probe_copy    a streaming copy (calibrates the HBM byte counters);
+//  probe_valu<F> issue rate of one VALU instruction form: every lane runs
+//                eight independent chains of it, so the rate is the form's
+//                throughput (wavefront instructions per CU per clock) with
+//                enough waves to hide its latency.  The forms are the ones the
+//                backprojector's inner loop is built from or was tried with:
+//                FP32 FMA with VGPR operands and with a scalar operand (the
+//                per-slice table value), the mixed FMA that blends FP16
+//                records, its FP16-result forms (mixlo / mixhi), the packed FP16
+//                dot product and FMA, the floor conversion, the shift-add of
+//                the LDS address, and the reciprocal of the 3D weight.
 #include "arecon_internal.h"
 
 #include <algorithm>
@@ -136,5 +148,82 @@ extern "C" int arecon_probe_copy(void* d_dst, const void* d_src, long long bytes
                            static_cast<hipStream_t>(stream), static_cast<float4*>(d_dst),
                            static_cast<const float4*>(d_src), n16);
         ARECON_HIP(hipGetLastError());
+    });
+}
+
+namespace arecon {
+namespace {
+
+enum { VF_FMAC = 0, VF_FMAC_SGPR, VF_FMA_MIX, VF_FMA_MIXLO, VF_DOT2, VF_PK_FMA_F16, VF_CVT_FLR,
+       VF_LSHL_ADD, VF_RCP, VF_COUNT };
+constexpr int kValuIters = 4096;
+
+template <int F>
+__global__ __launch_bounds__(256) void probe_valu(float* __restrict__ sink, float seed, float sv) {
+    float a[8];
+    unsigned u[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        a[i] = seed + threadIdx.x * 0.001f + i;
+        u[i] = __float_as_uint(a[i]) ^ 0x3c003c00u;
+    }
+    const float fr = seed * 0.5f;
+    const unsigned w = 0x3c003800u;
+    for (int it = 0; it < kValuIters; ++it) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if constexpr (F == VF_FMAC)
+                asm volatile("v_fmac_f32 %0, %1, %2" : "+v"(a[i]) : "v"(fr), "v"(a[(i + 1) & 7]));
+            if constexpr (F == VF_FMAC_SGPR)
+                asm volatile("v_fmac_f32 %0, %1, %2" : "+v"(a[i]) : "s"(sv), "v"(a[(i + 1) & 7]));
+            if constexpr (F == VF_FMA_MIX)
+                asm volatile("v_fma_mix_f32 %0, %1, %2, %2 op_sel:[0,1,0] op_sel_hi:[0,1,1]"
+                             : "=v"(a[i]) : "v"(fr), "v"(u[i]));
+            if constexpr (F == VF_FMA_MIXLO)
+                asm volatile("v_fma_mixlo_f16 %0, %1, %2, %2 op_sel:[0,1,0] op_sel_hi:[0,1,1]"
+                             : "+v"(u[i]) : "v"(fr), "v"(u[(i + 3) & 7]));
+            if constexpr (F == VF_DOT2)
+                asm volatile("v_dot2_f32_f16 %0, %1, %2, %0" : "+v"(a[i]) : "v"(u[i]), "v"(w));
+            if constexpr (F == VF_PK_FMA_F16)
+                asm volatile("v_pk_fma_f16 %0, %1, %2, %0" : "+v"(u[i]) : "v"(w), "v"(u[(i + 5) & 7]));
+            if constexpr (F == VF_CVT_FLR)
+                asm volatile("v_cvt_flr_i32_f32 %0, %1" : "=v"(u[i]) : "v"(a[i]));
+            if constexpr (F == VF_LSHL_ADD)
+                asm volatile("v_lshl_add_u32 %0, %1, 7, %2"
+                             : "=v"(u[i]) : "v"(u[(i + 1) & 7]), "v"(u[(i + 2) & 7]));
+            if constexpr (F == VF_RCP)
+                asm volatile("v_rcp_f32 %0, %1" : "=v"(a[i]) : "v"(a[(i + 1) & 7]));
+        }
+    }
+    float t = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) t += a[i] + __uint_as_float(u[i] & 0x3fffffffu);
+    sink[(long long)blockIdx.x * blockDim.x + threadIdx.x] = t;
+}
+
+}  // namespace
+}  // namespace arecon
+
+extern "C" int arecon_probe_valu(float* d_sink, int form, int blocks, double* instructions,
+                                 void* stream) {
+    return guarded([&] {
+        ARECON_REQUIRE(d_sink && instructions, "arecon_probe_valu: null pointer");
+        ARECON_REQUIRE(form >= 0 && form < VF_COUNT && blocks > 0,
+                       "arecon_probe_valu: unknown form or no blocks");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const dim3 g(blocks), b(256);
+        switch (form) {
+        case VF_FMAC: hipLaunchKernelGGL(probe_valu<VF_FMAC>, g, b, 0, s, d_sink, 1.f, 0.5f); break;
+        case VF_FMAC_SGPR: hipLaunchKernelGGL(probe_valu<VF_FMAC_SGPR>, g, b, 0, s, d_sink, 1.f, 0.5f); break;
+        case VF_FMA_MIX: hipLaunchKernelGGL(probe_valu<VF_FMA_MIX>, g, b, 0, s, d_sink, 1.f, 0.5f); break;
+        case VF_FMA_MIXLO: hipLaunchKernelGGL(probe_valu<VF_FMA_MIXLO>, g, b, 0, s, d_sink, 1.f, 0.5f); break;
+        case VF_DOT2: hipLaunchKernelGGL(probe_valu<VF_DOT2>, g, b, 0, s, d_sink, 1.f, 0.5f); break;
+        case VF_PK_FMA_F16: hipLaunchKernelGGL(probe_valu<VF_PK_FMA_F16>, g, b, 0, s, d_sink, 1.f, 0.5f); break;
+        case VF_CVT_FLR: hipLaunchKernelGGL(probe_valu<VF_CVT_FLR>, g, b, 0, s, d_sink, 1.f, 0.5f); break;
+        case VF_LSHL_ADD: hipLaunchKernelGGL(probe_valu<VF_LSHL_ADD>, g, b, 0, s, d_sink, 1.f, 0.5f); break;
+        default: hipLaunchKernelGGL(probe_valu<VF_RCP>, g, b, 0, s, d_sink, 1.f, 0.5f); break;
+        }
+        ARECON_HIP(hipGetLastError());
+        *instructions = (double)blocks * 4.0 * kValuIters * 8.0;   // wavefront instructions
     });
 }

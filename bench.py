@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--no-slabs", action="store_true",
                     help="chosen form without slab batching (FP16 storage only)")
     ap.add_argument("--counter-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--workload", default="clinical", choices=["clinical", "batch", "cnn", "cpu"],
+    ap.add_argument("--workload", default="clinical", choices=["clinical", "batch", "cnn", "cpu", "probes"],
                     help="clinical: the 888x64x11808 -> 512x512x400 pass (default); batch: "
                          "3 scans of 4/9/15 rotations, 18 uneven slabs, batched + HIP graph; "
                          "cnn: train the learned correction and run it on 512x512x256; "
@@ -128,6 +128,9 @@ def main():
         return
     if args.workload == "cpu":
         bench_cpu(args)
+        return
+    if args.workload == "probes":
+        print(json.dumps({"probes": run_probes()}), flush=True)
         return
     bench_clinical(args)
 
@@ -327,7 +330,25 @@ def run_probes():
                                          a.numel() * 4, st), "arecon_probe_copy")
         return 2.0 * a.numel() * 4
     out["hbm_copy_TBs"] = round(timed(copy), 2)
-    del a, b2, src, sink
+    del a, b2, src
+    # VALU issue rates (wavefront instructions per CU per ns) of the instruction
+    # forms the backprojector is built from; at the ~2.26 GHz these kernels run
+    # at, 2.26 per ns is one instruction per CU per clock
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    forms = ["fma_f32", "fma_f32_sgpr_operand", "fma_mix_f32", "fma_mixlo_f16", "dot2_f32_f16",
+             "pk_fma_f16", "cvt_flr_i32_f32", "lshl_add_u32", "rcp_f32"]
+    vb = cus * 8
+    vsink = torch.empty(vb * 256, device="cuda")
+    rates = {}
+    for f, name in enumerate(forms):
+        def valu(f=f):
+            ni = C.c_double()
+            _lib.check(lib.arecon_probe_valu(C.c_void_p(vsink.data_ptr()), f, vb, C.byref(ni), st),
+                       "arecon_probe_valu")
+            return ni.value
+        rates[name] = round(timed(valu, 3) * 1e12 / cus / 1e9, 3)
+    out["valu_wave_instr_per_cu_per_ns"] = rates
+    del sink, vsink
     return out
 
 

@@ -312,6 +312,13 @@ int arecon_probe_l2(float* d_sink, const void* d_src, long long span_bytes, int 
 /* Streaming copy of `bytes` (a multiple of 16) from d_src to d_dst: the
  * reference kernel for calibrating HBM byte counters (reads and writes `bytes`). */
 int arecon_probe_copy(void* d_dst, const void* d_src, long long bytes, void* stream);
+/* Issue rate of one VALU instruction form (0 FMA, 1 FMA with a scalar operand,
+ * 2 mixed-precision FMA, 3 mixed FMA with an FP16 result, 4 packed FP16 dot
+ * product, 5 packed FP16 FMA, 6 floor conversion, 7 shift-add, 8 reciprocal):
+ * `blocks` workgroups of 256 lanes; *instructions receives the wavefront
+ * instructions issued (time the launch with events on `stream`).  d_sink holds
+ * blocks * 256 floats. */
+int arecon_probe_valu(float* d_sink, int form, int blocks, double* instructions, void* stream);
 
 /* Builds the reference's weight table w[N_2pi]
  * (src/Parallel_FDK_Helical_3DWeighting.cpp:156-170) on the host. */
