@@ -88,8 +88,7 @@ struct BPArgs {
 // XCD): on the 512x512x400 clinical grid, bands of 6 rows take 29.9 ms against
 // 31.2 ms row by row (bands of 3 to 12 rows: 30.2-30.4 ms; tile-major: 42 ms).
 constexpr int kTileBand = 6;
-__host__ __device__ __forceinline__ void wg_tile(int lb, const BPArgs& A, int& zs, int& tyb,
-                                                 int& txb) {
+__device__ __forceinline__ void wg_tile(int lb, const BPArgs& A, int& zs, int& tyb, int& txb) {
     int tile;
     if (A.order == 1) { tile = lb / A.nzsets; zs = lb - tile * A.nzsets; }
     else { zs = lb / A.ntiles; tile = lb - zs * A.ntiles; }
@@ -101,7 +100,7 @@ __host__ __device__ __forceinline__ void wg_tile(int lb, const BPArgs& A, int& z
     const int tilesy = A.ntiles / A.tilesx;
     const int band = tile / (kTileBand * A.tilesx);
     const int y0 = band * kTileBand;
-    const int bh = tilesy - y0 < kTileBand ? tilesy - y0 : kTileBand;   // rows in this band
+    const int bh = min(kTileBand, tilesy - y0);           // rows in this band
     const int idx = tile - band * kTileBand * A.tilesx;
     txb = idx / bh;
     tyb = y0 + idx - txb * bh;
@@ -1286,6 +1285,7 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     typedef Gather<T, LAYOUT> Gt;
     typedef LdsGather<T, LAYOUT> Lg;
 
+    const int lb0 = lb;
     lb += A.lb_offset;
     int zs, tyb, txb;
     wg_tile(lb, A, zs, tyb, txb);
@@ -1301,17 +1301,17 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     // Every view's footprint comes from the plan's table (bp_zs_table, built once
     // per plan and layout; kept in LDS it cost the sixth or seventh workgroup per
     // CU, and recomputed here about two VALU per update).  A workgroup with a
-    // view that does not fit returns: the z-column kernel's body runs it in a
-    // launch of its own (bp_fdk3dw_zs_direct), whose registers do not count
-    // against this kernel's occupancy (16-slice blocks: 74 VGPRs with that body
-    // inlined here, 50 without; six waves per SIMD against eight).
+    // view that does not fit runs the z-column kernel's body instead.
     const int tile = tyb * A.tilesx + txb;
     const cptr<unsigned> ftab = (cptr<unsigned>)const_ptr(
         uniform_ptr(A.zs_tab) + *(cptr<long long>)const_ptr(uniform_ptr(A.zs_boff) + zs) +
         (long long)tile * zs_table_views(pmin, pmax));
     const int staged = *(cptr<int>)const_ptr(uniform_ptr(A.zs_fits) + (zs * A.ntiles + tile));
     if (threadIdx.x == 0 && A.stats) atomicAdd(A.stats + (staged ? 0 : 1), 1ull);
-    if (!staged) return;
+    if (!staged) {
+        zc_run<ZB, 0, T, LAYOUT, INSIDE>(lb0, A, img, fp, vtab, blocks, masks, table);
+        return;
+    }
     // the last row of every slot is zero and stays zero (a view fits only when
     // its loaded rows, a multiple of four, are fewer than RMAX): rows past the
     // view's last staged row (slices outside the view's window, temp2 = 1e30,
@@ -1532,22 +1532,6 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zs(float* __restrict__ img,
     zs_run<ZB, T, LAYOUT, RMAX, INSIDE>(xcd_logical_block(), A, img, fp, vtab, blocks, masks, table);
 }
 
-// The workgroups of an LDS-staged launch whose footprints do not all fit: the
-// z-column kernel's body (gathers from global memory), one workgroup per entry
-// of the plan's list of such logical workgroups (A.lb_offset 0).
-template <int ZB, class T, int LAYOUT, bool INSIDE>
-__global__ __launch_bounds__(256) void bp_fdk3dw_zs_direct(float* __restrict__ img,
-                                                           const T* __restrict__ fp,
-                                                           const float2* __restrict__ vtab,
-                                                           const ZBlock* __restrict__ blocks,
-                                                           const uint2* __restrict__ masks,
-                                                           const float2* __restrict__ table,
-                                                           const int* __restrict__ list,
-                                                           BPArgs A) {
-    const int lb = __builtin_amdgcn_readfirstlane(list[blockIdx.x]);
-    zc_run<ZB, 0, T, LAYOUT, INSIDE>(lb, A, img, fp, vtab, blocks, masks, table);
-}
-
 // Batched z-column launch: several plans (z-slabs of one or more scans, each
 // with its own geometry, volume and filtered sinogram) in one grid.  Slab s owns
 // logical workgroups [wg_begin, wg_begin + its grid); the XCD remap runs over
@@ -1588,20 +1572,6 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zs_batch(const SlabDesc* __rest
     const SlabDesc& d = slabs[s];
     zs_run<ZB, T, LAYOUT, RMAX, INSIDE>(lb - d.wg_begin, d.A, d.img, static_cast<const T*>(d.fp),
                                         d.vtab, d.blocks, d.masks, d.table);
-}
-
-// The direct workgroups of an LDS-staged batch: list holds batch-wide logical
-// workgroups (the slab's wg_begin plus its own logical workgroup).
-template <int ZB, class T, int LAYOUT, bool INSIDE>
-__global__ __launch_bounds__(256) void bp_fdk3dw_zs_direct_batch(const SlabDesc* __restrict__ slabs,
-                                                                 int nslabs,
-                                                                 const int* __restrict__ list) {
-    const int lb = __builtin_amdgcn_readfirstlane(list[blockIdx.x]);
-    int s = 0;
-    while (s + 1 < nslabs && lb >= slabs[s + 1].wg_begin) ++s;
-    const SlabDesc& d = slabs[s];
-    zc_run<ZB, 0, T, LAYOUT, INSIDE>(lb - d.wg_begin, d.A, d.img, static_cast<const T*>(d.fp),
-                                     d.vtab, d.blocks, d.masks, d.table);
 }
 
 // FP64 path: the same math as the reference, one thread per voxel, in double.
@@ -1730,10 +1700,6 @@ struct BPPlan {
         unsigned* tab = nullptr;
         long long* boff = nullptr;
         int* fits = nullptr;
-        // logical workgroups (ascending) with a view that does not fit: run by
-        // bp_fdk3dw_zs_direct
-        int* direct = nullptr;
-        std::vector<int> direct_host;
     };
     ZsTable zs_tables[3];
     std::mutex zs_mu;
@@ -2047,7 +2013,6 @@ void bp_plan_destroy(BPPlan* P) {
         if (z.tab) (void)hipFree(z.tab);
         if (z.boff) (void)hipFree(z.boff);
         if (z.fits) (void)hipFree(z.fits);
-        if (z.direct) (void)hipFree(z.direct);
     }
     (void)hipSetDevice(cur);
     delete P;
@@ -2108,30 +2073,13 @@ static const BPPlan::ZsTable& zs_table(const BPPlan* P, hipStream_t s) {
                                t.fits, nrec);
         ARECON_HIP(hipGetLastError());
         ARECON_HIP(hipStreamSynchronize(s));
-        // the workgroups the staged kernel leaves to the direct one, in the order
-        // of the launch's logical workgroups
-        const int nwg = A.nzsets * A.ntiles;
-        std::vector<int> fits(nwg);
-        ARECON_HIP(hipMemcpy(fits.data(), t.fits, sizeof(int) * (size_t)nwg,
-                             hipMemcpyDeviceToHost));
-        for (int lb = 0; lb < nwg; ++lb) {
-            int zs, tyb, txb;
-            wg_tile(lb, A, zs, tyb, txb);
-            if (!fits[zs * A.ntiles + tyb * A.tilesx + txb]) t.direct_host.push_back(lb);
-        }
-        if (!t.direct_host.empty()) {
-            ARECON_HIP(hipMalloc(&t.direct, sizeof(int) * t.direct_host.size()));
-            ARECON_HIP(hipMemcpy(t.direct, t.direct_host.data(),
-                                 sizeof(int) * t.direct_host.size(), hipMemcpyHostToDevice));
-        }
     } catch (...) {
         if (t.tab) (void)hipFree(t.tab);
         if (t.boff) (void)hipFree(t.boff);
         if (t.fits) (void)hipFree(t.fits);
-        if (t.direct) (void)hipFree(t.direct);
         throw;
     }
-    z = std::move(t);
+    z = t;
     return z;
 }
 
@@ -2149,21 +2097,6 @@ static void launch_zs(const BPPlan* P, BPArgs a, float* img, const T* fp, hipStr
     else
         hipLaunchKernelGGL((bp_fdk3dw_zs<NZ, T, LY, RMAX, false>), dim3(grid), dim3(256), 0, s,
                            img, fp, P->vtab, P->zblocks, P->zmasks, P->vt_table, a);
-    // the launch's direct workgroups: list entries in [lb_offset, lb_offset + grid)
-    const std::vector<int>& d = t.direct_host;
-    const long long lo = std::lower_bound(d.begin(), d.end(), a.lb_offset) - d.begin();
-    const long long hi = std::lower_bound(d.begin(), d.end(), a.lb_offset + grid) - d.begin();
-    if (hi > lo) {
-        a.lb_offset = 0;                                  // list entries are absolute
-        if (a.inside)
-            hipLaunchKernelGGL((bp_fdk3dw_zs_direct<NZ, T, LY, true>), dim3((unsigned)(hi - lo)),
-                               dim3(256), 0, s, img, fp, P->vtab, P->zblocks, P->zmasks,
-                               P->vt_table, t.direct + lo, a);
-        else
-            hipLaunchKernelGGL((bp_fdk3dw_zs_direct<NZ, T, LY, false>),
-                               dim3((unsigned)(hi - lo)), dim3(256), 0, s, img, fp, P->vtab,
-                               P->zblocks, P->zmasks, P->vt_table, t.direct + lo, a);
-    }
 }
 
 template <int NZ, class T>
@@ -2304,8 +2237,6 @@ struct BPBatch {
     int nslabs = 0;
     long long grid = 0;
     SlabDesc* d_slabs = nullptr;
-    int* d_direct = nullptr;    // LDS-staged batch: its direct workgroups (batch-wide)
-    int ndirect = 0;
 };
 
 // A plan's footprint table for the LDS-staged batch kernel (built on the null
@@ -2341,7 +2272,6 @@ BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const v
                        fp_layout == ARECON_FP_QUADS || fp_layout == ARECON_FP_ROWDIFF,
                    "bp batch: unknown fp layout");
     std::vector<SlabDesc> slabs(n);
-    std::vector<int> direct;
     long long wg = 0;
     for (int i = 0; i < n; ++i) {
         const BPPlan* P = plans[i];
@@ -2370,7 +2300,6 @@ BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const v
             d.A.zs_tab = t->tab;
             d.A.zs_boff = t->boff;
             d.A.zs_fits = t->fits;
-            for (int lb : t->direct_host) direct.push_back((int)wg + lb);
         }
         d.img = imgs[i];
         d.fp = fps[i];
@@ -2401,12 +2330,6 @@ BPBatch* bp_batch_create(const BPPlan* const* plans, float* const* imgs, const v
         ARECON_HIP(hipMalloc(&B->d_slabs, sizeof(SlabDesc) * slabs.size()));
         ARECON_HIP(hipMemcpy(B->d_slabs, slabs.data(), sizeof(SlabDesc) * slabs.size(),
                              hipMemcpyHostToDevice));
-        if (!direct.empty()) {
-            ARECON_HIP(hipMalloc(&B->d_direct, sizeof(int) * direct.size()));
-            ARECON_HIP(hipMemcpy(B->d_direct, direct.data(), sizeof(int) * direct.size(),
-                                 hipMemcpyHostToDevice));
-            B->ndirect = (int)direct.size();
-        }
         ARECON_HIP(hipSetDevice(cur));
     } catch (...) {
         (void)hipSetDevice(cur);
@@ -2422,7 +2345,6 @@ void bp_batch_destroy(BPBatch* B) {
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(B->device);
     if (B->d_slabs) (void)hipFree(B->d_slabs);
-    if (B->d_direct) (void)hipFree(B->d_direct);
     (void)hipSetDevice(cur);
     delete B;
 }
@@ -2472,24 +2394,19 @@ static void batch_launch_nz(const BPBatch* B, hipStream_t s) {
     }
 }
 
-template <int NZ, class T, int LY, bool INSIDE>
-static void batch_launch_zs_ly(const BPBatch* B, hipStream_t s) {
-    constexpr int RMAX = NZ == 8 ? 21 : 40;
-    hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, LY, RMAX, INSIDE>), dim3((unsigned)B->grid),
-                       dim3(256), 0, s, B->d_slabs, B->nslabs);
-    if (B->ndirect > 0)
-        hipLaunchKernelGGL((bp_fdk3dw_zs_direct_batch<NZ, T, LY, INSIDE>),
-                           dim3((unsigned)B->ndirect), dim3(256), 0, s, B->d_slabs, B->nslabs,
-                           B->d_direct);
-}
 template <int NZ, class T, bool INSIDE>
 static void batch_launch_zs_in(const BPBatch* B, hipStream_t s) {
+    constexpr int RMAX = NZ == 8 ? 21 : 40;
+    const dim3 grid((unsigned)B->grid), block(256);
     if (B->layout == ARECON_FP_ROWDIFF)
-        batch_launch_zs_ly<NZ, T, ARECON_FP_ROWDIFF, INSIDE>(B, s);
+        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, ARECON_FP_ROWDIFF, RMAX, INSIDE>), grid,
+                           block, 0, s, B->d_slabs, B->nslabs);
     else if constexpr (std::is_same<T, float>::value)
-        batch_launch_zs_ly<NZ, T, ARECON_FP_ROWPAIRS, INSIDE>(B, s);
+        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, ARECON_FP_ROWPAIRS, RMAX, INSIDE>), grid,
+                           block, 0, s, B->d_slabs, B->nslabs);
     else
-        batch_launch_zs_ly<NZ, T, ARECON_FP_QUADS, INSIDE>(B, s);
+        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, ARECON_FP_QUADS, RMAX, INSIDE>), grid,
+                           block, 0, s, B->d_slabs, B->nslabs);
 }
 template <int NZ, class T>
 static void batch_launch_zs(const BPBatch* B, hipStream_t s) {
