@@ -346,13 +346,18 @@ template <int N, int S = 0> constexpr int tpt() {
     return N >= 2048 ? (S == 1 ? N / 8 : N / 16) : 64;
 }
 
-// LDS slot of element k of a transform: one float2 of padding after every 8.
-// A pass writes its outputs R apart (k = 8 j + r for the first radix-8 pass), so
-// unpadded the 64-bit writes of 16 lanes would hit only two of the 32 banks;
-// padded they cover all 32.  Reads (consecutive k) lose at most a 2-way
-// conflict.  An LDS transform buffer holds N + N / 8 float2.
-__device__ __forceinline__ int lpad(int k) { return k + (k >> 3); }
-template <int N> constexpr int lds_fft_len() { return N + N / 8; }
+// LDS slot of element k of a transform: one float2 of padding after every 32.
+// gfx950's LDS serves 64 banks of 4 bytes, so 32 lanes of 64-bit accesses per
+// cycle, conflict-free when their slots differ mod 32.  Reads (consecutive k in
+// aligned runs of 32) then never conflict, nor do the first pass's writes
+// (k = R j + r: slot R j + r + j R / 32); the second radix-8 pass's writes
+// (k = 64 q + j % 8 + 8 r) do.  Measured on the 888 x 64 x 11,808 sinogram, the
+// 2048-point FP32 filter takes 2.83 ms against 2.97 ms with a pad after every 8
+// (2-way conflicts on every read), 2.96 ms after every 64 and 3.49 ms unpadded.
+// An LDS transform buffer holds N + N / 32 float2.
+constexpr int kPadShift = 5;
+__device__ __forceinline__ int lpad(int k) { return k + (k >> kPadShift); }
+template <int N> constexpr int lds_fft_len() { return N + (N >> kPadShift); }
 
 template <int N, int R, int Ns, int SRC, int DST, class OutT, int S>
 __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict__ twg,
@@ -369,9 +374,9 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
         for (int r = 0; r < R; ++r) {
             const int n = j + r * NB;
             if (SRC == SRC_LDS) {
-                // lpad(n) = lpad(j) + lpad(r NB) (NB a multiple of 8): one base
-                // register and immediate offsets
-                static_assert(NB % 8 == 0, "padded reads need NB % 8 == 0");
+                // lpad(n) = lpad(j) + lpad(r NB) (NB a multiple of the pad
+                // period): one base register and immediate offsets
+                static_assert(NB % (1 << kPadShift) == 0, "padded reads need NB % 32 == 0");
                 v[b][r] = lds[lpad(j) + lpad(r * NB)];
             } else if (r < R / 2) {                          // n < N/2; n >= YL reads zero
                 float2 x = make_float2(0.f, 0.f);
@@ -393,10 +398,12 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
         if (Ns > 1) twiddle<R>(v[b], twg[jm * (N / (Ns * R))]);
         dft<R>(v[b]);
         const int d = (j - jm) * R + jm;
-        // lpad(d + r Ns) = lpad(d) + lpad(r Ns) when the low three bits of d and
-        // r Ns do not carry: Ns a multiple of 8, or Ns = 1 with d a multiple of
-        // R >= 4 (r < R).  Then one base register serves every r.
-        constexpr bool kSplit = Ns % 8 == 0 || (Ns == 1 && R >= 4 && R % 4 == 0);
+        // lpad(d + r Ns) = lpad(d) + lpad(r Ns) when the low kPadShift bits of d
+        // and r Ns do not carry: Ns a multiple of the pad period P, or Ns = 1 with
+        // d = R j and R a multiple or a divisor of P (r < R).  Then one base
+        // register serves every r.
+        constexpr int kP = 1 << kPadShift;
+        constexpr bool kSplit = Ns % kP == 0 || (Ns == 1 && (R % kP == 0 || kP % R == 0));
         const int dp = lpad(d);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
