@@ -316,6 +316,124 @@ __global__ __launch_bounds__(256) void conv3_wgrad(float* __restrict__ part,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Weight gradient with the input staged in LDS.  The gathering kernel above
+// issues two 16-byte global loads per MFMA, so the texture addresser (a 64-lane
+// load every 16 clocks per CU), not the matrix cores, bounds it.  Here a
+// workgroup takes one 16-channel input tile and NCO 16-channel output tiles and
+// walks voxel tiles of 2 (z) x 4 (y) x 32 (x).  Per voxel tile it stages the
+// input tile's three dx-shifted channel-major copies with their z/y halo,
+// [3][16][4][6][32] BF16 (one 16-byte pad per channel plane, so the 16 lanes of
+// a B read cover all 64 banks), 74.5 KB: two workgroups per CU.  A K step is one
+// x row of 32 voxels: each wavefront loads its output tile's dY for the row once
+// (16 bytes per lane, channel-major) and reads the shifted X rows of its taps
+// from LDS, one MFMA per tap.  S = 1: a wavefront owns all 27 taps of its output
+// tile (27 accumulators); S = 3: one dz plane of them (9), three wavefronts per
+// output tile.  Partial sums per (split, co, tap, ci) go to part and sum_parts
+// adds the splits in order, so the gradient is deterministic.
+// ---------------------------------------------------------------------------
+constexpr int kWgTz = 2, kWgTy = 4, kWgTx = 32;
+constexpr int kWgZh = kWgTz + 2, kWgYh = kWgTy + 2;
+constexpr int kWgPlane = kWgZh * kWgYh * kWgTx * 2 + 16;        // bytes per channel plane
+constexpr int kWgCopy = 16 * kWgPlane;                          // bytes per dx copy
+constexpr int kWgLds = 3 * kWgCopy;                             // 74,496 bytes
+
+template <int NCO, int S>
+__global__ __launch_bounds__(64 * NCO * S) void conv3_wgrad_lds(
+    float* __restrict__ part, const u16* __restrict__ dyT, const u16* __restrict__ xT3, Vol v,
+    int cin, int cout, int tiles_per_split, int ntiles) {
+    constexpr int NT = S == 1 ? 27 : 9;                  // taps per wavefront
+    constexpr int NTHR = 64 * NCO * S;
+    extern __shared__ uint4 xs4[];
+    char* xs = reinterpret_cast<char*>(xs4);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int split = blockIdx.x, ct = blockIdx.y;
+    const int cot = blockIdx.z * NCO + wave / S;         // this wavefront's output tile
+    const int dzw = S == 1 ? 0 : wave % S;               // its dz plane (S = 3)
+    const int col = lane & 15, g = lane >> 4;
+    const long long nv = v.nvox();
+    const int TX = (v.W + kWgTx - 1) / kWgTx, TY = (v.H + kWgTy - 1) / kWgTy;
+    const int TZ = (v.D + kWgTz - 1) / kWgTz;
+    const int co = cot * 16 + col;
+    const bool co_ok = co < cout;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int tb = split * tiles_per_split, te = min(ntiles, tb + tiles_per_split);
+    for (int tile = tb; tile < te; ++tile) {
+        int r = tile;
+        const int tx = r % TX; r /= TX;
+        const int ty = r % TY; r /= TY;
+        const int tz = r % TZ;
+        const int b = r / TZ;
+        const int x0 = tx * kWgTx, y0 = ty * kWgTy, z0 = tz * kWgTz;
+        // stage: 3 copies x 16 channels x 4 x 6 rows x 4 pieces of 8 voxels
+        constexpr int kPieces = 3 * 16 * kWgZh * kWgYh * 4;
+        for (int i = threadIdx.x; i < kPieces; i += NTHR) {
+            const int q = i & 3;
+            int rr = i >> 2;
+            const int yh = rr % kWgYh; rr /= kWgYh;
+            const int zh = rr % kWgZh; rr /= kWgZh;
+            const int ci = rr & 15, c = rr >> 4;
+            const int z = z0 + zh - 1, y = y0 + yh - 1, x = x0 + 8 * q;
+            const int cg = ct * 16 + ci;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (z >= 0 && z < v.D && y >= 0 && y < v.H && x < v.W && cg < cin)
+                val = *reinterpret_cast<const uint4*>(
+                    xT3 + ((long long)c * cin + cg) * nv +
+                    (((long long)b * v.D + z) * v.H + y) * v.W + x);
+            *reinterpret_cast<uint4*>(xs + c * kWgCopy + ci * kWgPlane +
+                                      ((zh * kWgYh + yh) * kWgTx + 8 * q) * 2) = val;
+        }
+        __syncthreads();
+        if (cot * 16 < cout) {
+            // this lane's B rows: channel col, voxels 8g .. 8g+7 of a row
+            const char* bl = xs + col * kWgPlane + 16 * g + dzw * kWgYh * kWgTx * 2;
+            // dY of row k (zero outside the volume), loaded a row ahead
+            auto load_a = [&](int k) {
+                const int z = z0 + k / kWgTy, y = y0 + k % kWgTy, x = x0 + 8 * g;
+                bf16x8 a = zero8();
+                if (co_ok && z < v.D && y < v.H && x < v.W)
+                    a = load8(dyT + (long long)co * nv +
+                              (((long long)b * v.D + z) * v.H + y) * v.W + x);
+                return a;
+            };
+            bf16x8 an = load_a(0);
+#pragma unroll 1
+            for (int k = 0; k < kWgTz * kWgTy; ++k) {
+                const int zz = k / kWgTy, yy = k % kWgTy;
+                const bf16x8 a = an;
+                if (k + 1 < kWgTz * kWgTy) an = load_a(k + 1);
+                const char* bk = bl + (zz * kWgYh + yy) * kWgTx * 2;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int dz = S == 1 ? t / 9 : 0, dy = (t / 3) % 3, dx = t % 3;
+                    const bf16x8 bv = __builtin_bit_cast(
+                        bf16x8, *reinterpret_cast<const uint4*>(
+                                    bk + dx * kWgCopy + (dz * kWgYh + dy) * kWgTx * 2));
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();                  // every read of the tile before the next staging
+    }
+    if (cot * 16 >= cout) return;
+    // D[co = 4g + q][ci = col] of tap t
+    float* ps = part + (long long)split * cout * 27 * cin;
+    const int ci = ct * 16 + col;
+    if (ci >= cin) return;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int tap = S == 1 ? t : dzw * 9 + t;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c_o = cot * 16 + 4 * g + q;
+            if (c_o < cout) ps[((long long)c_o * 27 + tap) * cin + ci] = acc[t][q];
+        }
+    }
+}
+
 // dst[i] += sum over k = 0 .. nparts-1, in order, of part[k * n + i].
 __global__ __launch_bounds__(256) void sum_parts(float* __restrict__ dst,
                                                  const float* __restrict__ part, long long n,
@@ -325,6 +443,20 @@ __global__ __launch_bounds__(256) void sum_parts(float* __restrict__ dst,
         float acc = 0.f;
         for (int k = 0; k < nparts; ++k) acc += part[(long long)k * n + i];
         dst[i] += acc;
+    }
+}
+
+// First level of a two-level ordered sum: part2[gi][i] = sum over the parts
+// k = 32 gi .. 32 gi + 31 (< nparts), in order, of part[k * n + i].
+__global__ __launch_bounds__(256) void sum_part_groups(float* __restrict__ part2,
+                                                       const float* __restrict__ part,
+                                                       long long n, int nparts) {
+    const int gi = blockIdx.y, k1 = min(nparts, 32 * gi + 32);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        float acc = 0.f;
+        for (int k = 32 * gi; k < k1; ++k) acc += part[(long long)k * n + i];
+        part2[(long long)gi * n + i] = acc;
     }
 }
 
@@ -382,6 +514,57 @@ __global__ void to_channel_major3(u16* __restrict__ xT3, const u16* __restrict__
             const bool ok = xx + dx >= 0 && xx + dx < v.W;
             xT3[(long long)s * c * nv + i] = ok ? x[(n + dx) * c + ch] : (u16)0;
         }
+    }
+}
+
+// The same two layouts when W is a multiple of 8 (every 8-voxel run lies in one
+// row): a thread takes one channel of one 8-voxel run and writes 16-byte pieces
+// of the channel-major copies; consecutive threads take consecutive channels, so
+// each of its loads reads one voxel's channels, contiguous across the
+// wavefront.  (The per-element kernels above read with a stride of c elements
+// and run about 7x below the copy rate.)
+__device__ __forceinline__ uint4 pack_u16x8(const u16* e) {
+    return make_uint4((unsigned)e[0] | ((unsigned)e[1] << 16), (unsigned)e[2] | ((unsigned)e[3] << 16),
+                      (unsigned)e[4] | ((unsigned)e[5] << 16), (unsigned)e[6] | ((unsigned)e[7] << 16));
+}
+
+__global__ void relu_bwd8(u16* __restrict__ dyp, u16* __restrict__ dypT,
+                          const u16* __restrict__ dy, const u16* __restrict__ y, long long nv,
+                          int c, int relu) {
+    GRID_STRIDE(i, nv / 8 * c) {
+        const long long grp = i / c;
+        const int ch = (int)(i - grp * c);
+        const long long n0 = grp * 8;
+        u16 o[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const long long idx = (n0 + k) * c + ch;
+            u16 gv = dy[idx];
+            if (relu && !(bf2f(y[idx]) > 0.f)) gv = 0;
+            dyp[idx] = gv;
+            o[k] = gv;
+        }
+        *reinterpret_cast<uint4*>(dypT + (long long)ch * nv + n0) = pack_u16x8(o);
+    }
+}
+
+__global__ void to_channel_major3_8(u16* __restrict__ xT3, const u16* __restrict__ x, Vol v,
+                                    int c) {
+    const long long nv = v.nvox();
+    GRID_STRIDE(i, nv / 8 * c) {
+        const long long grp = i / c;
+        const int ch = (int)(i - grp * c);
+        const long long n0 = grp * 8;
+        const int x0 = (int)(n0 % v.W);
+        u16 e[10];                                   // voxels n0 - 1 .. n0 + 8 of the row
+        e[0] = x0 > 0 ? x[(n0 - 1) * c + ch] : (u16)0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) e[k + 1] = x[(n0 + k) * c + ch];
+        e[9] = x0 + 8 < v.W ? x[(n0 + 8) * c + ch] : (u16)0;
+#pragma unroll
+        for (int sft = 0; sft < 3; ++sft)            // dx = sft - 1: copy[n] = x[n + dx]
+            *reinterpret_cast<uint4*>(xT3 + ((long long)sft * c + ch) * nv + n0) =
+                pack_u16x8(e + sft);
     }
 }
 
@@ -654,6 +837,48 @@ void cnn_conv3_wgrad(float* dw, const void* dyT, const void* xT3, const int* dim
     ARECON_REQUIRE(cin % 8 == 0 && cout % 4 == 0, "cnn: channel counts");
     const long long nv = v.nvox();
     const int ct = (cout + 15) / 16, it = (cin + 15) / 16;
+    const char* lds_env = std::getenv("ARECON_CNN_WGRAD_LDS");
+    if (!(lds_env && lds_env[0] == '0')) {
+        // the LDS-staged kernel: NCO output tiles per workgroup, S wavefronts each
+        // (S = 3 for one or two output tiles, so a workgroup has 3 or 6 wavefronts)
+        const int nco = std::min(4, ct);
+        const int cog = (ct + nco - 1) / nco;
+        const long long ntiles = (long long)v.B * ((v.D + kWgTz - 1) / kWgTz) *
+                                 ((v.H + kWgTy - 1) / kWgTy) * ((v.W + kWgTx - 1) / kWgTx);
+        ARECON_REQUIRE(ntiles < (1LL << 31), "cnn: volume too large for one launch");
+        long long nsplit = std::max<long long>(
+            1, std::min<long long>(ntiles, (1024 + (long long)it * cog - 1) / ((long long)it * cog)));
+        const long long per = (ntiles + nsplit - 1) / nsplit;
+        nsplit = (ntiles + per - 1) / per;
+        const long long nw = 27LL * cout * cin;
+        DevBuf part(sizeof(float) * (size_t)(nw * nsplit), s);
+        const dim3 grid((unsigned)nsplit, (unsigned)it, (unsigned)cog);
+        auto go = [&](auto kern, int threads) {
+            (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      kWgLds);
+            hipLaunchKernelGGL(kern, grid, dim3(threads), kWgLds, s, part.as<float>(),
+                               static_cast<const u16*>(dyT), static_cast<const u16*>(xT3), v, cin,
+                               cout, (int)per, (int)ntiles);
+        };
+        switch (nco) {
+        case 4: go(conv3_wgrad_lds<4, 1>, 256); break;
+        case 3: go(conv3_wgrad_lds<3, 1>, 192); break;
+        case 2: go(conv3_wgrad_lds<2, 3>, 384); break;
+        default: go(conv3_wgrad_lds<1, 3>, 192); break;
+        }
+        ARECON_HIP(hipGetLastError());
+        // the splits' partial sums, added in a fixed order: in groups of 32, then the groups
+        // (one thread per weight adding hundreds of splits in sequence was latency-bound)
+        const int ngroups = (int)((nsplit + 31) / 32);
+        DevBuf part2(sizeof(float) * (size_t)(nw * ngroups), s);
+        hipLaunchKernelGGL(sum_part_groups, dim3(grid_for(nw), (unsigned)ngroups), dim3(256), 0,
+                           s, part2.as<float>(), part.as<float>(), nw, (int)nsplit);
+        ARECON_HIP(hipGetLastError());
+        hipLaunchKernelGGL(sum_parts, dim3(grid_for(nw)), dim3(256), 0, s, dw, part2.as<float>(),
+                           nw, ngroups);
+        ARECON_HIP(hipGetLastError());
+        return;
+    }
     const long long tiles = 27LL * ct * it;
     // enough waves to fill the GPU: split the voxel range (multiples of 32)
     long long nsplit = std::max<long long>(1, std::min<long long>((nv + 2047) / 2048,
@@ -676,9 +901,16 @@ void cnn_conv3_wgrad(float* dw, const void* dyT, const void* xT3, const int* dim
 void cnn_relu_bwd(void* dyp, void* dypT, float* db, const void* dy, const void* y,
                   const int* dims, int c, int relu, hipStream_t s) {
     const Vol v = mkvol(dims);
-    hipLaunchKernelGGL(relu_bwd, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
-                       static_cast<u16*>(dyp), static_cast<u16*>(dypT),
-                       static_cast<const u16*>(dy), static_cast<const u16*>(y), v.nvox(), c, relu);
+    if (v.W % 8 == 0)
+        hipLaunchKernelGGL(relu_bwd8, dim3(grid_for(v.nvox() / 8 * c)), dim3(256), 0, s,
+                           static_cast<u16*>(dyp), static_cast<u16*>(dypT),
+                           static_cast<const u16*>(dy), static_cast<const u16*>(y), v.nvox(), c,
+                           relu);
+    else
+        hipLaunchKernelGGL(relu_bwd, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+                           static_cast<u16*>(dyp), static_cast<u16*>(dypT),
+                           static_cast<const u16*>(dy), static_cast<const u16*>(y), v.nvox(), c,
+                           relu);
     ARECON_HIP(hipGetLastError());
     if (db) {
         const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>(
@@ -695,8 +927,12 @@ void cnn_relu_bwd(void* dyp, void* dypT, float* db, const void* dy, const void* 
 
 void cnn_channel_major3(void* xT3, const void* x, const int* dims, int c, hipStream_t s) {
     const Vol v = mkvol(dims);
-    hipLaunchKernelGGL(to_channel_major3, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
-                       static_cast<u16*>(xT3), static_cast<const u16*>(x), v, c);
+    if (v.W % 8 == 0)
+        hipLaunchKernelGGL(to_channel_major3_8, dim3(grid_for(v.nvox() / 8 * c)), dim3(256), 0, s,
+                           static_cast<u16*>(xT3), static_cast<const u16*>(x), v, c);
+    else
+        hipLaunchKernelGGL(to_channel_major3, dim3(grid_for(v.nvox() * c)), dim3(256), 0, s,
+                           static_cast<u16*>(xT3), static_cast<const u16*>(x), v, c);
     ARECON_HIP(hipGetLastError());
 }
 

@@ -152,6 +152,85 @@ def test_conv_gradients(gpu, cin, cout):
     assert rel(gb.cpu(), dy.sum(dim=(0, 1, 2, 3))) < 1e-4
 
 
+@pytest.mark.parametrize("cin,cout", [(8, 32), (32, 8), (32, 32), (64, 32), (32, 64),
+                                      (128, 128), (16, 48)])
+def test_wgrad_lds_against_autograd_and_the_gathering_kernel(gpu, cin, cout, monkeypatch):
+    """The LDS-staged weight gradient (voxel tiles of 2 x 4 x 32, the input's
+    three shifted copies staged with their halo) against autograd, on a volume
+    whose every dimension leaves a partial tile, and against the gathering
+    kernel (ARECON_CNN_WGRAD_LDS=0), which sums the same products in another
+    order."""
+    g = torch.Generator().manual_seed(5 * cin + cout)
+    B, D, H, W = 2, 5, 7, 40
+    n = B * D * H * W
+    x = bf(torch.randn(B, D, H, W, cin, generator=g))
+    dy = bf(torch.randn(B, D, H, W, cout, generator=g))
+    xt = x.permute(0, 4, 1, 2, 3).clone()
+    wt = torch.zeros(cout, cin, 3, 3, 3, requires_grad=True)
+    F.conv3d(xt, wt, None, padding=1).backward(dy.permute(0, 4, 1, 2, 3))
+    want = wt.grad.reshape(cout, cin, 27).permute(0, 2, 1)          # [Cout][27][Cin]
+    L = _lib.lib()
+    s = st()
+    dims = cnn._dims(B, D, H, W)
+    dyd, xd = to_dev_bf16(dy), to_dev_bf16(x)      # held until the kernels have run
+    dyp = torch.empty((n, cout), dtype=torch.int16, device="cuda")
+    dypT = torch.empty((cout, n), dtype=torch.int16, device="cuda")
+    _lib.check(L.arecon_cnn_relu_bwd(cnn._ptr(dyp), cnn._ptr(dypT), cnn._ptr(None),
+                                     cnn._ptr(dyd), cnn._ptr(None), dims, cout, 0, s), "relu_bwd")
+    xT3 = torch.empty((3, cin, n), dtype=torch.int16, device="cuda")
+    _lib.check(L.arecon_cnn_channel_major3(cnn._ptr(xT3), cnn._ptr(xd), dims, cin, s), "T3")
+    got = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("ARECON_CNN_WGRAD_LDS", mode)
+        gw = torch.zeros((cout, 27, cin), device="cuda")
+        _lib.check(L.arecon_cnn_conv3_wgrad(cnn._ptr(gw), cnn._ptr(dypT), cnn._ptr(xT3), dims,
+                                            cin, cout, s), "wgrad")
+        torch.cuda.synchronize()
+        got[mode] = gw.cpu()
+    e_lds, e_both = rel(got["1"], want), rel(got["1"], got["0"])
+    print(f"wgrad {cin}->{cout}: LDS vs autograd {e_lds:.2e}, vs gathering {e_both:.2e}",
+          flush=True)
+    assert e_lds < 1e-4 and e_both < 1e-5
+
+
+@pytest.mark.parametrize("W", [24, 20])
+@pytest.mark.parametrize("c", [8, 32, 128])
+def test_channel_major_layouts(gpu, c, W):
+    """relu_bwd's masked gradient in both layouts and the input's three
+    dx-shifted channel-major copies, bit for bit against torch: W = 24 runs the
+    8-voxel-run kernels, W = 20 the per-element ones."""
+    g = torch.Generator().manual_seed(c + W)
+    B, D, H = 2, 3, 5
+    n = B * D * H * W
+    x = bf(torch.randn(B, D, H, W, c, generator=g))
+    dy = bf(torch.randn(B, D, H, W, c, generator=g))
+    L = _lib.lib()
+    s = st()
+    dims = cnn._dims(B, D, H, W)
+    dyd, xd = to_dev_bf16(dy), to_dev_bf16(x)      # held until the kernels have run
+    dyp = torch.empty((n, c), dtype=torch.int16, device="cuda")
+    dypT = torch.empty((c, n), dtype=torch.int16, device="cuda")
+    _lib.check(L.arecon_cnn_relu_bwd(cnn._ptr(dyp), cnn._ptr(dypT), cnn._ptr(None),
+                                     cnn._ptr(dyd), cnn._ptr(xd), dims, c, 1, s), "relu_bwd")
+    xT3 = torch.empty((3, c, n), dtype=torch.int16, device="cuda")
+    _lib.check(L.arecon_cnn_channel_major3(cnn._ptr(xT3), cnn._ptr(xd), dims, c, s), "T3")
+    torch.cuda.synchronize()
+    masked = torch.where(x > 0, dy, torch.zeros_like(dy)).reshape(n, c)
+    assert torch.equal(from_dev_bf16(dyp, (n, c)), masked)
+    assert torch.equal(from_dev_bf16(dypT, (c, n)), masked.t())
+    got = from_dev_bf16(xT3, (3, c, B, D, H, W))
+    xcm = x.permute(4, 0, 1, 2, 3)                                  # [c][B][D][H][W]
+    for sft, dx in enumerate((-1, 0, 1)):
+        want = torch.zeros_like(xcm)
+        if dx < 0:
+            want[..., 1:] = xcm[..., :-1]
+        elif dx > 0:
+            want[..., :-1] = xcm[..., 1:]
+        else:
+            want = xcm
+        assert torch.equal(got[sft], want), (c, W, dx)
+
+
 def test_pool_upsample_and_adjoints(gpu):
     g = torch.Generator().manual_seed(3)
     B, D, H, W, c = 2, 4, 6, 8, 16
