@@ -98,8 +98,19 @@ __global__ __launch_bounds__(256) void conv3_fwd(u16* __restrict__ y, const u16*
         for (int u = 0; u < 2; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int nchunks = kp / 32;
     for (int c = 0; c < nchunks; ++c) {
-        const int k0 = c * 32 + kg * 8;
-        const int tap = k0 / cin, ci0 = k0 - tap * cin;
+        // K steps in the tiled kernel's order: Cin a multiple of 32, 32-channel
+        // passes, each over the 27 taps (step c: tap c % 27, channels
+        // 32 (c / 27) + ..); otherwise 32 consecutive k
+        int k0, tap, ci0;
+        if (cin % 32 == 0) {
+            tap = c % 27;
+            ci0 = 32 * (c / 27) + kg * 8;
+            k0 = tap * cin + ci0;
+        } else {
+            k0 = c * 32 + kg * 8;
+            tap = k0 / cin;
+            ci0 = k0 - tap * cin;
+        }
         bf16x8 b[2];
         if (tap < 27) {
             const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
@@ -160,16 +171,33 @@ __global__ __launch_bounds__(256) void conv3_fwd(u16* __restrict__ y, const u16*
 // chunks of 32), so every output sees the same MFMA sequence and the result is
 // identical.
 // ---------------------------------------------------------------------------
+// LDS swizzle of the tiled convolution's input tile: a voxel's C/8 16-byte
+// pieces are stored in the order p ^ cnn_swz(hx), hx its x in the tile.  The 16
+// lanes of a B read take 16 consecutive voxels' same piece; unswizzled, their
+// voxel stride of C * 2 bytes put them on 2 (C = 64) or 4 (C = 32) of the 16
+// four-bank groups of gfx950's 64-bank LDS, an 8- or 4-way conflict; swizzled
+// they cover all 16.
+template <int C8> __device__ __forceinline__ int cnn_swz(int hx) {
+    static_assert(C8 >= 1 && C8 <= 16 && (C8 & (C8 - 1)) == 0, "pieces per voxel");
+    constexpr int sh = C8 == 1 ? 4 : C8 == 2 ? 3 : C8 == 4 ? 2 : C8 == 8 ? 1 : 0;
+    return (hx >> sh) & (C8 - 1);
+}
+
 template <int MT, int NV, int CIN>
-__global__ __launch_bounds__(256) void conv3_fwd_tiled(u16* __restrict__ y,
+__global__ __launch_bounds__(256, 2) void conv3_fwd_tiled(u16* __restrict__ y,
                                                        const u16* __restrict__ x,
                                                        const u16* __restrict__ w,
                                                        const float* __restrict__ bias,
                                                        const u16* __restrict__ skip, Vol v,
                                                        int cout, int relu,
                                                        int tilesx, int tilesy, int tilesz) {
-    constexpr int cin = CIN, kp = (27 * CIN + 31) / 32 * 32, nchunks = kp / 32;
-    extern __shared__ uint4 tile[];          // [6][NV+2][18][cin/8] 16-byte pieces
+    constexpr int cin = CIN, kp = (27 * CIN + 31) / 32 * 32;
+    // channels staged per pass (CS) and passes: all of them up to 32, else 32 at
+    // a time, so the tile takes 69 KB and two workgroups share a CU (one stages
+    // while the other computes)
+    constexpr int CS = CIN < 32 ? CIN : 32, NPASS = CIN / CS;
+    constexpr int nchunks = CIN < 32 ? kp / 32 : 27;   // K steps per pass
+    extern __shared__ uint4 tile[];          // [6][NV+2][18][CS/8] 16-byte pieces
     constexpr int TY = NV + 2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int t = blockIdx.x;
@@ -178,58 +206,150 @@ __global__ __launch_bounds__(256) void conv3_fwd_tiled(u16* __restrict__ y,
     const int tz = t % tilesz;
     const int b = t / tilesz;
     const int x0 = tx * 16, y0 = ty * NV, z0 = tz * 4;
-    constexpr int c8 = CIN / 8;
+    constexpr int c8 = CS / 8;
     constexpr int npieces = 6 * TY * 18 * c8;
     const long long zbase = (long long)b * v.D;
-    for (int i = threadIdx.x; i < npieces; i += 256) {
-        const int p = i % c8;
-        int r = i / c8;
-        const int hx = r % 18;
-        r /= 18;
-        const int hy = r % TY, hz = r / TY;
-        const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
-        uint4 val = make_uint4(0u, 0u, 0u, 0u);
-        if (gz >= 0 && gz < v.D && gy >= 0 && gy < v.H && gx >= 0 && gx < v.W)
-            val = *reinterpret_cast<const uint4*>(
-                x + (((zbase + gz) * v.H + gy) * v.W + gx) * cin + 8 * p);
-        tile[i] = val;
-    }
-    __syncthreads();
     const int col = lane & 15, kg = lane >> 4;
     f32x4 acc[MT][NV];
 #pragma unroll
     for (int tt = 0; tt < MT; ++tt)
 #pragma unroll
         for (int r = 0; r < NV; ++r) acc[tt][r] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // A of the next chunk is loaded while this chunk's MFMAs run; all NV B reads
-    // of a chunk are issued before its first MFMA.  The padded K tail (k >= 27
-    // Cin, only for Cin = 8) has zero weights, so its B may be any finite
-    // samples: the tap is clamped to 26 for the address.
     const u16* wl = w + (long long)col * kp + kg * 8;
-    bf16x8 a[MT], an[MT];
+#pragma unroll 1
+    for (int pass = 0; pass < NPASS; ++pass) {
+        if (pass > 0) __syncthreads();       // every read of the last pass's tile
+        // kSB pieces per thread in flight: every load of a batch is issued before
+        // the first LDS write waits on one (one piece per loop turn serialised
+        // the turns' memory latencies)
+        constexpr int kSB = 8;
+        for (int i0 = threadIdx.x; i0 < npieces; i0 += 256 * kSB) {
+            uint4 val[kSB];
+            int dst[kSB];
 #pragma unroll
-    for (int tt = 0; tt < MT; ++tt) a[tt] = load8(wl + (long long)16 * tt * kp);
-    for (int c = 0; c < nchunks; ++c) {
-        if (c + 1 < nchunks) {
+            for (int u = 0; u < kSB; ++u) {
+                const int i = i0 + 256 * u;
+                val[u] = make_uint4(0u, 0u, 0u, 0u);
+                dst[u] = -1;
+                if (i < npieces) {
+                    const int p = i % c8;
+                    int r = i / c8;
+                    const int hx = r % 18;
+                    r /= 18;
+                    const int hy = r % TY, hz = r / TY;
+                    const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
+                    if (gz >= 0 && gz < v.D && gy >= 0 && gy < v.H && gx >= 0 && gx < v.W)
+                        val[u] = *reinterpret_cast<const uint4*>(
+                            x + (((zbase + gz) * v.H + gy) * v.W + gx) * cin + pass * CS + 8 * p);
+                    dst[u] = i - p + (p ^ cnn_swz<c8>(hx));
+                }
+            }
 #pragma unroll
-            for (int tt = 0; tt < MT; ++tt) an[tt] = load8(wl + (long long)16 * tt * kp + 32 * (c + 1));
+            for (int u = 0; u < kSB; ++u)
+                if (dst[u] >= 0) tile[dst[u]] = val[u];
         }
-        const int k0 = c * 32 + kg * 8;
-        const int tap_k = k0 / CIN, ci0 = k0 - tap_k * CIN;
-        const int tap = min(tap_k, 26);
-        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;     // halo-tile offsets
-        const int base = (((wave + dz) * TY + dy) * 18 + col + dx) * c8 + (ci0 >> 3);
-        bf16x8 bv[NV];
+        __syncthreads();
+        // K step c of this pass: k0 = its first k (the lane adds 8 kg), and the
+        // tap and staged piece its B reads.  Cin >= 32: one tap's 32 channels
+        // of the pass; Cin < 32: 32 consecutive k of the whole K.  The padded
+        // K tail (k >= 27 Cin, only for Cin = 8) has zero weights, so its B may
+        // be any finite samples: the tap is clamped to 26 for the address.
+        auto kstep = [&](int c, int& k0, int& tap, int& piece) {
+            if constexpr (CIN >= 32) {
+                tap = c;
+                k0 = c * CIN + pass * 32;
+                piece = kg;
+            } else {
+                const int kl = c * 32 + kg * 8;
+                const int tk = kl / CIN;
+                tap = min(tk, 26);
+                k0 = c * 32;
+                piece = (kl - tk * CIN) >> 3;
+            }
+        };
+        auto aaddr = [&](int c) {
+            int k0, tap, piece;
+            kstep(c, k0, tap, piece);
+            return wl + k0;
+        };
+        auto bload = [&](int c, bf16x8* bv) {
+            int k0, tap, piece;
+            kstep(c, k0, tap, piece);
+            const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;     // halo-tile offsets
+            const int base = (((wave + dz) * TY + dy) * 18 + col + dx) * c8 +
+                             (piece ^ cnn_swz<c8>(col + dx));
 #pragma unroll
-        for (int r = 0; r < NV; ++r) bv[r] = __builtin_bit_cast(bf16x8, tile[base + r * 18 * c8]);
+            for (int r = 0; r < NV; ++r)
+                bv[r] = __builtin_bit_cast(bf16x8, tile[base + r * 18 * c8]);
+        };
+        // Software pipeline (MT <= 2).  A (weights, from L1/L2) is loaded kPA
+        // steps ahead in a ring of registers: with one or two wavefronts per SIMD
+        // nothing else hides a global load.  B (the NV LDS reads of a step) is
+        // read one step ahead, into the other of two register sets.  With MT >= 4
+        // a step's 4 NV MFMAs cover one step of A latency, and the accumulators
+        // leave no room for the rings: A one step ahead, B at the step's start.
+        if constexpr (MT <= 2) {
+            constexpr int kPA = 4;
+            bf16x8 ar[kPA][MT];
 #pragma unroll
-        for (int r = 0; r < NV; ++r)
+            for (int i = 0; i < kPA; ++i)
+                if (i < nchunks) {
+                    const u16* pa = aaddr(i);
 #pragma unroll
-            for (int tt = 0; tt < MT; ++tt)
-                acc[tt][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[tt], bv[r], acc[tt][r], 0, 0, 0);
-        if (c + 1 < nchunks) {
+                    for (int tt = 0; tt < MT; ++tt) ar[i][tt] = load8(pa + (long long)16 * tt * kp);
+                }
+            bf16x8 bb[2][NV];
+            bload(0, bb[0]);
+            // not unrolled: each turn's loads for the next turn stay a turn ahead
+            // (fully unrolled, the scheduler moved every load next to its use)
+#pragma unroll 1
+            for (int c0 = 0; c0 < nchunks; c0 += kPA) {
 #pragma unroll
-            for (int tt = 0; tt < MT; ++tt) a[tt] = an[tt];
+                for (int i = 0; i < kPA; ++i) {  // kPA even: step c's B set is c & 1 = i & 1
+                    const int c = c0 + i;
+                    if (c < nchunks) {
+                        if (c + 1 < nchunks) bload(c + 1, bb[(i + 1) & 1]);
+#pragma unroll
+                        for (int r = 0; r < NV; ++r)
+#pragma unroll
+                            for (int tt = 0; tt < MT; ++tt)
+                                acc[tt][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    ar[i][tt], bb[i & 1][r], acc[tt][r], 0, 0, 0);
+                        if (c + kPA < nchunks) {
+                            const u16* pa = aaddr(c + kPA);
+#pragma unroll
+                            for (int tt = 0; tt < MT; ++tt)
+                                ar[i][tt] = load8(pa + (long long)16 * tt * kp);
+                        }
+                    }
+                }
+            }
+        } else {
+            bf16x8 a[MT], an[MT];
+            {
+                const u16* pa = aaddr(0);
+#pragma unroll
+                for (int tt = 0; tt < MT; ++tt) a[tt] = load8(pa + (long long)16 * tt * kp);
+            }
+            for (int c = 0; c < nchunks; ++c) {
+                if (c + 1 < nchunks) {
+                    const u16* pa = aaddr(c + 1);
+#pragma unroll
+                    for (int tt = 0; tt < MT; ++tt) an[tt] = load8(pa + (long long)16 * tt * kp);
+                }
+                bf16x8 bv[NV];
+                bload(c, bv);
+#pragma unroll
+                for (int r = 0; r < NV; ++r)
+#pragma unroll
+                    for (int tt = 0; tt < MT; ++tt)
+                        acc[tt][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[tt], bv[r],
+                                                                              acc[tt][r], 0, 0, 0);
+                if (c + 1 < nchunks) {
+#pragma unroll
+                    for (int tt = 0; tt < MT; ++tt) a[tt] = an[tt];
+                }
+            }
         }
     }
     const int zz = z0 + wave, xx = x0 + col;
@@ -780,12 +900,13 @@ void cnn_conv3_fwd(void* y, const void* x, const void* w, const float* bias, con
     const int kp = (27 * cin + 31) / 32 * 32;
     const int mt = (cout + 15) / 16;
     const char* tiled_env = std::getenv("ARECON_CNN_CONV_TILED");
-    if ((cin == 8 || cin == 16 || cin == 32 || cin == 64) && !(tiled_env && tiled_env[0] == '0')) {
+    if ((cin == 8 || cin == 16 || cin == 32 || cin == 64 || cin == 128) &&
+        !(tiled_env && tiled_env[0] == '0')) {
         const int nv = mt == 8 ? 4 : 8;            // rows of 16 voxels per wavefront
         const int tilesx = (v.W + 15) / 16, tilesy = (v.H + nv - 1) / nv, tilesz = (v.D + 3) / 4;
         const long long nblk = (long long)v.B * tilesz * tilesy * tilesx;
         ARECON_REQUIRE(nblk < (1LL << 31), "cnn: volume too large for one launch");
-        const size_t lds = (size_t)6 * (nv + 2) * 18 * cin * 2;
+        const size_t lds = (size_t)6 * (nv + 2) * 18 * std::min(cin, 32) * 2;   // one pass
         const dim3 grid((unsigned)nblk), block(256);
         auto* Y = static_cast<u16*>(y);
         auto* X = static_cast<const u16*>(x);
@@ -809,7 +930,8 @@ void cnn_conv3_fwd(void* y, const void* x, const void* w, const float* bias, con
         else if (cin == 16) { ARECON_CONV_C(16) }
         else if (cin == 32) { ARECON_CONV_C(32) }
         else if (cin == 64) { ARECON_CONV_C(64) }
-        else fail(ARECON_ERR_ARG, "cnn: tiled convolution takes 8, 16, 32 or 64 input channels");
+        else if (cin == 128) { ARECON_CONV_C(128) }
+        else fail(ARECON_ERR_ARG, "cnn: tiled convolution takes 8, 16, 32, 64 or 128 input channels");
 #undef ARECON_CONV_C
         ARECON_HIP(hipGetLastError());
         return;

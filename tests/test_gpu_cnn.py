@@ -77,11 +77,13 @@ def test_conv_forward(gpu, cin, cout, relu, skip):
 @pytest.mark.parametrize("cin,cout,dims", [(8, 32, (2, 6, 5, 32)), (32, 32, (1, 9, 11, 40)),
                                            (64, 32, (1, 5, 7, 20)), (32, 64, (2, 4, 8, 16)),
                                            (64, 128, (1, 6, 6, 33)), (32, 8, (1, 8, 9, 17)),
-                                           (32, 16, (3, 4, 4, 4))])
+                                           (32, 16, (3, 4, 4, 4)), (128, 64, (1, 5, 9, 24)),
+                                           (128, 128, (2, 4, 4, 16))])
 def test_conv_tiled_equals_untiled(gpu, cin, cout, dims, monkeypatch):
-    """The LDS-tiled forward convolution (Cin <= 64) runs the untiled kernel's
-    K loop in the same order, so both give identical BF16 outputs; ragged tiles
-    in every dimension, several volumes per batch."""
+    """The LDS-tiled forward convolution (32-channel staging passes beyond 32
+    input channels) runs the untiled kernel's K loop in the same order, so both
+    give identical BF16 outputs; ragged tiles in every dimension, several
+    volumes per batch."""
     print(f"conv tiled {cin}->{cout} dims={dims}", flush=True)
     g = torch.Generator().manual_seed(7 * cin + cout)
     B, D, H, W = dims
