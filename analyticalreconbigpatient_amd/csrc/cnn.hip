@@ -39,6 +39,8 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
+typedef __attribute__((address_space(3))) char lds_char;
+typedef __attribute__((address_space(3))) void lds_void;
 
 struct Vol {            // batch of volumes, channels-last
     int B, D, H, W;
@@ -207,7 +209,8 @@ __global__ __launch_bounds__(256, 2) void conv3_fwd_tiled(u16* __restrict__ y,
     const int b = t / tilesz;
     const int x0 = tx * 16, y0 = ty * NV, z0 = tz * 4;
     constexpr int c8 = CS / 8;
-    constexpr int npieces = 6 * TY * 18 * c8;
+    // the tile's 16-byte slots, padded to whole 64-lane staging instructions
+    constexpr int NSLOT = 6 * TY * 18 * c8, NINS = (NSLOT + 63) / 64, NSLOTP = NINS * 64;
     const long long zbase = (long long)b * v.D;
     const int col = lane & 15, kg = lane >> 4;
     f32x4 acc[MT][NV];
@@ -216,39 +219,50 @@ __global__ __launch_bounds__(256, 2) void conv3_fwd_tiled(u16* __restrict__ y,
 #pragma unroll
         for (int r = 0; r < NV; ++r) acc[tt][r] = f32x4{0.f, 0.f, 0.f, 0.f};
     const u16* wl = w + (long long)col * kp + kg * 8;
+    // Staging: direct global -> LDS loads (no VGPRs, asynchronous), 64 slots per
+    // instruction, the wavefronts taking every fourth instruction.  Lane slot s
+    // of the tile (voxel-major, pieces swizzled) loads its voxel's piece through a
+    // buffer descriptor based at the halo's origin voxel; a voxel outside the
+    // volume (or a padding slot) gets an offset past the descriptor's range and
+    // loads zeros.  One buffer (69 KB at most): two workgroups share a CU, and
+    // one stages while the other computes.  (Double-buffered, pass h + 1 loading
+    // while pass h computes, the 139 KB tile left one workgroup per CU: 64-channel
+    // layer 16.3 -> 22.7 ms.)
+    const long long org = ((zbase + z0 - 1) * v.H + (y0 - 1)) * (long long)v.W + (x0 - 1);
+    auto stage = [&](int pass, int buf) {
+        const unsigned long long bp =
+            reinterpret_cast<unsigned long long>(x) + (unsigned long long)((org * cin + pass * CS) * 2);
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)bp);
+        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(bp >> 32));
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            reinterpret_cast<void*>(((unsigned long long)hi << 32) | lo), (short)0, 0x7fffffff,
+            0x00020000);
+        for (int j = wave; j < NINS; j += 4) {
+            const int sl = 64 * j + lane;
+            unsigned off = 0x80000000u;
+            if (sl < NSLOT) {
+                const int pp = sl % c8;
+                int r = sl / c8;
+                const int hx = r % 18;
+                r /= 18;
+                const int hy = r % TY, hz = r / TY;
+                const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
+                if (gz >= 0 && gz < v.D && gy >= 0 && gy < v.H && gx >= 0 && gx < v.W)
+                    off = (unsigned)(((((long long)hz * v.H + hy) * v.W + hx) * cin +
+                                      8 * (pp ^ cnn_swz<c8>(hx))) * 2);
+            }
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rs, (lds_void*)((lds_char*)tile + ((long long)buf * NSLOTP + 64 * j) * 16), 16, off,
+                0, 0, 0);
+        }
+    };
 #pragma unroll 1
     for (int pass = 0; pass < NPASS; ++pass) {
         if (pass > 0) __syncthreads();       // every read of the last pass's tile
-        // kSB pieces per thread in flight: every load of a batch is issued before
-        // the first LDS write waits on one (one piece per loop turn serialised
-        // the turns' memory latencies)
-        constexpr int kSB = 8;
-        for (int i0 = threadIdx.x; i0 < npieces; i0 += 256 * kSB) {
-            uint4 val[kSB];
-            int dst[kSB];
-#pragma unroll
-            for (int u = 0; u < kSB; ++u) {
-                const int i = i0 + 256 * u;
-                val[u] = make_uint4(0u, 0u, 0u, 0u);
-                dst[u] = -1;
-                if (i < npieces) {
-                    const int p = i % c8;
-                    int r = i / c8;
-                    const int hx = r % 18;
-                    r /= 18;
-                    const int hy = r % TY, hz = r / TY;
-                    const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
-                    if (gz >= 0 && gz < v.D && gy >= 0 && gy < v.H && gx >= 0 && gx < v.W)
-                        val[u] = *reinterpret_cast<const uint4*>(
-                            x + (((zbase + gz) * v.H + gy) * v.W + gx) * cin + pass * CS + 8 * p);
-                    dst[u] = i - p + (p ^ cnn_swz<c8>(hx));
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kSB; ++u)
-                if (dst[u] >= 0) tile[dst[u]] = val[u];
-        }
-        __syncthreads();
+        stage(pass, 0);
+        __builtin_amdgcn_s_waitcnt(0);       // this wavefront's loads landed
+        __syncthreads();                     // and every other wavefront's
+        const int bufo = 0;
         // K step c of this pass: k0 = its first k (the lane adds 8 kg), and the
         // tap and staged piece its B reads.  Cin >= 32: one tap's 32 channels
         // of the pass; Cin < 32: 32 consecutive k of the whole K.  The padded
@@ -276,19 +290,64 @@ __global__ __launch_bounds__(256, 2) void conv3_fwd_tiled(u16* __restrict__ y,
             int k0, tap, piece;
             kstep(c, k0, tap, piece);
             const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;     // halo-tile offsets
-            const int base = (((wave + dz) * TY + dy) * 18 + col + dx) * c8 +
+            const int base = bufo + (((wave + dz) * TY + dy) * 18 + col + dx) * c8 +
                              (piece ^ cnn_swz<c8>(col + dx));
 #pragma unroll
             for (int r = 0; r < NV; ++r)
                 bv[r] = __builtin_bit_cast(bf16x8, tile[base + r * 18 * c8]);
         };
-        // Software pipeline (MT <= 2).  A (weights, from L1/L2) is loaded kPA
-        // steps ahead in a ring of registers: with one or two wavefronts per SIMD
-        // nothing else hides a global load.  B (the NV LDS reads of a step) is
-        // read one step ahead, into the other of two register sets.  With MT >= 4
-        // a step's 4 NV MFMAs cover one step of A latency, and the accumulators
-        // leave no room for the rings: A one step ahead, B at the step's start.
-        if constexpr (MT <= 2) {
+        // Cin >= 32: the pass's 27 steps are one tap each, unrolled, so every LDS
+        // and weight address is a per-lane base plus an immediate.  A (weights,
+        // from L1/L2) is loaded kPA steps ahead in a ring of registers; B (the NV
+        // LDS reads of a step) one step ahead into the other of two register
+        // sets.  A scheduling barrier closes each step, so the compiler keeps the
+        // loads where they are issued (left free, it moved each next to its use;
+        // with the steps computed at run time, their address arithmetic cost
+        // about 90 VALU per step).  MT = 8 (128 output channels) has no room for
+        // the rings beside its 128 accumulator registers and takes the run-time
+        // loop below.
+        if constexpr (CIN >= 32 && MT <= 4) {
+            int lb[3];                        // per-lane base of a B read, by dx
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+                lb[dx] = bufo + (wave * TY * 18 + col + dx) * c8 + (kg ^ cnn_swz<c8>(col + dx));
+            const u16* wlp = wl + pass * 32;
+            constexpr int kPA = MT <= 2 ? 4 : 2;
+            bf16x8 ar[kPA][MT];
+#pragma unroll
+            for (int i = 0; i < kPA; ++i)
+#pragma unroll
+                for (int tt = 0; tt < MT; ++tt)
+                    ar[i][tt] = load8(wlp + (long long)16 * tt * kp + i * CIN);
+            bf16x8 bb[2][NV];
+#pragma unroll
+            for (int r = 0; r < NV; ++r)
+                bb[0][r] = __builtin_bit_cast(bf16x8, tile[lb[0] + r * 18 * c8]);
+#pragma unroll
+            for (int c = 0; c < 27; ++c) {
+                if (c + 1 < 27) {
+                    const int tn = c + 1, dz = tn / 9, dy = (tn / 3) % 3, dx = tn % 3;
+#pragma unroll
+                    for (int r = 0; r < NV; ++r)
+                        bb[(c + 1) & 1][r] = __builtin_bit_cast(
+                            bf16x8, tile[lb[dx] + ((dz * TY + dy) * 18 + r * 18) * c8]);
+                }
+#pragma unroll
+                for (int r = 0; r < NV; ++r)
+#pragma unroll
+                    for (int tt = 0; tt < MT; ++tt)
+                        acc[tt][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            ar[c % kPA][tt], bb[c & 1][r], acc[tt][r], 0, 0, 0);
+                if (c + kPA < 27) {
+#pragma unroll
+                    for (int tt = 0; tt < MT; ++tt)
+                        ar[c % kPA][tt] = load8(wlp + (long long)16 * tt * kp + (c + kPA) * CIN);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else if constexpr (MT <= 2) {
+            // Cin < 32, MT <= 2: the same pipeline over run-time steps (a step
+            // spans taps: 32 consecutive k)
             constexpr int kPA = 4;
             bf16x8 ar[kPA][MT];
 #pragma unroll
@@ -325,6 +384,8 @@ __global__ __launch_bounds__(256, 2) void conv3_fwd_tiled(u16* __restrict__ y,
                 }
             }
         } else {
+            // MT >= 4 (Cin < 32, or MT = 8): A one step ahead, B at the step's
+            // start (the accumulators leave no room for the rings)
             bf16x8 a[MT], an[MT];
             {
                 const u16* pa = aaddr(0);
@@ -906,7 +967,11 @@ void cnn_conv3_fwd(void* y, const void* x, const void* w, const float* bias, con
         const int tilesx = (v.W + 15) / 16, tilesy = (v.H + nv - 1) / nv, tilesz = (v.D + 3) / 4;
         const long long nblk = (long long)v.B * tilesz * tilesy * tilesx;
         ARECON_REQUIRE(nblk < (1LL << 31), "cnn: volume too large for one launch");
-        const size_t lds = (size_t)6 * (nv + 2) * 18 * std::min(cin, 32) * 2;   // one pass
+        // the tile's 16-byte slots padded to whole 64-slot staging instructions
+        const size_t slots = ((size_t)6 * (nv + 2) * 18 * (std::min(cin, 32) / 8) + 63) / 64 * 64;
+        const size_t lds = slots * 16;
+        ARECON_REQUIRE((long long)6 * v.H * v.W * cin * 2 < (1LL << 31),
+                       "cnn: a 6-plane tile of this volume exceeds the staging descriptor's range");
         const dim3 grid((unsigned)nblk), block(256);
         auto* Y = static_cast<u16*>(y);
         auto* X = static_cast<const u16*>(x);
