@@ -173,18 +173,6 @@ __global__ __launch_bounds__(256) void conv3_fwd(u16* __restrict__ y, const u16*
 // chunks of 32), so every output sees the same MFMA sequence and the result is
 // identical.
 // ---------------------------------------------------------------------------
-// LDS swizzle of the tiled convolution's input tile: a voxel's C/8 16-byte
-// pieces are stored in the order p ^ cnn_swz(hx), hx its x in the tile.  The 16
-// lanes of a B read take 16 consecutive voxels' same piece; unswizzled, their
-// voxel stride of C * 2 bytes put them on 2 (C = 64) or 4 (C = 32) of the 16
-// four-bank groups of gfx950's 64-bank LDS, an 8- or 4-way conflict; swizzled
-// they cover all 16.
-template <int C8> __device__ __forceinline__ int cnn_swz(int hx) {
-    static_assert(C8 >= 1 && C8 <= 16 && (C8 & (C8 - 1)) == 0, "pieces per voxel");
-    constexpr int sh = C8 == 1 ? 4 : C8 == 2 ? 3 : C8 == 4 ? 2 : C8 == 8 ? 1 : 0;
-    return (hx >> sh) & (C8 - 1);
-}
-
 template <int MT, int NV, int CIN>
 __global__ __launch_bounds__(256, 2) void conv3_fwd_tiled(u16* __restrict__ y,
                                                        const u16* __restrict__ x,
@@ -221,7 +209,7 @@ __global__ __launch_bounds__(256, 2) void conv3_fwd_tiled(u16* __restrict__ y,
     const u16* wl = w + (long long)col * kp + kg * 8;
     // Staging: direct global -> LDS loads (no VGPRs, asynchronous), 64 slots per
     // instruction, the wavefronts taking every fourth instruction.  Lane slot s
-    // of the tile (voxel-major, pieces swizzled) loads its voxel's piece through a
+    // of the tile (voxel-major) loads its voxel's piece through a
     // buffer descriptor based at the halo's origin voxel; a voxel outside the
     // volume (or a padding slot) gets an offset past the descriptor's range and
     // loads zeros.  One buffer (69 KB at most): two workgroups share a CU, and
@@ -249,7 +237,7 @@ __global__ __launch_bounds__(256, 2) void conv3_fwd_tiled(u16* __restrict__ y,
                 const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
                 if (gz >= 0 && gz < v.D && gy >= 0 && gy < v.H && gx >= 0 && gx < v.W)
                     off = (unsigned)(((((long long)hz * v.H + hy) * v.W + hx) * cin +
-                                      8 * (pp ^ cnn_swz<c8>(hx))) * 2);
+                                      8 * pp) * 2);
             }
             __builtin_amdgcn_raw_ptr_buffer_load_lds(
                 rs, (lds_void*)((lds_char*)tile + ((long long)buf * NSLOTP + 64 * j) * 16), 16, off,
@@ -291,7 +279,7 @@ __global__ __launch_bounds__(256, 2) void conv3_fwd_tiled(u16* __restrict__ y,
             kstep(c, k0, tap, piece);
             const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;     // halo-tile offsets
             const int base = bufo + (((wave + dz) * TY + dy) * 18 + col + dx) * c8 +
-                             (piece ^ cnn_swz<c8>(col + dx));
+                             piece;
 #pragma unroll
             for (int r = 0; r < NV; ++r)
                 bv[r] = __builtin_bit_cast(bf16x8, tile[base + r * 18 * c8]);
@@ -310,7 +298,7 @@ __global__ __launch_bounds__(256, 2) void conv3_fwd_tiled(u16* __restrict__ y,
             int lb[3];                        // per-lane base of a B read, by dx
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx)
-                lb[dx] = bufo + (wave * TY * 18 + col + dx) * c8 + (kg ^ cnn_swz<c8>(col + dx));
+                lb[dx] = bufo + (wave * TY * 18 + col + dx) * c8 + kg;
             const u16* wlp = wl + pass * 32;
             constexpr int kPA = MT <= 2 ? 4 : 2;
             bf16x8 ar[kPA][MT];
