@@ -737,6 +737,88 @@ __global__ void to_channel_major3_8(u16* __restrict__ xT3, const u16* __restrict
     }
 }
 
+// The same two layouts through LDS, when W is a multiple of 8 and C of 8 and at
+// most 128: a workgroup takes 64 consecutive voxels (with one voxel of halo on
+// each side for the shifted copies), loads their channels-last block (one
+// contiguous run of memory) with 16-byte loads, and writes the channel-major
+// rows in 16-byte pieces of 8 voxels, eight lanes per 128-byte row segment.
+// (The 8-voxel-run kernels above write 16 bytes per lane to 64 different rows
+// per instruction: 1.5 TB/s.)
+constexpr int kLtV = 64;                                  // voxels per workgroup
+
+__global__ __launch_bounds__(256) void to_channel_major3_lds(u16* __restrict__ xT3,
+                                                            const u16* __restrict__ x, Vol v,
+                                                            int c) {
+    __shared__ uint4 blk4[(kLtV + 2) * 128 / 8];
+    u16* blk = reinterpret_cast<u16*>(blk4);            // [kLtV + 2][c], voxel n0 - 1 first
+    const long long nv = v.nvox();
+    const long long n0 = (long long)blockIdx.x * kLtV;
+    const int c8 = c / 8;
+    // voxels n0 - 1 .. n0 + kLtV (those outside [0, nv) load as zeros)
+    for (int i = threadIdx.x; i < (kLtV + 2) * c8; i += 256) {
+        const int vv = i / c8, p = i - vv * c8;
+        const long long n = n0 - 1 + vv;
+        uint4 val = make_uint4(0u, 0u, 0u, 0u);
+        if (n >= 0 && n < nv) val = *reinterpret_cast<const uint4*>(x + n * c + 8 * p);
+        blk4[i] = val;
+    }
+    __syncthreads();
+    // pieces: (shift, channel, 8-voxel run q)
+    for (int i = threadIdx.x; i < 3 * c * (kLtV / 8); i += 256) {
+        const int q = i & 7, r = i >> 3;
+        const int ch = r % c, sft = r / c;
+        const long long nq = n0 + 8 * q;
+        if (nq >= nv) continue;
+        const int xq = (int)(nq % v.W);                  // the run lies in one row
+        u16 e[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int xx = xq + k + sft - 1;              // x of the voxel copied
+            e[k] = (xx >= 0 && xx < v.W) ? blk[(8 * q + k + sft) * c + ch] : (u16)0;
+        }
+        *reinterpret_cast<uint4*>(xT3 + ((long long)sft * c + ch) * nv + nq) = pack_u16x8(e);
+    }
+}
+
+__global__ __launch_bounds__(256) void relu_bwd_lds(u16* __restrict__ dyp, u16* __restrict__ dypT,
+                                                   const u16* __restrict__ dy,
+                                                   const u16* __restrict__ y, long long nv, int c,
+                                                   int relu) {
+    __shared__ uint4 blk4[kLtV * 128 / 8];
+    u16* blk = reinterpret_cast<u16*>(blk4);            // [kLtV][c]: the masked gradient
+    const long long n0 = (long long)blockIdx.x * kLtV;
+    const int c8 = c / 8;
+    for (int i = threadIdx.x; i < kLtV * c8; i += 256) {
+        const long long e0 = n0 * c + 8LL * i;           // element offset of the piece
+        if (e0 >= nv * c) continue;
+        uint4 g = *reinterpret_cast<const uint4*>(dy + e0);
+        if (relu) {
+            const uint4 yv = *reinterpret_cast<const uint4*>(y + e0);
+            unsigned gw[4] = {g.x, g.y, g.z, g.w};
+            const unsigned yw[4] = {yv.x, yv.y, yv.z, yv.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const unsigned lo = bf2f((u16)(yw[k] & 0xffffu)) > 0.f ? 0x0000ffffu : 0u;
+                const unsigned hi = bf2f((u16)(yw[k] >> 16)) > 0.f ? 0xffff0000u : 0u;
+                gw[k] &= lo | hi;
+            }
+            g = make_uint4(gw[0], gw[1], gw[2], gw[3]);
+        }
+        *reinterpret_cast<uint4*>(dyp + e0) = g;
+        blk4[i] = g;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < c * (kLtV / 8); i += 256) {
+        const int q = i & 7, ch = i >> 3;
+        const long long nq = n0 + 8 * q;
+        if (nq >= nv) continue;
+        u16 e[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) e[k] = blk[(8 * q + k) * c + ch];
+        *reinterpret_cast<uint4*>(dypT + (long long)ch * nv + nq) = pack_u16x8(e);
+    }
+}
+
 // 2x2x2 average pooling and its adjoint; nearest 2x upsampling and its adjoint.
 // One thread per voxel and 8 channels (one 16-byte piece; C is a multiple of 8).
 __device__ __forceinline__ void unpack8(const uint4& q, float* f) {
@@ -1076,7 +1158,12 @@ void cnn_conv3_wgrad(float* dw, const void* dyT, const void* xT3, const int* dim
 void cnn_relu_bwd(void* dyp, void* dypT, float* db, const void* dy, const void* y,
                   const int* dims, int c, int relu, hipStream_t s) {
     const Vol v = mkvol(dims);
-    if (v.W % 8 == 0)
+    if (v.W % 8 == 0 && c % 8 == 0 && c <= 128)
+        hipLaunchKernelGGL(relu_bwd_lds, dim3((unsigned)((v.nvox() + kLtV - 1) / kLtV)), dim3(256), 0,
+                           s, static_cast<u16*>(dyp), static_cast<u16*>(dypT),
+                           static_cast<const u16*>(dy), static_cast<const u16*>(y), v.nvox(), c,
+                           relu);
+    else if (v.W % 8 == 0)
         hipLaunchKernelGGL(relu_bwd8, dim3(grid_for(v.nvox() / 8 * c)), dim3(256), 0, s,
                            static_cast<u16*>(dyp), static_cast<u16*>(dypT),
                            static_cast<const u16*>(dy), static_cast<const u16*>(y), v.nvox(), c,
@@ -1102,7 +1189,11 @@ void cnn_relu_bwd(void* dyp, void* dypT, float* db, const void* dy, const void* 
 
 void cnn_channel_major3(void* xT3, const void* x, const int* dims, int c, hipStream_t s) {
     const Vol v = mkvol(dims);
-    if (v.W % 8 == 0)
+    if (v.W % 8 == 0 && c % 8 == 0 && c <= 128)
+        hipLaunchKernelGGL(to_channel_major3_lds, dim3((unsigned)((v.nvox() + kLtV - 1) / kLtV)),
+                           dim3(256), 0, s, static_cast<u16*>(xT3), static_cast<const u16*>(x), v,
+                           c);
+    else if (v.W % 8 == 0)
         hipLaunchKernelGGL(to_channel_major3_8, dim3(grid_for(v.nvox() / 8 * c)), dim3(256), 0, s,
                            static_cast<u16*>(xT3), static_cast<const u16*>(x), v, c);
     else
