@@ -308,6 +308,74 @@ class HelicalFDK:
         main.wait_stream(bp)
         return self.volume
 
+    def fan_view_margin(self) -> int:
+        """Fan views beyond a parallel view that its rebinning reads: parallel
+        view i, channel j blends fan views ceil((i DeltaTheta + asin(t_j/SO)) /
+        DeltaFai) - 1 and the next (src/ParallelRebinningCBCurve.cpp:36-69), with
+        |t_j| at most max(PLC, YL-1-PLC) DeltaT."""
+        a = self.ra
+        t = max(abs(float(a.PLC0)), abs(a.YL - 1 - float(a.PLC0))) * abs(float(a.DeltaT))
+        spread = math.asin(min(1.0, t / float(a.SO))) / abs(float(a.DeltaFai))
+        return int(math.ceil(spread)) + 2
+
+    def run_streamed(self, host_raw: torch.Tensor, raw: torch.Tensor, nchunks: int = 8,
+                     ngroups: int = 8):
+        """One pass that starts from the raw sinogram in pinned host memory: its
+        views are copied into ``raw`` (on the device) in ``nchunks`` chunks on a
+        copy stream, and each chunk of parallel views is rebinned and filtered
+        on the current stream as soon as the fan views it reads have arrived,
+        while the slice blocks whose views are filtered are backprojected on a
+        third stream (as ``run_overlapped``).  The copy of the next pass waits
+        for this pass's rebinning.  Same result as copying, then ``run``.
+        Needs the slice-major workgroup order."""
+        assert host_raw.is_pinned() and raw.is_cuda and host_raw.shape == raw.shape
+        main = torch.cuda.current_stream()
+        if not hasattr(self, "_bp_stream"):
+            self._bp_stream = torch.cuda.Stream()
+            self._events = []
+        if not hasattr(self, "_copy_stream"):
+            self._copy_stream = torch.cuda.Stream()
+            self._copy_events = []
+        cp, bp = self._copy_stream, self._bp_stream
+        V = self.ViewN
+        bounds = [round(V * c / nchunks) for c in range(nchunks + 1)]
+        while len(self._events) < nchunks:
+            self._events.append(torch.cuda.Event())
+        while len(self._copy_events) < nchunks:
+            self._copy_events.append(torch.cuda.Event())
+        cp.wait_stream(main)            # the previous pass has read `raw`
+        with torch.cuda.stream(cp):
+            for c in range(nchunks):
+                v0, v1 = bounds[c], bounds[c + 1]
+                if v1 > v0:
+                    raw[v0:v1].copy_(host_raw[v0:v1], non_blocking=True)
+                self._copy_events[c].record(cp)
+        margin = self.fan_view_margin()
+        for c in range(nchunks):
+            v0, v1 = bounds[c], bounds[c + 1]
+            if v1 <= v0:
+                continue
+            need = min(V - 1, v1 - 1 + margin)
+            k = next(i for i in range(nchunks) if bounds[i + 1] > need)
+            main.wait_event(self._copy_events[k])
+            rebin_views_dev(self.rebinned, raw, self.setup.geom, v0, v1, main)
+            filter_dev(self.filtered[v0:v1], self.rebinned[v0:v1], self.setup, main, self.layout)
+            self._events[c].record(main)
+        nb = self.plan.nblocks
+        gb = [round(nb * g / ngroups) for g in range(ngroups + 1)]
+        fp = self.filtered[self.first_view:self.first_view + self.n_views]
+        for g in range(ngroups):
+            b0, b1 = gb[g], gb[g + 1]
+            if b1 <= b0:
+                continue
+            need = max((self.plan.block_views(b)[1] for b in range(b0, b1)), default=-1)
+            last_view = self.first_view + max(need, 0)
+            c = next((i for i in range(nchunks) if bounds[i + 1] > last_view), nchunks - 1)
+            bp.wait_event(self._events[c])
+            self.plan.execute_blocks(self.volume, fp, b0, b1, bp, self.layout)
+        main.wait_stream(bp)
+        return self.volume
+
     def run_timed(self, raw: torch.Tensor) -> StageTimes:
         """One pass with HIP events around each stage on the current stream."""
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]

@@ -572,6 +572,10 @@ def make_roofline(fdk, plain, results, probes, counters, wl):
     return roof
 
 
+# View chunks of the streamed host-to-device pass.
+STREAM_CHUNKS = 16
+
+
 def h2d_inclusive(raw, results, chosen, args):
     """The pass including the host-to-device copy of the raw sinogram from
     pinned host memory (the reference copies the sinogram to the device in
@@ -601,6 +605,18 @@ def h2d_inclusive(raw, results, chosen, args):
         ms = ev[0].elapsed_time(ev[1]) / steps
         out[name] = {"ms_per_step": round(ms, 3),
                      "value": round(fdk.updates / (ms * 1e-3) / 1e9, 2)}
+        # the copy streamed in view chunks, overlapped with rebin + filter of
+        # the chunks that have arrived and the backprojection of ready blocks
+        fdk.run_streamed(host, dev, nchunks=STREAM_CHUNKS, ngroups=STREAM_CHUNKS)
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(steps):
+            fdk.run_streamed(host, dev, nchunks=STREAM_CHUNKS, ngroups=STREAM_CHUNKS)
+        ev[1].record()
+        ev[1].synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / steps
+        out[name + "_streamed"] = {"ms_per_step": round(ms, 3), "chunks": STREAM_CHUNKS,
+                                   "value": round(fdk.updates / (ms * 1e-3) / 1e9, 2)}
     rb = chosen.get("recon")
     if rb is not None:
         target = rb.raws[0]
