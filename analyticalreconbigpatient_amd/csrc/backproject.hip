@@ -2083,6 +2083,19 @@ static const BPPlan::ZsTable& zs_table(const BPPlan* P, hipStream_t s) {
     return z;
 }
 
+// Slot rows of the LDS-staged kernel (a view's footprint loads in multiples of
+// four rows, fewer than RMAX; the slot's last row is the zero row).  8-slice
+// blocks: 21 rows (a 20-row footprint), 21.5 KB of LDS per workgroup with
+// 8-byte records, seven workgroups per CU (25 rows: 10% of the clinical
+// workgroups gather -> 1%, but six workgroups per CU: 2.5% slower).  16-slice
+// blocks of 4-byte records: 48 rows, 24.6 KB, six workgroups per CU (the VGPRs
+// allow six): 40 rows leave 10% of the clinical workgroups to gather from
+// global memory, 48 rows 1.4% (backprojection 23.7 -> 23.1 ms).  16-slice
+// blocks of 8-byte records: 40 rows (40 KB, four workgroups per CU).
+template <int NZ, class T, int LY> constexpr int zs_rmax() {
+    return NZ == 8 ? 21 : (Gather<T, LY>::kElem == 4 ? 48 : 40);
+}
+
 // One launch of the LDS-staged kernel for layout LY.
 template <int NZ, class T, int LY, int RMAX>
 static void launch_zs(const BPPlan* P, BPArgs a, float* img, const T* fp, hipStream_t s,
@@ -2118,13 +2131,7 @@ static void launch_nz(const BPPlan* P, float* img, const T* fp, int layout, hipS
     }
     if constexpr (NZ == 8 || NZ == 16) {
         if (P->variant == 3) {
-            // rows staged per view: the row span of NZ slices plus margins.  Rows
-            // load in multiples of four, so 21 rows hold a 20-row footprint and
-            // the zero row: 25.9 KB of LDS per workgroup with 8-byte records, six
-            // workgroups per CU (24 rows fit the same footprints but only five
-            // workgroups: 9% slower); 15.1 KB with 4-byte records
-            constexpr int RMAX = NZ == 8 ? 21 : 40;
-#define ARECON_ZS_(LY) launch_zs<NZ, T, LY, RMAX>(P, a, img, fp, s, grid)
+#define ARECON_ZS_(LY) launch_zs<NZ, T, LY, zs_rmax<NZ, T, LY>()>(P, a, img, fp, s, grid)
             if constexpr (std::is_same<T, float>::value) {
                 ARECON_REQUIRE(layout == ARECON_FP_ROWPAIRS || layout == ARECON_FP_ROWDIFF,
                                "bp: variant 3 reads FP32 row pairs or row differences");
@@ -2244,16 +2251,17 @@ struct BPBatch {
 // of the single launches.
 template <int NZ, class T>
 static const BPPlan::ZsTable& batch_zs_table_t(const BPPlan* P, int layout) {
-    constexpr int RMAX = NZ == 8 ? 21 : 40;
-    if (layout == ARECON_FP_ROWDIFF) return zs_table<NZ, T, ARECON_FP_ROWDIFF, RMAX>(P, nullptr);
+    if (layout == ARECON_FP_ROWDIFF)
+        return zs_table<NZ, T, ARECON_FP_ROWDIFF, zs_rmax<NZ, T, ARECON_FP_ROWDIFF>()>(P, nullptr);
     if constexpr (std::is_same<T, float>::value) {
         ARECON_REQUIRE(layout == ARECON_FP_ROWPAIRS,
                        "bp batch: variant 3 reads FP32 row pairs or row differences");
-        return zs_table<NZ, T, ARECON_FP_ROWPAIRS, RMAX>(P, nullptr);
+        return zs_table<NZ, T, ARECON_FP_ROWPAIRS, zs_rmax<NZ, T, ARECON_FP_ROWPAIRS>()>(P,
+                                                                                     nullptr);
     } else {
         ARECON_REQUIRE(layout == ARECON_FP_QUADS,
                        "bp batch: variant 3 reads 16-bit quads or row differences");
-        return zs_table<NZ, T, ARECON_FP_QUADS, RMAX>(P, nullptr);
+        return zs_table<NZ, T, ARECON_FP_QUADS, zs_rmax<NZ, T, ARECON_FP_QUADS>()>(P, nullptr);
     }
 }
 template <int NZ>
@@ -2396,16 +2404,16 @@ static void batch_launch_nz(const BPBatch* B, hipStream_t s) {
 
 template <int NZ, class T, bool INSIDE>
 static void batch_launch_zs_in(const BPBatch* B, hipStream_t s) {
-    constexpr int RMAX = NZ == 8 ? 21 : 40;
     const dim3 grid((unsigned)B->grid), block(256);
+    constexpr int kD = ARECON_FP_ROWDIFF, kP = ARECON_FP_ROWPAIRS, kQ = ARECON_FP_QUADS;
     if (B->layout == ARECON_FP_ROWDIFF)
-        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, ARECON_FP_ROWDIFF, RMAX, INSIDE>), grid,
+        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, kD, zs_rmax<NZ, T, kD>(), INSIDE>), grid,
                            block, 0, s, B->d_slabs, B->nslabs);
     else if constexpr (std::is_same<T, float>::value)
-        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, ARECON_FP_ROWPAIRS, RMAX, INSIDE>), grid,
+        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, kP, zs_rmax<NZ, T, kP>(), INSIDE>), grid,
                            block, 0, s, B->d_slabs, B->nslabs);
     else
-        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, ARECON_FP_QUADS, RMAX, INSIDE>), grid,
+        hipLaunchKernelGGL((bp_fdk3dw_zs_batch<NZ, T, kQ, zs_rmax<NZ, T, kQ>(), INSIDE>), grid,
                            block, 0, s, B->d_slabs, B->nslabs);
 }
 template <int NZ, class T>

@@ -86,7 +86,8 @@ def kernel_name(fdk) -> str:
         g = 0 if fdk.group == 8 else fdk.group
         return f"bp_fdk3dw_zc<{fdk.plan_nz}, {g}, {t}, {lay}, {g == 0 and inside or 'false'}>"
     if fdk.variant == 3:
-        rmax = 21 if fdk.plan_nz == 8 else 40
+        rmax = 21 if fdk.plan_nz == 8 else (48 if fdk.storage != "f32" and
+                                            fdk.layout == "rowdiff" else 40)
         return f"bp_fdk3dw_zs<{fdk.plan_nz}, {t}, {lay}, {rmax}, {inside}>"
     return f"bp_fdk3dw{'_vt' if fdk.variant == 0 else ''}<{fdk.plan_nz}, {t}>"
 
@@ -948,8 +949,16 @@ def bench_cnn(args):
     spec = learned.PairSpec()
     cspec = learned.clinical_spec()
     t0 = time.perf_counter()
-    pairs = [learned.make_pair(1000 + i, spec) for i in range(args.cnn_pairs)]
-    cpairs = [learned.make_pair(3000 + i, cspec) for i in range(args.cnn_clinical_pairs)]
+    def make(seed0, n, sp, what):
+        out = []
+        for i in range(n):
+            out.append(learned.make_pair(seed0 + i, sp))
+            if (i + 1) % 50 == 0:
+                log(f"{what}: {i + 1} of {n} pairs ({time.perf_counter() - t0:.1f} s)")
+        return out
+
+    pairs = make(1000, args.cnn_pairs, spec, "368x16")
+    cpairs = make(3000, args.cnn_clinical_pairs, cspec, "888x64")
     held = [learned.make_pair(2000 + i, spec) for i in range(10)]
     cheld = [learned.make_pair(4000 + i, cspec) for i in range(10)]
     torch.cuda.synchronize()
