@@ -1113,7 +1113,12 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
 // Each wave stages one view of a chunk of four, then every wave consumes the
 // four views.
 // ---------------------------------------------------------------------------
-constexpr int kZsCW = 32;                    // records per staged row
+// Records per staged row: 28 for 8-byte records (224-byte rows, so that 25-row
+// slots of four views fit seven workgroups per CU), 32 for 4-byte records.  A
+// 16x16-column tile spans at most 16 sqrt(2) dx / channel pitch channels (21.8
+// on the 512^2 clinical grid) plus the blend's second channel and a channel
+// of margin on each side.
+__host__ __device__ constexpr int zs_cw(int rec_bytes) { return rec_bytes == 8 ? 28 : 32; }
 enum { ZS_SKIP = 0, ZS_LDS = 1, ZS_DIRECT = 2 };
 
 typedef __attribute__((address_space(3))) char lds_char;
@@ -1176,9 +1181,12 @@ __device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsig
     const float eY = ex * fabsf(cs.y) + ey * fabsf(cs.x);
     const float eU = ex * fabsf(cs.x) + ey * fabsf(cs.y);
     const float ua = (Yrc - eY) * A.invdYL + A.YLC, ub = (Yrc + eY) * A.invdYL + A.YLC;
-    int c0 = (int)floorf(fminf(ua, ub)) - 2;
+    // a lane's channel pair (floor(U1), floor(U1) + 1) lies in [c0, ceil(ub) + 1]
+    // with a margin of kM - 1 channels on each side
+    constexpr int kCW = zs_cw(PAIRS ? 8 : 4), kM = kCW == 32 ? 2 : 1;
+    int c0 = (int)floorf(fminf(ua, ub)) - kM;
     if constexpr (!INSIDE && PAIRS) c0 &= ~1;           // round down to even
-    if ((int)ceilf(fmaxf(ua, ub)) + 2 > c0 + kZsCW - 1) return v;
+    if ((int)ceilf(fmaxf(ua, ub)) + kM > c0 + kCW - 1) return v;
     const float ya = fmaxf(fabsf(Yrc) - eY, 0.f), yb = fabsf(Yrc) + eY;
     const float t1lo2 = A.SO2 - yb * yb;
     if (!(t1lo2 > 0.f)) return v;
@@ -1203,7 +1211,7 @@ __device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsig
         // last record of the view: channel nrec-1 of row ZL-2, staged as the first
         // of a pair (an even distance from c0) when both lie in the footprint
         const int last = zlm1 - 1;
-        if (rB + 1 - nl <= last && last <= rB && nrec - 1 <= c0 + kZsCW - 1 &&
+        if (rB + 1 - nl <= last && last <= rB && nrec - 1 <= c0 + kCW - 1 &&
             ((nrec - 1 - c0) & 1) == 0)
             return v;
     }
@@ -1277,9 +1285,11 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     constexpr int kRec = (int)Gather<T, LAYOUT>::kElem;   // bytes per record
     static_assert(kRec == 8 || kRec == 4, "LDS staging takes 8- or 4-byte records");
     static_assert(ZB <= 16 && RMAX >= 5, "bad block shape");
-    constexpr int kRowB = kZsCW * kRec;                    // bytes per staged row
-    constexpr int kRowShift = kRec == 8 ? 8 : 7, kRecShift = kRec == 8 ? 3 : 2;
-    static_assert(kRowB == 1 << kRowShift && kRec == 1 << kRecShift, "shifts");
+    constexpr int kCW = zs_cw(kRec);
+    constexpr int kRowB = kCW * kRec;                      // bytes per staged row
+    constexpr int kRecShift = kRec == 8 ? 3 : 2;
+    constexpr int kRowShift = kRowB == 256 ? 8 : 7;       // used when kRowB is a power of 2
+    static_assert(kRec == 1 << kRecShift, "record shift");
     constexpr int kSlot = RMAX * kRowB;
     __shared__ __attribute__((aligned(16))) char stage0[4 * kSlot];
     typedef Gather<T, LAYOUT> Gt;
@@ -1347,6 +1357,9 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
 #pragma unroll
     for (int j = 0; j < ZB; ++j) acc[j] = 0.f;
 
+    // a staging lane's row and record pair within a four-row load of 8-byte
+    // records
+    const int srow = lane / (kCW / 2), spair = lane - srow * (kCW / 2);
     // stage: wave w copies the footprint of view p0 + w into slot w
     auto stage_chunk = [&](int p0) {
         const int p = p0 + wave;
@@ -1364,17 +1377,19 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
                 const int nl = 4 * ((rB - rA + 4) / 4);
                 lds_char* dst = lstage + wave * kSlot + (RMAX - 1 - nl) * kRowB;
                 if constexpr (kRec == 8) {
-                    // 16-byte loads, four rows each.  Lane: row
-                    // rB + 1 - nl + i + lane / 16, records c0 + 2 (lane % 16) + {0, 1}
-                    const int off = (rB + 1 - nl + (lane >> 4)) * rowb +
-                                    (c0 + 2 * (lane & 15)) * kRec;
-                    for (int i = 0; i < nl; i += 4)
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                            rs, (lds_void*)(dst + i * kRowB), 16, (unsigned)(off + i * rowb),
-                            0, 0, 0);
+                    // 16-byte loads, four rows each (4 kCW / 2 lanes: the rest
+                    // idle).  Lane: row rB + 1 - nl + i + srow, records
+                    // c0 + 2 spair + {0, 1}
+                    const int off = (rB + 1 - nl + srow) * rowb + (c0 + 2 * spair) * kRec;
+                    if (lane < 2 * kCW)
+                        for (int i = 0; i < nl; i += 4)
+                            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                                rs, (lds_void*)(dst + i * kRowB), 16,
+                                (unsigned)(off + i * rowb), 0, 0, 0);
                 } else {
                     // 4-byte loads (one record per lane), two rows each.  Lane: row
                     // rB + 1 - nl + i + lane / 32, record c0 + lane % 32
+                    static_assert(kCW == 32, "one 64-lane load: two rows of 4-byte records");
                     const int off = (rB + 1 - nl + (lane >> 5)) * rowb + (c0 + (lane & 31)) * kRec;
                     for (int i = 0; i < nl; i += 2)
                         __builtin_amdgcn_raw_ptr_buffer_load_lds(
@@ -1456,26 +1471,51 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
                 // separate blocks the clamp's two paths would give every
                 // accumulator a copy at the join
                 int a[4];
-                asm("v_cvt_flr_i32_f32 %[a0], %[v0]\n\t"
-                    "v_cvt_flr_i32_f32 %[a1], %[v1]\n\t"
-                    "v_cvt_flr_i32_f32 %[a2], %[v2]\n\t"
-                    "v_cvt_flr_i32_f32 %[a3], %[v3]\n\t"
-                    "s_cmp_eq_u32 %[cl], 0\n\t"
-                    "s_cbranch_scc1 1f\n\t"
-                    "v_med3_i32 %[a0], %[a0], %[rA], %[rB]\n\t"
-                    "v_med3_i32 %[a1], %[a1], %[rA], %[rB]\n\t"
-                    "v_med3_i32 %[a2], %[a2], %[rA], %[rB]\n\t"
-                    "v_med3_i32 %[a3], %[a3], %[rA], %[rB]\n"
-                    "1:\n\t"
-                    "v_lshl_add_u32 %[a0], %[a0], %[sh], %[ct]\n\t"
-                    "v_lshl_add_u32 %[a1], %[a1], %[sh], %[ct]\n\t"
-                    "v_lshl_add_u32 %[a2], %[a2], %[sh], %[ct]\n\t"
-                    "v_lshl_add_u32 %[a3], %[a3], %[sh], %[ct]"
-                    : [a0] "=&v"(a[0]), [a1] "=&v"(a[1]), [a2] "=&v"(a[2]), [a3] "=&v"(a[3])
-                    : [v0] "v"(V1[0]), [v1] "v"(V1[1]), [v2] "v"(V1[2]), [v3] "v"(V1[3]),
-                      [cl] "s"(clamp), [rA] "s"(rA), [rB] "v"(rBv), [ct] "v"(colterm),
-                      [sh] "i"(kRowShift)
-                    : "scc");
+                // row * kRowB + colterm: a shift-add for power-of-two rows, else a
+                // signed 24-bit multiply-add (the row may be -1)
+                if constexpr ((kRowB & (kRowB - 1)) == 0) {
+                    asm("v_cvt_flr_i32_f32 %[a0], %[v0]\n\t"
+                        "v_cvt_flr_i32_f32 %[a1], %[v1]\n\t"
+                        "v_cvt_flr_i32_f32 %[a2], %[v2]\n\t"
+                        "v_cvt_flr_i32_f32 %[a3], %[v3]\n\t"
+                        "s_cmp_eq_u32 %[cl], 0\n\t"
+                        "s_cbranch_scc1 1f\n\t"
+                        "v_med3_i32 %[a0], %[a0], %[rA], %[rB]\n\t"
+                        "v_med3_i32 %[a1], %[a1], %[rA], %[rB]\n\t"
+                        "v_med3_i32 %[a2], %[a2], %[rA], %[rB]\n\t"
+                        "v_med3_i32 %[a3], %[a3], %[rA], %[rB]\n"
+                        "1:\n\t"
+                        "v_lshl_add_u32 %[a0], %[a0], %[rw], %[ct]\n\t"
+                        "v_lshl_add_u32 %[a1], %[a1], %[rw], %[ct]\n\t"
+                        "v_lshl_add_u32 %[a2], %[a2], %[rw], %[ct]\n\t"
+                        "v_lshl_add_u32 %[a3], %[a3], %[rw], %[ct]"
+                        : [a0] "=&v"(a[0]), [a1] "=&v"(a[1]), [a2] "=&v"(a[2]), [a3] "=&v"(a[3])
+                        : [v0] "v"(V1[0]), [v1] "v"(V1[1]), [v2] "v"(V1[2]), [v3] "v"(V1[3]),
+                          [cl] "s"(clamp), [rA] "s"(rA), [rB] "v"(rBv), [ct] "v"(colterm),
+                          [rw] "i"(kRowShift)
+                        : "scc");
+                } else {
+                    asm("v_cvt_flr_i32_f32 %[a0], %[v0]\n\t"
+                        "v_cvt_flr_i32_f32 %[a1], %[v1]\n\t"
+                        "v_cvt_flr_i32_f32 %[a2], %[v2]\n\t"
+                        "v_cvt_flr_i32_f32 %[a3], %[v3]\n\t"
+                        "s_cmp_eq_u32 %[cl], 0\n\t"
+                        "s_cbranch_scc1 1f\n\t"
+                        "v_med3_i32 %[a0], %[a0], %[rA], %[rB]\n\t"
+                        "v_med3_i32 %[a1], %[a1], %[rA], %[rB]\n\t"
+                        "v_med3_i32 %[a2], %[a2], %[rA], %[rB]\n\t"
+                        "v_med3_i32 %[a3], %[a3], %[rA], %[rB]\n"
+                        "1:\n\t"
+                        "v_mad_i32_i24 %[a0], %[a0], %[rw], %[ct]\n\t"
+                        "v_mad_i32_i24 %[a1], %[a1], %[rw], %[ct]\n\t"
+                        "v_mad_i32_i24 %[a2], %[a2], %[rw], %[ct]\n\t"
+                        "v_mad_i32_i24 %[a3], %[a3], %[rw], %[ct]"
+                        : [a0] "=&v"(a[0]), [a1] "=&v"(a[1]), [a2] "=&v"(a[2]), [a3] "=&v"(a[3])
+                        : [v0] "v"(V1[0]), [v1] "v"(V1[1]), [v2] "v"(V1[2]), [v3] "v"(V1[3]),
+                          [cl] "s"(clamp), [rA] "s"(rA), [rB] "v"(rBv), [ct] "v"(colterm),
+                          [rw] "s"(kRowB)
+                        : "scc");
+                }
                 typename Gt::Raw q[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) q[i] = Lg::load(lstage + s * kSlot + a[i]);
@@ -2085,15 +2125,16 @@ static const BPPlan::ZsTable& zs_table(const BPPlan* P, hipStream_t s) {
 
 // Slot rows of the LDS-staged kernel (a view's footprint loads in multiples of
 // four rows, fewer than RMAX; the slot's last row is the zero row).  8-slice
-// blocks: 21 rows (a 20-row footprint), 21.5 KB of LDS per workgroup with
-// 8-byte records, seven workgroups per CU (25 rows: 10% of the clinical
-// workgroups gather -> 1%, but six workgroups per CU: 2.5% slower).  16-slice
-// blocks of 4-byte records: 48 rows, 24.6 KB, six workgroups per CU (the VGPRs
-// allow six): 40 rows leave 10% of the clinical workgroups to gather from
-// global memory, 48 rows 1.4% (backprojection 23.7 -> 23.1 ms).  16-slice
-// blocks of 8-byte records: 40 rows (40 KB, four workgroups per CU).
+// blocks of 8-byte records: 25 rows of 224 bytes, 21.9 KB of LDS per
+// workgroup, seven workgroups per CU (21 rows of 256 bytes, the same LDS, left
+// 10% of the clinical workgroups with a footprint too tall to stage, gathering
+// from global memory; 25 rows, 1%).  16-slice blocks of 4-byte records: 48
+// rows, 24.6 KB, six workgroups per CU (as the VGPRs allow): 40 rows leave 10%
+// of the clinical workgroups to gather, 48 rows 1.4% (backprojection 23.7 ->
+// 23.1 ms).  8-slice blocks of 4-byte records: 21 rows; 16-slice blocks of
+// 8-byte records: 40 rows (35 KB, four workgroups per CU).
 template <int NZ, class T, int LY> constexpr int zs_rmax() {
-    return NZ == 8 ? 21 : (Gather<T, LY>::kElem == 4 ? 48 : 40);
+    return Gather<T, LY>::kElem == 4 ? (NZ == 8 ? 21 : 48) : (NZ == 8 ? 25 : 40);
 }
 
 // One launch of the LDS-staged kernel for layout LY.

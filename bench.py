@@ -86,8 +86,8 @@ def kernel_name(fdk) -> str:
         g = 0 if fdk.group == 8 else fdk.group
         return f"bp_fdk3dw_zc<{fdk.plan_nz}, {g}, {t}, {lay}, {g == 0 and inside or 'false'}>"
     if fdk.variant == 3:
-        rmax = 21 if fdk.plan_nz == 8 else (48 if fdk.storage != "f32" and
-                                            fdk.layout == "rowdiff" else 40)
+        rec4 = fdk.storage != "f32" and fdk.layout == "rowdiff"     # 4-byte records
+        rmax = (21 if fdk.plan_nz == 8 else 48) if rec4 else (25 if fdk.plan_nz == 8 else 40)
         return f"bp_fdk3dw_zs<{fdk.plan_nz}, {t}, {lay}, {rmax}, {inside}>"
     return f"bp_fdk3dw{'_vt' if fdk.variant == 0 else ''}<{fdk.plan_nz}, {t}>"
 
