@@ -79,6 +79,10 @@ SIGNATURES = {
                                     C.POINTER(_p)]),
     "arecon_bp_batch_destroy": (_i, [_p]),
     "arecon_bp_batch_execute": (_i, [_p, _p]),
+    "arecon_graph_capture_begin": (_i, [_p]),
+    "arecon_graph_capture_end": (_i, [_p, C.POINTER(_p)]),
+    "arecon_graph_launch": (_i, [_p, _p]),
+    "arecon_graph_destroy": (_i, [_p]),
     # include/arecon_cnn.h
     "arecon_cnn_conv3": (_i, [_p, _p, _p, _p, _p, _ip, _i, _i, _i, _p]),
     "arecon_cnn_conv3_wgrad": (_i, [_p, _p, _p, _ip, _i, _i, _p]),

@@ -90,6 +90,44 @@ def slab_params(p: BPParams, slab: Slab) -> BPParams:
     return dataclasses.replace(p, ZN=slab.nz, ZNC=-slab.z0 / slab.dz)
 
 
+class Graph:
+    """Owner of an ``arecon_graph``: a pass captured on a stream and
+    instantiated by libarecon (hipStreamBeginCapture / EndCapture /
+    GraphInstantiate), replayed with ``launch``."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    @classmethod
+    def capture(cls, issue, stream: torch.cuda.Stream) -> "Graph":
+        """Capture the calls ``issue()`` makes on ``stream``."""
+        sp = _stream_ptr(stream)
+        _lib.check(_lib.lib().arecon_graph_capture_begin(sp), "arecon_graph_capture_begin")
+        h = C.c_void_p()
+        try:
+            issue()
+        finally:
+            # ends the capture even when issue() raised
+            status = _lib.lib().arecon_graph_capture_end(sp, C.byref(h))
+        _lib.check(status, "arecon_graph_capture_end")
+        return cls(h)
+
+    def launch(self, stream=None) -> None:
+        _lib.check(_lib.lib().arecon_graph_launch(self._h, _stream_ptr(stream)),
+                   "arecon_graph_launch")
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            _lib.lib().arecon_graph_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class BatchedRecon:
     """Scans (setup + raw sinogram on the device) and slabs, reconstructed in
     one pass.  ``volumes[i]`` is slab i's [YN][XN][nz] FP32 volume."""
@@ -148,6 +186,8 @@ class BatchedRecon:
         self._batch = h
 
     def _release_batch(self):
+        if self.graph is not None:           # before the batch whose slabs it reads
+            self.graph.close()
         self.graph = None
         if self._batch is not None and self._batch.value:
             _lib.lib().arecon_bp_batch_destroy(self._batch)
@@ -186,25 +226,23 @@ class BatchedRecon:
         self.prepare(stream)
         self.backproject(stream)
 
-    def capture(self) -> torch.cuda.CUDAGraph:
-        """Record one pass as a HIP graph.  Run the pass once eagerly first so
-        every cached table (filter spectra, rebinning coefficients) exists."""
+    def capture(self) -> "Graph":
+        """Record one pass as a HIP graph through the library's capture entry
+        points (arecon_graph_capture_begin / _end).  Runs the pass once eagerly
+        first so every cached table (filter spectra, rebinning coefficients,
+        footprint tables) exists."""
         self.run()
         torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            with torch.cuda.graph(g, stream=s):
-                self.run(s)
-        torch.cuda.current_stream().wait_stream(s)
+        self.graph = None
+        self.graph = Graph.capture(lambda: self.run(s), s)
         torch.cuda.synchronize()
-        self.graph = g
-        return g
+        return self.graph
 
-    def replay(self) -> None:
+    def replay(self, stream=None) -> None:
         assert self.graph is not None, "capture() first"
-        self.graph.replay()
+        self.graph.launch(stream)
 
     # -- tuning ------------------------------------------------------------
     def signature(self) -> str:

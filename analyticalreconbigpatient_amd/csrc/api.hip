@@ -114,6 +114,13 @@ void fdk_weights(double* w, int N_2pi, double HSCoef, int delta) {
 
 using namespace arecon;
 
+// An instantiated HIP graph of a captured pass (include/arecon.h).
+struct arecon_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    int device;
+};
+
 extern "C" {
 
 int arecon_last_error(char* buf, size_t len) {
@@ -392,6 +399,54 @@ int arecon_bp_plan_execute_blocks(const arecon_bp_plan* plan, float* d_img, cons
     return guarded([&] {
         bp_plan_launch(reinterpret_cast<const BPPlan*>(plan), d_img, d_fp, fp_dtype, fp_layout,
                        static_cast<hipStream_t>(stream), block_begin, block_end);
+    });
+}
+
+int arecon_graph_capture_begin(void* stream) {
+    return guarded([&] {
+        ARECON_REQUIRE(stream != nullptr, "arecon_graph_capture_begin: the null stream cannot "
+                                          "be captured");
+        ARECON_HIP(hipStreamBeginCapture(static_cast<hipStream_t>(stream),
+                                         hipStreamCaptureModeThreadLocal));
+    });
+}
+
+int arecon_graph_capture_end(void* stream, arecon_graph** out) {
+    return guarded([&] {
+        ARECON_REQUIRE(stream != nullptr && out != nullptr,
+                       "arecon_graph_capture_end: null argument");
+        *out = nullptr;
+        hipGraph_t graph = nullptr;
+        // end the capture first, whatever follows: a stream left capturing
+        // would fail every later call on it
+        ARECON_HIP(hipStreamEndCapture(static_cast<hipStream_t>(stream), &graph));
+        ARECON_REQUIRE(graph != nullptr, "arecon_graph_capture_end: nothing was captured");
+        auto* g = new arecon_graph{graph, nullptr, 0};
+        const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            delete g;
+            ARECON_HIP(e);
+        }
+        (void)hipGetDevice(&g->device);
+        *out = g;
+    });
+}
+
+int arecon_graph_launch(const arecon_graph* graph, void* stream) {
+    return guarded([&] {
+        ARECON_REQUIRE(graph != nullptr && graph->exec != nullptr,
+                       "arecon_graph_launch: null graph");
+        ARECON_HIP(hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)));
+    });
+}
+
+int arecon_graph_destroy(arecon_graph* graph) {
+    return guarded([&] {
+        if (!graph) return;
+        if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+        if (graph->graph) (void)hipGraphDestroy(graph->graph);
+        delete graph;
     });
 }
 

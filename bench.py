@@ -67,9 +67,9 @@ def parse():
                          "cpu: the reference CPU path on the 368x16x2160 scan, 1 thread and "
                          "all threads")
     ap.add_argument("--cnn-pairs", type=int, default=200, help="training phantoms (cnn)")
-    ap.add_argument("--cnn-clinical-pairs", type=int, default=96,
+    ap.add_argument("--cnn-clinical-pairs", type=int, default=192,
                     help="training crops of the 888x64 scan (cnn)")
-    ap.add_argument("--cnn-steps", type=int, default=4000,
+    ap.add_argument("--cnn-steps", type=int, default=8000,
                     help="training steps of each network (cnn)")
     ap.add_argument("--cnn-seeds", type=int, default=3,
                     help="network seeds of the 888x64 correction (cnn)")

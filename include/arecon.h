@@ -278,6 +278,25 @@ int arecon_bp_batch_create(const arecon_bp_plan* const* plans, float* const* d_i
 int arecon_bp_batch_destroy(arecon_bp_batch* batch);
 int arecon_bp_batch_execute(const arecon_bp_batch* batch, void* stream);
 
+/* HIP-graph capture of a device-resident pass (no counterpart in the
+ * reference, whose functions are synchronous; for callers that replay the same
+ * pass many times, such as the batched reconstruction of several scans).
+ * arecon_graph_capture_begin starts capturing `stream` (not the null stream);
+ * the pass's calls on that stream (arecon_rebin_dev, arecon_filter_dev,
+ * arecon_bp_plan_execute, arecon_bp_batch_execute, ...) are then recorded, not
+ * run; arecon_graph_capture_end stops the capture and instantiates the graph
+ * into *out (on failure the capture is ended and *out stays null).
+ * arecon_graph_launch replays it on any stream of the same device.  Run the
+ * pass once before capturing it: first calls build and cache tables (device
+ * allocations, a stream synchronization), which a capture cannot hold and
+ * which then fail with an error.  The graph reads and writes the buffers the
+ * captured calls named; they must outlive it. */
+typedef struct arecon_graph arecon_graph;
+int arecon_graph_capture_begin(void* stream);
+int arecon_graph_capture_end(void* stream, arecon_graph** out);
+int arecon_graph_launch(const arecon_graph* graph, void* stream);
+int arecon_graph_destroy(arecon_graph* graph);
+
 /* FP64 backprojection on device buffers (same math in double throughout).
  * Synchronises `stream` before returning (w is a host buffer). */
 int arecon_bp_dev_f64(double* d_img, const double* d_fp, const arecon_bp_geom* g,

@@ -87,3 +87,14 @@ def test_gateway_arity_checks():
     with pytest.raises(mexapi.MexArgumentError):
         mexapi.ParallelRebinningCBCurve(np.zeros((2, 2, 2), np.float32), 2, 2, 2, 1, 1, 1, 1, 1,
                                         1, 541, 0)
+
+
+def test_graph_capture_rejects_the_null_stream():
+    """Host-side argument checks of the capture entry points (no GPU needed)."""
+    import ctypes as C
+    lib = _lib.lib()
+    assert lib.arecon_graph_capture_begin(None) != 0
+    h = C.c_void_p()
+    assert lib.arecon_graph_capture_end(None, C.byref(h)) != 0
+    assert lib.arecon_graph_launch(None, None) != 0
+    assert lib.arecon_graph_destroy(None) == 0

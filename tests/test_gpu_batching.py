@@ -185,3 +185,39 @@ def test_lds_batch_mixing_inside_and_edge_slabs(gpu, nthreads):
             assert np.abs(want).max() > 0
             assert rel_l2(vol.cpu().numpy(), want) < 1e-4, sl
         r.close()
+
+
+def test_c_abi_graph_of_a_filter_pass(gpu):
+    """arecon_graph_capture_begin / _end around a filter call on a stream, the
+    graph replayed on another stream three times (eager filter calls between):
+    every replay equals the eager result."""
+    import ctypes as C
+    from analyticalreconbigpatient_amd import _lib
+    lib = _lib.lib()
+    YL, ZL, ViewN = 300, 6, 10
+    src = torch.randn(ViewN, ZL, YL, device="cuda")
+    out = torch.empty(ViewN, ZL - 1, YL, 2, device="cuda")
+    f32 = np.float32
+    args = (YL, ZL, ViewN, f32(149.5), f32(2.5), f32(0.95), f32(1.92), f32(541.0))
+
+    def filt(dst, stream):
+        _lib.check(lib.arecon_filter_dev(C.c_void_p(dst.data_ptr()), _lib.ARECON_F32,
+                                         _lib.ARECON_FP_ROWDIFF, C.c_void_p(src.data_ptr()),
+                                         *args, C.c_void_p(stream.cuda_stream)), "filter")
+
+    s = torch.cuda.Stream()
+    filt(out, s)                        # tables built and cached before the capture
+    s.synchronize()
+    want = out.clone()
+    g = B.Graph.capture(lambda: filt(out, s), s)
+    other = torch.cuda.Stream()
+    scratch = torch.empty_like(out)
+    for k in range(3):
+        out.fill_(float("nan"))
+        filt(scratch, torch.cuda.current_stream())       # eager work between replays
+        torch.cuda.synchronize()
+        g.launch(other)
+        other.synchronize()
+        assert torch.equal(out, want), f"replay {k}"
+        assert torch.equal(scratch, want)
+    g.close()
