@@ -403,8 +403,8 @@ def measure_counters(args, results):
     if not Path(exe).exists():
         log("counters: rocprofv3 not found")
         return None
-    passes = {"sq": ["SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "SQ_WAVES", "GRBM_GUI_ACTIVE",
-                     "FETCH_SIZE"],
+    passes = {"sq": ["SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "SQ_ACTIVE_INST_VALU2",
+                     "SQ_WAVES", "GRBM_GUI_ACTIVE", "FETCH_SIZE"],
               "wr": ["WRITE_SIZE"]}
     per_kernel: dict[str, dict] = {}
     for name, ctrs in passes.items():
@@ -459,7 +459,8 @@ def measure_counters(args, results):
             e["fetch_bytes"] = c["FETCH_SIZE"] * 1024.0 * fetch_f
         if "WRITE_SIZE" in c:
             e["write_bytes"] = c["WRITE_SIZE"] * 1024.0 * write_f
-        for n in ("SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "SQ_WAVES", "GRBM_GUI_ACTIVE"):
+        for n in ("SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "SQ_ACTIVE_INST_VALU2", "SQ_WAVES",
+                  "GRBM_GUI_ACTIVE"):
             if n in c:
                 e[n] = c[n]
         out["kernels"][k] = e
@@ -477,17 +478,29 @@ def _match_kernel(counters, name):
     return None
 
 
+def _issue_slots(k):
+    """VALU issue slots of a dispatch: a SIMD issues one wavefront VALU
+    instruction per quad-cycle, and gfx950 can issue a second in the same
+    quad-cycle (an FP32 FMA/add/mul whose operands are all VGPRs), so a slot
+    holds one or two instructions.  Slots = SQ_INSTS_VALU - SQ_ACTIVE_INST_VALU2
+    (quad-cycles that issued two).  Transcendentals run beside the VALU and are
+    left out, which can only lower the count and so raise the roof."""
+    return k["SQ_INSTS_VALU"] - k.get("SQ_INSTS_VALU_TRANS_F32", 0.0) - \
+        k.get("SQ_ACTIVE_INST_VALU2", 0.0)
+
+
 def make_roofline(fdk, plain, results, probes, counters, wl):
     """Roofline of the dominant kernel, the plain path's backprojector.  The
     LDS-staged kernel (variant 3) reads its samples from LDS and is bound by VALU
-    issue: one wavefront instruction per CU per clock (transcendentals issue
-    beside them), so at V non-transcendental instructions per voxel-update its
-    roof is N_CU * clock * 64 / V voxel-updates/s.  V is measured in this run
-    (rocprofv3 SQ_INSTS_VALU - SQ_INSTS_VALU_TRANS_F32 over the launch's
-    wavefront-updates).  Beside it: the compulsory HBM bytes (filtered sinogram
-    once, volume once) against the 8 TB/s peak, and the algorithmic sample
-    bytes (16 B per voxel-update) against the LDS read bandwidth the probe
-    measured."""
+    issue: one slot per SIMD per quad-cycle, i.e. one slot per CU per clock,
+    each slot issuing one wavefront instruction or a dual-issued pair.  At S
+    slots per voxel-update the roof is N_CU * clock * 64 / S voxel-updates/s,
+    with S measured in this run (rocprofv3 SQ_INSTS_VALU - SQ_INSTS_VALU_TRANS_F32
+    - SQ_ACTIVE_INST_VALU2 over the launch's wavefront-updates), so the fraction
+    cannot pass 1 at the peak clock.  Beside it: the VALU instructions per
+    update, the compulsory HBM bytes (filtered sinogram once, volume once)
+    against the 8 TB/s peak, and the algorithmic sample bytes (16 B per
+    voxel-update) against the LDS read bandwidth the probe measured."""
     p = fdk.params
     updates = fdk.updates
     nvox = p.XN * p.YN * p.ZN
@@ -502,24 +515,25 @@ def make_roofline(fdk, plain, results, probes, counters, wl):
     if k and "fetch_bytes" in k and "write_bytes" in k:
         traffic = round(k["fetch_bytes"] + k["write_bytes"])
     roof = {"kernel": kname, "kernel_ms": round(bp_ms, 3)}
-    vpu = None
-    if k and "SQ_INSTS_VALU" in k:
+    spu = vpu = None
+    if k and "SQ_INSTS_VALU" in k and "SQ_ACTIVE_INST_VALU2" in k:
         vpu = (k["SQ_INSTS_VALU"] - k.get("SQ_INSTS_VALU_TRANS_F32", 0.0)) / (updates / 64.0)
-    if fdk.variant == 3 and vpu:
-        peak = N_CU * CLOCK_HZ * 64 / vpu / 1e9
+        spu = _issue_slots(k) / (updates / 64.0)
+    if fdk.variant == 3 and spu:
+        peak = N_CU * CLOCK_HZ * 64 / spu / 1e9
         roof.update({"bound": "valu_issue", "achieved": round(gvu, 1), "peak": round(peak, 1),
                      "unit": "GVoxel-updates/s", "frac": round(gvu / peak, 4),
-                     "traffic": traffic, "valu_per_update": round(vpu, 3),
-                     "valu_source": "rocprofv3 --pmc SQ_INSTS_VALU - SQ_INSTS_VALU_TRANS_F32, "
-                                    "this run",
-                     "peak_basis": f"{N_CU} CUs x {CLOCK_HZ / 1e9} GHz x 64 lanes / "
-                                   f"{vpu:.3f} VALU per voxel-update"})
+                     "traffic": traffic, "issue_slots_per_update": round(spu, 3),
+                     "valu_per_update": round(vpu, 3),
+                     "valu_source": "rocprofv3 --pmc SQ_INSTS_VALU - SQ_INSTS_VALU_TRANS_F32 - "
+                                    "SQ_ACTIVE_INST_VALU2, this run",
+                     "peak_basis": f"{N_CU} CUs x {CLOCK_HZ / 1e9} GHz x 1 issue slot x 64 "
+                                   f"lanes / {spu:.3f} slots per voxel-update"})
         if k.get("GRBM_GUI_ACTIVE"):
             clk = k["GRBM_GUI_ACTIVE"] / 8.0 / (bp_ms * 1e-3)
-            busy = (k["SQ_INSTS_VALU"] - k.get("SQ_INSTS_VALU_TRANS_F32", 0.0)) / N_CU / \
-                (k["GRBM_GUI_ACTIVE"] / 8.0)
+            busy = _issue_slots(k) / N_CU / (k["GRBM_GUI_ACTIVE"] / 8.0)
             roof["measured_clock_GHz"] = round(clk / 1e9, 3)
-            roof["valu_busy_at_measured_clock"] = round(busy, 4)
+            roof["issue_busy_at_measured_clock"] = round(busy, 4)
     else:
         # no counters: the HBM roof of the compulsory bytes is what can be stated
         roof.update({"bound": "hbm", "achieved": round(compulsory / (bp_ms * 1e-3) / 1e9, 1),
@@ -562,10 +576,12 @@ def make_roofline(fdk, plain, results, probes, counters, wl):
         ck = _match_kernel(counters, cn)
         c = {"kernel": cn, "kernel_ms": round(cms, 3),
              "achieved": round(cf.updates / (cms * 1e-3) / 1e9, 1)}
-        if ck and "SQ_INSTS_VALU" in ck:
+        if ck and "SQ_INSTS_VALU" in ck and "SQ_ACTIVE_INST_VALU2" in ck:
             cv = (ck["SQ_INSTS_VALU"] - ck.get("SQ_INSTS_VALU_TRANS_F32", 0.0)) / (cf.updates / 64)
-            cpk = N_CU * CLOCK_HZ * 64 / cv / 1e9
-            c.update({"valu_per_update": round(cv, 3), "peak": round(cpk, 1),
+            cs = _issue_slots(ck) / (cf.updates / 64)
+            cpk = N_CU * CLOCK_HZ * 64 / cs / 1e9
+            c.update({"bound": "valu_issue", "issue_slots_per_update": round(cs, 3),
+                      "valu_per_update": round(cv, 3), "peak": round(cpk, 1),
                       "frac": round(c["achieved"] / cpk, 4)})
         if ck and "fetch_bytes" in ck and "write_bytes" in ck:
             c["traffic"] = round(ck["fetch_bytes"] + ck["write_bytes"])
