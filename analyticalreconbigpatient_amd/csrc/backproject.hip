@@ -43,6 +43,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <mutex>
 #include <type_traits>
@@ -75,9 +76,14 @@ struct BPArgs {
     // LDS-staged kernel: the plan's footprint table (per slice block and tile, one
     // entry per view of the block, padded to a multiple of four), its per-block
     // offsets and per-(block, tile) flags (1: every footprint fits the slots)
-    const unsigned* zs_tab;
+    const uint2* zs_tab;
     const long long* zs_boff;
     const int* zs_fits;
+    // LDS-staged kernel: one record per (block, view), at the view's mask index:
+    // the slice masks, the view's cos / sin and the block's ZB (temp2, a) entries
+    // (4 + 2 ZB words), so a view's scalar operands come in one load from an
+    // offset that is a constant within a chunk
+    const unsigned* zs_rec;
 };
 
 // Logical workgroup lb -> slice block zs and 16x16-column tile (txb, tyb).
@@ -639,25 +645,28 @@ __device__ __forceinline__ void acc_weighted_p(float& acc, float2 c, float Dey, 
         : "scc");
 }
 
-// 3D-weighted accumulation of a group of four slices (samples t0..t3) with one
-// uniform branch per group instead of one per slice: `wt` holds the group's
-// weighted bits, `act` its active bits.  No weighted slice: four adds.  Every
-// active slice weighted: acc_i = fma(rcp(1 + a_i B), t_i, acc_i) for all four (an
-// inactive slice has t_i = 0 and a_i = 0, so it adds exactly zero).  Mixed:
-// slice by slice as acc_weighted.  The transcendental results are read three or
-// more instructions after they are written, so no wait states are needed there.
+// 3D-weighted accumulation of slices G..G+3 (a group of four; samples t0..t3)
+// with one uniform branch per group instead of one per slice.  `mw` is the view
+// record's weighted word: bit j = slice j weighted, bit 16 + G/4 = some slice of
+// the group weighted, bit 20 + G/4 = every active slice of the group weighted.
+// No weighted slice: four adds.  Every active slice weighted: acc_i =
+// fma(rcp(1 + a_i B), t_i, acc_i) for all four (an inactive slice has t_i = 0
+// and a_i = 0, so it adds exactly zero).  Mixed: slice by slice as
+// acc_weighted.  The transcendental results are read three or more
+// instructions after they are written, so no wait states are needed there.
+template <int G>
 __device__ __forceinline__ void acc_weighted_4(float& acc0, float& acc1, float& acc2,
                                                float& acc3, float t0, float t1, float t2,
-                                               float t3, unsigned wt, unsigned act, float a0,
-                                               float a1, float a2, float a3, float B) {
+                                               float t3, unsigned mw, float a0, float a1,
+                                               float a2, float a3, float B) {
     float w0, w1, w2, w3;
     asm volatile(
-        "s_cmp_eq_u32 %[wt], 0\n\t"
-        "s_cbranch_scc1 13f\n\t"
-        "s_cmp_eq_u32 %[wt], %[act]\n\t"
+        "s_bitcmp1_b32 %[mw], %[bany]\n\t"
+        "s_cbranch_scc0 13f\n\t"
+        "s_bitcmp1_b32 %[mw], %[ball]\n\t"
         "s_cbranch_scc1 12f\n\t"
         // mixed
-        "s_bitcmp1_b32 %[wt], 0\n\t"
+        "s_bitcmp1_b32 %[mw], %[b0]\n\t"
         "s_cbranch_scc0 20f\n\t"
         "v_fma_f32 %[w0], %[a0], %[B], 1.0\n\t"
         "v_rcp_f32 %[w0], %[w0]\n\t"
@@ -667,7 +676,7 @@ __device__ __forceinline__ void acc_weighted_4(float& acc0, float& acc1, float& 
         "20:\n\t"
         "v_add_f32 %[c0], %[c0], %[t0]\n"
         "21:\n\t"
-        "s_bitcmp1_b32 %[wt], 1\n\t"
+        "s_bitcmp1_b32 %[mw], %[b1]\n\t"
         "s_cbranch_scc0 22f\n\t"
         "v_fma_f32 %[w1], %[a1], %[B], 1.0\n\t"
         "v_rcp_f32 %[w1], %[w1]\n\t"
@@ -677,7 +686,7 @@ __device__ __forceinline__ void acc_weighted_4(float& acc0, float& acc1, float& 
         "22:\n\t"
         "v_add_f32 %[c1], %[c1], %[t1]\n"
         "23:\n\t"
-        "s_bitcmp1_b32 %[wt], 2\n\t"
+        "s_bitcmp1_b32 %[mw], %[b2]\n\t"
         "s_cbranch_scc0 24f\n\t"
         "v_fma_f32 %[w2], %[a2], %[B], 1.0\n\t"
         "v_rcp_f32 %[w2], %[w2]\n\t"
@@ -687,7 +696,7 @@ __device__ __forceinline__ void acc_weighted_4(float& acc0, float& acc1, float& 
         "24:\n\t"
         "v_add_f32 %[c2], %[c2], %[t2]\n"
         "25:\n\t"
-        "s_bitcmp1_b32 %[wt], 3\n\t"
+        "s_bitcmp1_b32 %[mw], %[b3]\n\t"
         "s_cbranch_scc0 26f\n\t"
         "v_fma_f32 %[w3], %[a3], %[B], 1.0\n\t"
         "v_rcp_f32 %[w3], %[w3]\n\t"
@@ -721,8 +730,10 @@ __device__ __forceinline__ void acc_weighted_4(float& acc0, float& acc1, float& 
         "14:"
         : [c0] "+v"(acc0), [c1] "+v"(acc1), [c2] "+v"(acc2), [c3] "+v"(acc3),
           [w0] "=&v"(w0), [w1] "=&v"(w1), [w2] "=&v"(w2), [w3] "=&v"(w3)
-        : [t0] "v"(t0), [t1] "v"(t1), [t2] "v"(t2), [t3] "v"(t3), [wt] "s"(wt), [act] "s"(act),
-          [a0] "s"(a0), [a1] "s"(a1), [a2] "s"(a2), [a3] "s"(a3), [B] "v"(B)
+        : [t0] "v"(t0), [t1] "v"(t1), [t2] "v"(t2), [t3] "v"(t3), [mw] "s"(mw),
+          [bany] "i"(16 + G / 4), [ball] "i"(20 + G / 4), [b0] "i"(G), [b1] "i"(G + 1),
+          [b2] "i"(G + 2), [b3] "i"(G + 3), [a0] "s"(a0), [a1] "s"(a1), [a2] "s"(a2),
+          [a3] "s"(a3), [B] "v"(B)
         : "scc");
 }
 
@@ -732,19 +743,20 @@ __device__ __forceinline__ void acc_weighted_4(float& acc0, float& acc1, float& 
 // acc_i = fma(rcp(1 + a_i B), t_i, acc_i) (an inactive slice has u_i = v_i = 0
 // and a_i = 0 and adds exactly zero); mixed, slice by slice.  The u_i are
 // clobbered (they hold t_i).
+template <int G>
 __device__ __forceinline__ void acc_weighted_4p(float& acc0, float& acc1, float& acc2,
                                                 float& acc3, float2 c0, float2 c1, float2 c2,
-                                                float2 c3, float Dey, float wu, unsigned wt,
-                                                unsigned act, float a0, float a1, float a2,
-                                                float a3, float B) {
+                                                float2 c3, float Dey, float wu, unsigned mw,
+                                                float a0, float a1, float a2, float a3,
+                                                float B) {
     float w0, w1, w2, w3, u0 = c0.x, u1 = c1.x, u2 = c2.x, u3 = c3.x;
     asm volatile(
-        "s_cmp_eq_u32 %[wt], 0\n\t"
-        "s_cbranch_scc1 13f\n\t"
-        "s_cmp_eq_u32 %[wt], %[act]\n\t"
+        "s_bitcmp1_b32 %[mw], %[bany]\n\t"
+        "s_cbranch_scc0 13f\n\t"
+        "s_bitcmp1_b32 %[mw], %[ball]\n\t"
         "s_cbranch_scc1 12f\n\t"
         // mixed
-        "s_bitcmp1_b32 %[wt], 0\n\t"
+        "s_bitcmp1_b32 %[mw], %[b0]\n\t"
         "s_cbranch_scc0 20f\n\t"
         "v_sub_f32 %[u0], %[u0], %[v0]\n\t"
         "v_fma_f32 %[w0], %[a0], %[B], 1.0\n\t"
@@ -757,7 +769,7 @@ __device__ __forceinline__ void acc_weighted_4p(float& acc0, float& acc1, float&
         "v_fmac_f32 %[c0], %[Dey], %[u0]\n\t"
         "v_fmac_f32 %[c0], %[wu], %[v0]\n"
         "21:\n\t"
-        "s_bitcmp1_b32 %[wt], 1\n\t"
+        "s_bitcmp1_b32 %[mw], %[b1]\n\t"
         "s_cbranch_scc0 22f\n\t"
         "v_sub_f32 %[u1], %[u1], %[v1]\n\t"
         "v_fma_f32 %[w1], %[a1], %[B], 1.0\n\t"
@@ -770,7 +782,7 @@ __device__ __forceinline__ void acc_weighted_4p(float& acc0, float& acc1, float&
         "v_fmac_f32 %[c1], %[Dey], %[u1]\n\t"
         "v_fmac_f32 %[c1], %[wu], %[v1]\n"
         "23:\n\t"
-        "s_bitcmp1_b32 %[wt], 2\n\t"
+        "s_bitcmp1_b32 %[mw], %[b2]\n\t"
         "s_cbranch_scc0 24f\n\t"
         "v_sub_f32 %[u2], %[u2], %[v2]\n\t"
         "v_fma_f32 %[w2], %[a2], %[B], 1.0\n\t"
@@ -783,7 +795,7 @@ __device__ __forceinline__ void acc_weighted_4p(float& acc0, float& acc1, float&
         "v_fmac_f32 %[c2], %[Dey], %[u2]\n\t"
         "v_fmac_f32 %[c2], %[wu], %[v2]\n"
         "25:\n\t"
-        "s_bitcmp1_b32 %[wt], 3\n\t"
+        "s_bitcmp1_b32 %[mw], %[b3]\n\t"
         "s_cbranch_scc0 26f\n\t"
         "v_sub_f32 %[u3], %[u3], %[v3]\n\t"
         "v_fma_f32 %[w3], %[a3], %[B], 1.0\n\t"
@@ -834,8 +846,9 @@ __device__ __forceinline__ void acc_weighted_4p(float& acc0, float& acc1, float&
           [w0] "=&v"(w0), [w1] "=&v"(w1), [w2] "=&v"(w2), [w3] "=&v"(w3),
           [u0] "+v"(u0), [u1] "+v"(u1), [u2] "+v"(u2), [u3] "+v"(u3)
         : [v0] "v"(c0.y), [v1] "v"(c1.y), [v2] "v"(c2.y), [v3] "v"(c3.y), [Dey] "v"(Dey),
-          [wu] "v"(wu), [wt] "s"(wt), [act] "s"(act), [a0] "s"(a0), [a1] "s"(a1),
-          [a2] "s"(a2), [a3] "s"(a3), [B] "v"(B)
+          [wu] "v"(wu), [mw] "s"(mw), [bany] "i"(16 + G / 4), [ball] "i"(20 + G / 4),
+          [b0] "i"(G), [b1] "i"(G + 1), [b2] "i"(G + 2), [b3] "i"(G + 3), [a0] "s"(a0),
+          [a1] "s"(a1), [a2] "s"(a2), [a3] "s"(a3), [B] "v"(B)
         : "scc");
 }
 
@@ -981,6 +994,15 @@ __device__ __forceinline__ long long uniform_i64(long long x) {
     return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
+// f(integral_constant<int, I>) for I = B .. E-1, each with I a constant
+// expression (asm immediates, template arguments).
+template <int B, int E, class F> __device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // Body of the z-column kernel for logical workgroup lb of one plan.
 template <int ZB, int G, class T, int LAYOUT, bool INSIDE>
 __device__ __forceinline__ void zc_run(int lb, const BPArgs& A, float* __restrict__ img,
@@ -1111,7 +1133,12 @@ __global__ __launch_bounds__(256) void bp_fdk3dw_zc(float* __restrict__ img,
 // quads; a staged row of CW records is 256 B) or 4 bytes (16-bit row
 // differences; 128 B rows, half the LDS per view, so more workgroups per CU).
 // Each wave stages one view of a chunk of four, then every wave consumes the
-// four views.
+// four views.  A view's scalar operands come from one per-(block, view) record
+// of the plan (slice masks with per-group class bits, cos / sin, the block's ZB
+// (temp2, a)), loaded whole behind one wait; its footprint entry also carries
+// the LDS row term and which groups of slices need their rows clamped.  Split
+// over the separate mask, view and slice tables, the same operands took three
+// dependent load waits and ~10 scalar address instructions per view.
 // ---------------------------------------------------------------------------
 // Records per staged row: 28 for 8-byte records (224-byte rows, so that 25-row
 // slots of four views fit seven workgroups per CU), 32 for 4-byte records.  A
@@ -1219,12 +1246,27 @@ __device__ __forceinline__ ZsView zs_footprint(const BPArgs& A, float2 cs, unsig
     return v;
 }
 
-// Footprint table entry: c0 (16 bits), rA + 1 and rB + 1 (8 bits each); a
-// view with rB < rA is skipped.
-__device__ __forceinline__ unsigned zs_pack(const ZsView& v) {
-    if (v.mode != ZS_LDS) return 1u << 16;                 // rA = 0, rB = -1: skip
-    return ((unsigned)v.c0 & 0xffffu) | ((unsigned)(v.rA + 1) << 16) |
-           ((unsigned)(v.rB + 1) << 24);
+// Footprint table entry, two words.  The first: c0 (16 bits), rA + 1 and rB + 1
+// (8 bits each); a view with rB < rA is skipped.  The second: 16 cterm + clamp
+// bits, cterm = -c0 kRec + (RMAX - 2 - rB) kRowB the LDS offset of record c0 of
+// row 0 when row rB is staged at slot row RMAX - 2, and bit k set when group k
+// of four slices needs its rows clamped.  Rows need no clamp in a group whose
+// four slices are all active when the view's row range was not cut at the
+// detector's first or last row and every lane is on the detector (INSIDE): the
+// footprint then bounds every active slice's row with a row of margin on each
+// side.
+template <int ZB, int RMAX, bool INSIDE, bool PAIRS>
+__device__ __forceinline__ uint2 zs_pack(const ZsView& v, unsigned act, int zl) {
+    if (v.mode != ZS_LDS) return make_uint2(1u << 16, 0u);   // rA = 0, rB = -1: skip
+    constexpr int kRec = PAIRS ? 8 : 4, kRowB = zs_cw(kRec) * kRec;
+    const int cterm = -v.c0 * kRec + (RMAX - 2 - v.rB) * kRowB;
+    const bool clipped = !INSIDE || v.rA < 0 || v.rB > zl - 2;
+    unsigned cl = 0u;
+    for (int k = 0; k < ZB / 4; ++k)
+        if (clipped || ((act >> (4 * k)) & 0xfu) != 0xfu) cl |= 1u << k;
+    return make_uint2(((unsigned)v.c0 & 0xffffu) | ((unsigned)(v.rA + 1) << 16) |
+                          ((unsigned)(v.rB + 1) << 24),
+                      ((unsigned)cterm << 4) | cl);
 }
 
 // Views of slice block zs in the footprint table: pmax - pmin + 1 rounded up to a
@@ -1243,7 +1285,7 @@ __global__ __launch_bounds__(256) void bp_zs_table(BPArgs A, const float2* __res
                                                    const ZBlock* __restrict__ blocks,
                                                    const uint2* __restrict__ masks,
                                                    const float2* __restrict__ table,
-                                                   unsigned* __restrict__ tab,
+                                                   uint2* __restrict__ tab,
                                                    const long long* __restrict__ boff,
                                                    int* __restrict__ fits, int nrec) {
     const int b = blockIdx.x;
@@ -1257,17 +1299,17 @@ __global__ __launch_bounds__(256) void bp_zs_table(BPArgs A, const float2* __res
     const float yc = ((float)(tyb * 16) + 7.5f - A.YNC) * A.dy;
     const float ex = 7.5f * fabsf(A.dx), ey = 7.5f * fabsf(A.dy);
     const int nvp = zs_table_views(pmin, pmax);
-    unsigned* out = tab + boff[zs] + (long long)tile * nvp;
+    uint2* out = tab + boff[zs] + (long long)tile * nvp;
     int ok = 1;
     for (int i = threadIdx.x; i < nvp; i += 256) {
         const int p = pmin + i;
-        unsigned e = 1u << 16;                                 // a skipped view
+        uint2 e = make_uint2(1u << 16, 0u);                    // a skipped view
         if (p <= pmax) {
+            const unsigned act = mrow[2LL * p];
             const ZsView v = zs_footprint<RMAX, INSIDE, PAIRS>(
-                A, vtab[p], mrow[2LL * p], const_ptr(trow + 2LL * p * ZB), xc, yc, ex, ey,
-                nrec);
+                A, vtab[p], act, const_ptr(trow + 2LL * p * ZB), xc, yc, ex, ey, nrec);
             if (v.mode == ZS_DIRECT) ok = 0;
-            e = zs_pack(v);
+            e = zs_pack<ZB, RMAX, INSIDE, PAIRS>(v, act, A.ZL);
         }
         out[i] = e;
     }
@@ -1302,17 +1344,17 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     const cptr<int> hbp = (cptr<int>)const_ptr(uniform_ptr(blocks) + zs);
     const int pmin = hbp[0], pmax = hbp[1];
     const long long hoff = *(cptr<long long>)(hbp + 2);
-    const cptr<unsigned> mrow =
-        (cptr<unsigned>)const_ptr(uniform_ptr(masks) + (hoff / ZB - pmin));
-    const cptr<float> trow =
-        (cptr<float>)const_ptr(uniform_ptr(table) + (hoff - (long long)pmin * ZB));
-    const cptr<float> vt = (cptr<float>)const_ptr(uniform_ptr(vtab));
+    // the block's view records (masks, cos / sin, ZB entries), from view pmin
+    constexpr int kRecW = 4 + 2 * ZB;
+    const cptr<unsigned> vrec =
+        const_ptr(uniform_ptr(A.zs_rec) + (hoff / ZB) * kRecW);
 
     // Every view's footprint comes from the plan's table (bp_zs_table, built once
     // per plan and layout; kept in LDS it cost the sixth or seventh workgroup per
     // CU, and recomputed here about two VALU per update).  A workgroup with a
     // view that does not fit runs the z-column kernel's body instead.
     const int tile = tyb * A.tilesx + txb;
+    // the tile's footprint entries (two words per view), from view pmin
     const cptr<unsigned> ftab = (cptr<unsigned>)const_ptr(
         uniform_ptr(A.zs_tab) + *(cptr<long long>)const_ptr(uniform_ptr(A.zs_boff) + zs) +
         (long long)tile * zs_table_views(pmin, pmax));
@@ -1364,7 +1406,7 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
     auto stage_chunk = [&](int p0) {
         const int p = p0 + wave;
         if (p <= pmax) {
-            const unsigned e = ftab[p - pmin];
+            const unsigned e = ftab[2 * (p - pmin)];
             const int rA = (int)((e >> 16) & 0xffu) - 1, rB = (int)(e >> 24) - 1;
             const int c0 = (int)(short)(e & 0xffffu);
             if (rB >= rA) {
@@ -1400,28 +1442,42 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
         }
     };
     // consume the chunk of views p0 .. p0+3; their four table entries in one
-    // scalar load
+    // scalar load, each view's record at a constant offset from the chunk's
+    typedef unsigned u32x4_v __attribute__((ext_vector_type(4)));
+    typedef unsigned u32x16_v __attribute__((ext_vector_type(16)));
+    typedef unsigned u32x8_v __attribute__((ext_vector_type(8)));
     auto consume_chunk = [&](int p0) {
-        typedef unsigned u32x4_v __attribute__((ext_vector_type(4)));
-        const u32x4_v e4 = *(cptr<u32x4_v>)(ftab + (p0 - pmin));
-        const unsigned ech[4] = {e4.x, e4.y, e4.z, e4.w};
+        const u32x8_v e8 = *(cptr<u32x8_v>)(ftab + 2 * (p0 - pmin));
+        const cptr<unsigned> rc = vrec + (p0 - pmin) * kRecW;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int p = p0 + s;
             if (p > pmax) break;
-            const unsigned e = ech[s];
+            // the view's whole record in flight at once, and one wait for it
+            // (and the chunk's table entries): left to the compiler, each part
+            // is loaded where it is first used, behind a wait of its own
+            const cptr<unsigned> r = rc + s * kRecW;
+            const u32x4_v hd = *(cptr<u32x4_v>)r;       // active, weighted, cos, sin
+            u32x16_v er[ZB / 8];                        // (temp2, a) of the ZB slices
+#pragma unroll
+            for (int k = 0; k < ZB / 8; ++k) er[k] = *(cptr<u32x16_v>)(r + 4 + 16 * k);
+            asm volatile("" ::"s"(hd));
+#pragma unroll
+            for (int k = 0; k < ZB / 8; ++k) asm volatile("" ::"s"(er[k]));
+            // table entry: (c0, rA + 1, rB + 1), and e1 = cterm * 16 + clamp bits
+            const unsigned e = e8[2 * s], e1 = e8[2 * s + 1];
             const int rA = (int)((e >> 16) & 0xffu) - 1, rB = (int)(e >> 24) - 1;
-            const int c0 = (int)(short)(e & 0xffffu);
             if (rB < rA) continue;                      // every sample would be zero
-            const uint2 m = cld_u2(mrow, p);               // scalar: active, weighted
-            const float2 cs = cld_f2(vt, p);
+            const uint2 m = make_uint2(hd.x, hd.y);
+            const float2 cs = make_float2(__uint_as_float(hd.z), __uint_as_float(hd.w));
             const ColTerms ct = column_terms<INSIDE>(x, y, cs, A, m.y, ylc);
             const float Dey = ct.Dey, wu = ct.wu, K = ct.K, B = ct.B;
             const bool uok = ct.uok;
-            const cptr<float> erow = trow + 2LL * p * ZB;
             float2 ent[ZB];
 #pragma unroll
-            for (int j = 0; j < ZB; ++j) ent[j] = cld_f2(erow, j);
+            for (int j = 0; j < ZB; ++j)
+                ent[j] = make_float2(__uint_as_float(er[j / 8][2 * (j % 8)]),
+                                     __uint_as_float(er[j / 8][2 * (j % 8) + 1]));
             // byte offset in the slot = row * kRowB + colterm, row clamped to
             // [rA, rBv]; a lane off the detector gets row -1e9 (clamped to rA) and
             // the slot's zero row
@@ -1430,7 +1486,7 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             // rest of the address is one per-view VGPR, kept whole (opaque to the
             // compiler, which otherwise splits its scalar part off and adds it
             // back at every update), so an update's address is one v_lshl_add_u32
-            const int cterm = -c0 * kRec + (RMAX - 2 - rB) * kRowB;   // row rB at RMAX - 2
+            const int cterm = (int)e1 >> 4;   // -c0 kRec + (RMAX - 2 - rB) kRowB: row rB at RMAX - 2
             int colterm;
             if constexpr (INSIDE) {
                 asm("v_lshl_add_u32 %0, %1, %2, %3"
@@ -1443,79 +1499,65 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
             }
             int rBv;   // clamp bound: row rB + 1, the slot's last row, kept zero
             asm("v_mov_b32 %0, %1" : "=v"(rBv) : "s"(rB + 1));
-            // Rows need no clamp in a group whose four slices are all active when
-            // the view's row range was not cut at the detector's first or last
-            // row and every lane is on the detector (INSIDE): the footprint then
-            // bounds every active slice's row with a row of margin on each side.
-            // clipped = 1 when the range was cut or the launch is not INSIDE
-            // (integer arithmetic keeps the flag scalar).
-            const unsigned clipped =
-                INSIDE ? ((unsigned)rA | (unsigned)(A.ZL - 2 - rB)) >> 31 : 1u;
             // groups of four slices, each skipped when none of its slices has the
-            // view in its window, and branch-free inside: a slice outside the
-            // window has temp2 = 1e30, so its row clamps onto the zero row (and a
-            // lane off the detector reads the zero record): it adds exactly zero
-#pragma unroll
-            for (int g = 0; g < ZB; g += 4) {
-                const unsigned act = (m.x >> g) & 0xfu, wt = (m.y >> g) & 0xfu;
-                if (act == 0u) continue;
-                const unsigned clamp = (act ^ 0xfu) | clipped;
+            // view in its window (bit 16 + g/4 of the record's active word), and
+            // branch-free inside: a slice outside the window has temp2 = 1e30, so
+            // its row clamps onto the zero row (and a lane off the detector reads
+            // the zero record): it adds exactly zero
+            static_for<0, ZB / 4>([&](auto gi) {
+                constexpr int g = 4 * decltype(gi)::value;
+                if (!((m.x >> (16 + g / 4)) & 1u)) return;
                 float V1[4], fr[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     V1[i] = fmaf(ent[g + i].x, K, zl);
                     fr[i] = __builtin_amdgcn_fractf(V1[i]);
                 }
-                // rows (floor, clamped to [rA, rBv] unless the group needs no
-                // clamp) and LDS offsets of the four slices in one block: as
+                // rows (floor, clamped to [rA, rBv] unless bit g/4 of the table
+                // entry says the group needs no clamp; always clamped off the
+                // INSIDE path) and LDS offsets of the four slices in one block: as
                 // separate blocks the clamp's two paths would give every
-                // accumulator a copy at the join
+                // accumulator a copy at the join.  row * kRowB + colterm: a
+                // shift-add for power-of-two rows, else a signed 24-bit
+                // multiply-add (the row may be -1)
                 int a[4];
-                // row * kRowB + colterm: a shift-add for power-of-two rows, else a
-                // signed 24-bit multiply-add (the row may be -1)
+#define ZS_ROWS_                                                                       \
+    "v_cvt_flr_i32_f32 %[a0], %[v0]\n\t"                                                \
+    "v_cvt_flr_i32_f32 %[a1], %[v1]\n\t"                                                \
+    "v_cvt_flr_i32_f32 %[a2], %[v2]\n\t"                                                \
+    "v_cvt_flr_i32_f32 %[a3], %[v3]\n\t"
+#define ZS_CLAMP_                                                                      \
+    "v_med3_i32 %[a0], %[a0], %[rA], %[rB]\n\t"                                         \
+    "v_med3_i32 %[a1], %[a1], %[rA], %[rB]\n\t"                                         \
+    "v_med3_i32 %[a2], %[a2], %[rA], %[rB]\n\t"                                         \
+    "v_med3_i32 %[a3], %[a3], %[rA], %[rB]\n"
+#define ZS_SKIP_ "s_bitcmp1_b32 %[cl], %[cb]\n\ts_cbranch_scc0 1f\n\t"
+#define ZS_ADDR_(OP)                                                                   \
+    "1:\n\t" OP " %[a0], %[a0], %[rw], %[ct]\n\t" OP " %[a1], %[a1], %[rw], %[ct]\n\t" OP  \
+    " %[a2], %[a2], %[rw], %[ct]\n\t" OP " %[a3], %[a3], %[rw], %[ct]"
+#define ZS_OPS_(RW)                                                                    \
+    : [a0] "=&v"(a[0]), [a1] "=&v"(a[1]), [a2] "=&v"(a[2]), [a3] "=&v"(a[3])             \
+    : [v0] "v"(V1[0]), [v1] "v"(V1[1]), [v2] "v"(V1[2]), [v3] "v"(V1[3]), [cl] "s"(e1), \
+      [cb] "i"(g / 4), [rA] "s"(rA), [rB] "v"(rBv), [ct] "v"(colterm), [rw] RW          \
+    : "scc"
                 if constexpr ((kRowB & (kRowB - 1)) == 0) {
-                    asm("v_cvt_flr_i32_f32 %[a0], %[v0]\n\t"
-                        "v_cvt_flr_i32_f32 %[a1], %[v1]\n\t"
-                        "v_cvt_flr_i32_f32 %[a2], %[v2]\n\t"
-                        "v_cvt_flr_i32_f32 %[a3], %[v3]\n\t"
-                        "s_cmp_eq_u32 %[cl], 0\n\t"
-                        "s_cbranch_scc1 1f\n\t"
-                        "v_med3_i32 %[a0], %[a0], %[rA], %[rB]\n\t"
-                        "v_med3_i32 %[a1], %[a1], %[rA], %[rB]\n\t"
-                        "v_med3_i32 %[a2], %[a2], %[rA], %[rB]\n\t"
-                        "v_med3_i32 %[a3], %[a3], %[rA], %[rB]\n"
-                        "1:\n\t"
-                        "v_lshl_add_u32 %[a0], %[a0], %[rw], %[ct]\n\t"
-                        "v_lshl_add_u32 %[a1], %[a1], %[rw], %[ct]\n\t"
-                        "v_lshl_add_u32 %[a2], %[a2], %[rw], %[ct]\n\t"
-                        "v_lshl_add_u32 %[a3], %[a3], %[rw], %[ct]"
-                        : [a0] "=&v"(a[0]), [a1] "=&v"(a[1]), [a2] "=&v"(a[2]), [a3] "=&v"(a[3])
-                        : [v0] "v"(V1[0]), [v1] "v"(V1[1]), [v2] "v"(V1[2]), [v3] "v"(V1[3]),
-                          [cl] "s"(clamp), [rA] "s"(rA), [rB] "v"(rBv), [ct] "v"(colterm),
-                          [rw] "i"(kRowShift)
-                        : "scc");
+                    if constexpr (INSIDE)
+                        asm(ZS_ROWS_ ZS_SKIP_ ZS_CLAMP_ ZS_ADDR_("v_lshl_add_u32")
+                            ZS_OPS_("i"(kRowShift)));
+                    else
+                        asm(ZS_ROWS_ ZS_CLAMP_ ZS_ADDR_("v_lshl_add_u32") ZS_OPS_("i"(kRowShift)));
                 } else {
-                    asm("v_cvt_flr_i32_f32 %[a0], %[v0]\n\t"
-                        "v_cvt_flr_i32_f32 %[a1], %[v1]\n\t"
-                        "v_cvt_flr_i32_f32 %[a2], %[v2]\n\t"
-                        "v_cvt_flr_i32_f32 %[a3], %[v3]\n\t"
-                        "s_cmp_eq_u32 %[cl], 0\n\t"
-                        "s_cbranch_scc1 1f\n\t"
-                        "v_med3_i32 %[a0], %[a0], %[rA], %[rB]\n\t"
-                        "v_med3_i32 %[a1], %[a1], %[rA], %[rB]\n\t"
-                        "v_med3_i32 %[a2], %[a2], %[rA], %[rB]\n\t"
-                        "v_med3_i32 %[a3], %[a3], %[rA], %[rB]\n"
-                        "1:\n\t"
-                        "v_mad_i32_i24 %[a0], %[a0], %[rw], %[ct]\n\t"
-                        "v_mad_i32_i24 %[a1], %[a1], %[rw], %[ct]\n\t"
-                        "v_mad_i32_i24 %[a2], %[a2], %[rw], %[ct]\n\t"
-                        "v_mad_i32_i24 %[a3], %[a3], %[rw], %[ct]"
-                        : [a0] "=&v"(a[0]), [a1] "=&v"(a[1]), [a2] "=&v"(a[2]), [a3] "=&v"(a[3])
-                        : [v0] "v"(V1[0]), [v1] "v"(V1[1]), [v2] "v"(V1[2]), [v3] "v"(V1[3]),
-                          [cl] "s"(clamp), [rA] "s"(rA), [rB] "v"(rBv), [ct] "v"(colterm),
-                          [rw] "s"(kRowB)
-                        : "scc");
+                    if constexpr (INSIDE)
+                        asm(ZS_ROWS_ ZS_SKIP_ ZS_CLAMP_ ZS_ADDR_("v_mad_i32_i24")
+                            ZS_OPS_("s"(kRowB)));
+                    else
+                        asm(ZS_ROWS_ ZS_CLAMP_ ZS_ADDR_("v_mad_i32_i24") ZS_OPS_("s"(kRowB)));
                 }
+#undef ZS_ROWS_
+#undef ZS_CLAMP_
+#undef ZS_SKIP_
+#undef ZS_ADDR_
+#undef ZS_OPS_
                 typename Gt::Raw q[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) q[i] = Lg::load(lstage + s * kSlot + a[i]);
@@ -1525,18 +1567,18 @@ __device__ __forceinline__ void zs_run(int lb, const BPArgs& A, float* __restric
                     float2 c[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) c[i] = Gt::chan(q[i], fr[i]);
-                    acc_weighted_4p(acc[g], acc[g + 1], acc[g + 2], acc[g + 3], c[0], c[1], c[2],
-                                    c[3], Dey, wu, wt, act, ent[g].y, ent[g + 1].y, ent[g + 2].y,
-                                    ent[g + 3].y, B);
+                    acc_weighted_4p<g>(acc[g], acc[g + 1], acc[g + 2], acc[g + 3], c[0], c[1],
+                                       c[2], c[3], Dey, wu, m.y, ent[g].y, ent[g + 1].y,
+                                       ent[g + 2].y, ent[g + 3].y, B);
                 } else {
                     float t[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) t[i] = Gt::lerp(q[i], fr[i], Dey, wu);
-                    acc_weighted_4(acc[g], acc[g + 1], acc[g + 2], acc[g + 3], t[0], t[1], t[2],
-                                   t[3], wt, act, ent[g].y, ent[g + 1].y, ent[g + 2].y,
-                                   ent[g + 3].y, B);
+                    acc_weighted_4<g>(acc[g], acc[g + 1], acc[g + 2], acc[g + 3], t[0], t[1],
+                                      t[2], t[3], m.y, ent[g].y, ent[g + 1].y, ent[g + 2].y,
+                                      ent[g + 3].y, B);
                 }
-            }
+            });
         }
     };
     for (int p0 = pmin; p0 <= pmax; p0 += 4) {
@@ -1725,6 +1767,7 @@ struct BPPlan {
     float2* vt_table = nullptr; // per (slice set or block, view, slice): temp2, a
     ZBlock* zblocks = nullptr;  // z-column kernel: per block view range and offset
     uint2* zmasks = nullptr;    // z-column kernel: per (block, view) slice masks
+    unsigned* zrec = nullptr;   // LDS-staged kernel: per (block, view) records
     BPZ* ztab = nullptr;
     unsigned long long* stats = nullptr;   // LDS-staged kernel: workgroups staged / direct
     BPArgs a{};
@@ -1737,7 +1780,7 @@ struct BPPlan {
     // record kind (8-byte records over YL channels, quads over YL-1, 4-byte
     // records over YL)
     struct ZsTable {
-        unsigned* tab = nullptr;
+        uint2* tab = nullptr;
         long long* boff = nullptr;
         int* fits = nullptr;
     };
@@ -1823,7 +1866,8 @@ static void build_view_table(BPPlan* P, const double* w, const std::vector<BPZ>&
 // Per-(block, view) masks and per-(block, view, slice) entries of the z-column
 // kernel.  Bit j of masks.x: slice j's window contains the view; bit j of
 // masks.y: that slice's weight has a conjugate term (a != 0).
-static void build_block_table(BPPlan* P, const double* w, const std::vector<BPZ>& zt) {
+static void build_block_table(BPPlan* P, const double* w, const std::vector<BPZ>& zt,
+                              const std::vector<float2>& vt) {
     const arecon_bp_geom& g = P->g;
     const int ZB = P->nz, nblk = P->a.nzsets;
     std::vector<ZBlock> blks(nblk);
@@ -1866,6 +1910,36 @@ static void build_block_table(BPPlan* P, const double* w, const std::vector<BPZ>
     ARECON_HIP(hipMemcpy(P->zblocks, blks.data(), sizeof(ZBlock) * blks.size(), hipMemcpyHostToDevice));
     ARECON_HIP(hipMemcpy(P->zmasks, msk.data(), sizeof(uint2) * msk.size(), hipMemcpyHostToDevice));
     ARECON_HIP(hipMemcpy(P->vt_table, tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice));
+    if (P->variant != 3) return;
+    // records of the LDS-staged kernel: record i (mask index i, view p of block
+    // zb at blks[zb].off / ZB + p - pmin) = masks, cos / sin of view p, the
+    // block's ZB entries
+    const int RW = 4 + 2 * ZB;
+    std::vector<unsigned> rec(msk.size() * RW, 0u);
+    for (int zb = 0; zb < nblk; ++zb) {
+        const ZBlock& h = blks[zb];
+        for (int p = h.pmin; p <= h.pmax; ++p) {
+            const long long i = h.off / ZB + (p - h.pmin);
+            unsigned* r = rec.data() + i * RW;
+            // bits 16 + k (active word: some slice of group k of four active;
+            // weighted word: some slice weighted) and 20 + k (weighted word: every
+            // active slice of the group weighted)
+            r[0] = msk[i].x;
+            r[1] = msk[i].y;
+            for (int k = 0; k < ZB / 4; ++k) {
+                const unsigned act = (msk[i].x >> (4 * k)) & 0xfu;
+                const unsigned wt = (msk[i].y >> (4 * k)) & 0xfu;
+                if (act) r[0] |= 1u << (16 + k);
+                if (wt) r[1] |= 1u << (16 + k);
+                if (wt && wt == act) r[1] |= 1u << (20 + k);
+            }
+            std::memcpy(r + 2, &vt[p], sizeof(float2));
+            std::memcpy(r + 4, &tab[i * ZB], sizeof(float2) * ZB);
+        }
+    }
+    ARECON_HIP(hipMalloc(&P->zrec, sizeof(unsigned) * rec.size()));
+    ARECON_HIP(hipMemcpy(P->zrec, rec.data(), sizeof(unsigned) * rec.size(), hipMemcpyHostToDevice));
+    P->a.zs_rec = P->zrec;
 }
 
 BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
@@ -1989,7 +2063,7 @@ BPPlan* bp_plan_create(const arecon_bp_geom& g, const double* w) {
         ARECON_HIP(hipMalloc(&P->ktab, sizeof(float4) * kt.size()));
         ARECON_HIP(hipMalloc(&P->ztab, sizeof(BPZ) * zt.size()));
         if (P->variant == 0) build_view_table(P, w, zt);
-        if (P->variant >= 2) build_block_table(P, w, zt);
+        if (P->variant >= 2) build_block_table(P, w, zt, vt);
         if (P->variant == 3) {
             ARECON_REQUIRE(g.ZL <= 250, "bp: variant 3 takes at most 250 detector rows");
             // the footprint table stores the first channel in 16 signed bits
@@ -2048,6 +2122,7 @@ void bp_plan_destroy(BPPlan* P) {
     if (P->vt_table) (void)hipFree(P->vt_table);
     if (P->zblocks) (void)hipFree(P->zblocks);
     if (P->zmasks) (void)hipFree(P->zmasks);
+    if (P->zrec) (void)hipFree(P->zrec);
     if (P->stats) (void)hipFree(P->stats);
     for (BPPlan::ZsTable& z : P->zs_tables) {
         if (z.tab) (void)hipFree(z.tab);
@@ -2098,7 +2173,7 @@ static const BPPlan::ZsTable& zs_table(const BPPlan* P, hipStream_t s) {
     try {
         ARECON_HIP(hipMalloc(&t.boff, sizeof(long long) * A.nzsets));
         ARECON_HIP(hipMalloc(&t.fits, sizeof(int) * (size_t)A.nzsets * A.ntiles));
-        ARECON_HIP(hipMalloc(&t.tab, sizeof(unsigned) * (size_t)std::max<long long>(total, 4)));
+        ARECON_HIP(hipMalloc(&t.tab, sizeof(uint2) * (size_t)std::max<long long>(total, 4)));
         ARECON_HIP(hipMemcpy(t.boff, boff.data(), sizeof(long long) * A.nzsets,
                              hipMemcpyHostToDevice));
         const int nrec = LY == ARECON_FP_QUADS ? A.YL - 1 : A.YL;
