@@ -188,24 +188,38 @@ def sample_batch(pairs, batch: int, patch: int, gen: torch.Generator):
 
 
 def train(net: Net, pairs, steps: int, batch: int = 4, patch: int = 64, lr: float = 1e-3,
-          seed: int = 0, log=None, log_every: int = 100, warmup: int = 300):
+          seed: int = 0, log=None, log_every: int = 100, warmup: int = 300,
+          ema: float = 0.0):
     """Adam with a linear learning-rate warmup over ``warmup`` steps, then a
     cosine decay; returns the loss history.  (Without the warmup one of three
     seeds of a 10,000-step, 192-crop run never left the plain FDK: its loss
-    stayed flat from the first steps.)"""
+    stayed flat from the first steps.)  ``ema`` > 0: the network ends with the
+    exponential moving average of its weights over the steps after the warmup
+    (w_avg += (1 - ema) (w - w_avg) after every step), which smooths the noise
+    of small-batch updates."""
     gen = torch.Generator().manual_seed(seed)
     net.pack()
     hist = []
+    avg = None
     t0 = time.perf_counter()
     for k in range(steps):
         x, t = sample_batch(pairs, batch, patch, gen)
         lr_k = lr * 0.5 * (1 + math.cos(math.pi * k / max(1, steps)))
         lr_k = max(lr_k, lr * 0.02) * min(1.0, (k + 1) / max(1, warmup))
         loss = net.train_step(x, t, lr=lr_k)
+        if ema > 0.0 and k + 1 >= warmup:
+            if avg is None:
+                avg = {n: (L.w.clone(), L.b.clone()) for n, L in net.layers.items()}
+            else:
+                for n, L in net.layers.items():
+                    avg[n][0].lerp_(L.w, 1.0 - ema)
+                    avg[n][1].lerp_(L.b, 1.0 - ema)
         if k % 10 == 0 or k == steps - 1:
             hist.append((k, float(loss)))
         if log and (k % log_every == 0 or k == steps - 1):
             log(f"step {k}: loss {hist[-1][1]:.5f} ({time.perf_counter() - t0:.1f} s)")
+    if avg is not None:
+        net.load_state_dict(avg)
     return hist
 
 

@@ -73,6 +73,9 @@ def parse():
                     help="training steps of each network (cnn)")
     ap.add_argument("--cnn-seeds", type=int, default=3,
                     help="network seeds of the 888x64 correction (cnn)")
+    ap.add_argument("--cnn-ema", type=float, default=0.0,
+                    help="weight moving-average factor of the training (cnn; 0: last weights)")
+    ap.add_argument("--cnn-batch", type=int, default=4, help="training patches per step (cnn)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch workload)")
     return ap.parse_args()
 
@@ -1025,8 +1028,8 @@ def bench_cnn(args):
 
     t0 = time.perf_counter()
     net = cnn.Net(seed=0)
-    hist = learned.train(net, pairs, steps=args.cnn_steps, batch=4, patch=64, lr=1e-3, log=log,
-                         log_every=1000)
+    hist = learned.train(net, pairs, steps=args.cnn_steps, batch=args.cnn_batch, patch=64,
+                         lr=1e-3, log=log, log_every=1000, ema=args.cnn_ema)
     torch.cuda.synchronize()
     t368 = time.perf_counter() - t0
     ev = learned.evaluate(net, held, spec, bound=0.3)
@@ -1035,8 +1038,8 @@ def bench_cnn(args):
     for seed in range(args.cnn_seeds):
         net = cnn.Net(seed=seed)
         t0 = time.perf_counter()
-        h = learned.train(net, cpairs, steps=args.cnn_steps, batch=4, patch=64, lr=1e-3,
-                          seed=seed, log=log, log_every=1000)
+        h = learned.train(net, cpairs, steps=args.cnn_steps, batch=args.cnn_batch, patch=64,
+                          lr=1e-3, seed=seed, log=log, log_every=1000, ema=args.cnn_ema)
         torch.cuda.synchronize()
         tt = time.perf_counter() - t0
         cev = learned.evaluate(net, cheld, cspec, bound=0.3)
@@ -1062,11 +1065,13 @@ def bench_cnn(args):
            "dtype": "bf16 weights/activations, f32 accumulate",
            "data": "synthetic: seeded random-ellipsoid phantoms and bodies, exact projections, "
                    "plain-FDK reconstructions",
-           "config": {"workload": "3-level 32/64/128 network, 64^3 patches, batch 4; 368x16 "
-                                  "phantoms -> 192x192x64; 888x64 body crops -> 192x192x64; "
-                                  "infer 512x512x256 (888x64, pitch 0.984)",
+           "config": {"workload": f"3-level 32/64/128 network, 64^3 patches, batch "
+                                  f"{args.cnn_batch}; 368x16 phantoms -> 192x192x64; 888x64 "
+                                  "body crops -> 192x192x64; infer 512x512x256 (888x64, "
+                                  "pitch 0.984)",
                       "train_pairs_368x16": len(pairs), "train_pairs_888x64": len(cpairs),
                       "steps": args.cnn_steps, "seeds": args.cnn_seeds,
+                      "weight_ema": args.cnn_ema,
                       "artifact_voxels": "phantom within 20 HU of its 3x3x3 neighbourhood mean",
                       "artifact_far_voxels": "artifact voxels more than 2 voxels from any "
                                              "voxel that is not one"},
