@@ -69,13 +69,13 @@ def parse():
     ap.add_argument("--cnn-pairs", type=int, default=200, help="training phantoms (cnn)")
     ap.add_argument("--cnn-clinical-pairs", type=int, default=192,
                     help="training crops of the 888x64 scan (cnn)")
-    ap.add_argument("--cnn-steps", type=int, default=8000,
+    ap.add_argument("--cnn-steps", type=int, default=12000,
                     help="training steps of each network (cnn)")
     ap.add_argument("--cnn-seeds", type=int, default=3,
                     help="network seeds of the 888x64 correction (cnn)")
     ap.add_argument("--cnn-ema", type=float, default=0.0,
                     help="weight moving-average factor of the training (cnn; 0: last weights)")
-    ap.add_argument("--cnn-batch", type=int, default=4, help="training patches per step (cnn)")
+    ap.add_argument("--cnn-batch", type=int, default=8, help="training patches per step (cnn)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch workload)")
     return ap.parse_args()
 
