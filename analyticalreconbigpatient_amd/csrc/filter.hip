@@ -34,6 +34,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -326,7 +327,11 @@ template <int R> __device__ __forceinline__ void twiddle(float2* v, float2 w1) {
 // Pass modes: first forward pass reads global rows (preweighted); last forward
 // pass multiplies by H and conjugates; last inverse pass stores rows, to global
 // memory or, as two real rows, to LDS (floats [0, YL) and [N, N + YL)).
-enum { SRC_LDS = 0, SRC_GLOBAL = 1 };
+// SRC_STAGED: the fused filter's next-pair buffer in dynamic LDS (fstage): rows
+// a and b at [0, YL) and [YL, 2 YL), their preweights at [2 YL, 3 YL) and
+// [3 YL, 4 YL), copied there by fstage_copy while the previous pair ran.
+enum { SRC_LDS = 0, SRC_GLOBAL = 1, SRC_STAGED = 2 };
+extern __shared__ __attribute__((aligned(16))) float fstage[];
 enum { DST_LDS = 0, DST_LDS_SPEC = 1, DST_GLOBAL = 2, DST_LDS_ROWS = 3 };
 
 struct RowCtx {
@@ -359,7 +364,10 @@ constexpr int kPadShift = 5;
 __device__ __forceinline__ int lpad(int k) { return k + (k >> kPadShift); }
 template <int N> constexpr int lds_fft_len() { return N + (N >> kPadShift); }
 
-template <int N, int R, int Ns, int SRC, int DST, class OutT, int S>
+// WAITV: wait for this wavefront's outstanding memory operations (the staging
+// copies of the next pair) before the pass's read/write barrier, which then
+// makes them visible to the workgroup.
+template <int N, int R, int Ns, int SRC, int DST, class OutT, int S, bool WAITV = false>
 __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict__ twg,
                                           const RowCtx& c, const RowOut<OutT>& o) {
     constexpr int T = tpt<N, S>();
@@ -381,8 +389,13 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
             } else if (r < R / 2) {                          // n < N/2; n >= YL reads zero
                 float2 x = make_float2(0.f, 0.f);
                 if (n < c.YL) {
-                    x.x = c.ia[n] * c.wa[n];
-                    if (c.has_b) x.y = c.ib[n] * c.wb[n];
+                    if (SRC == SRC_STAGED) {
+                        x.x = fstage[n] * fstage[2 * c.YL + n];
+                        if (c.has_b) x.y = fstage[c.YL + n] * fstage[3 * c.YL + n];
+                    } else {
+                        x.x = c.ia[n] * c.wa[n];
+                        if (c.has_b) x.y = c.ib[n] * c.wb[n];
+                    }
                 }
                 v[b][r] = x;
             } else {
@@ -390,6 +403,8 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
             }
         }
     }
+    static_assert(!WAITV || (SRC == SRC_LDS && DST != DST_GLOBAL), "the wait needs the pass's barrier");
+    if (WAITV) __builtin_amdgcn_s_waitcnt(0);
     if (SRC == SRC_LDS && DST != DST_GLOBAL) __syncthreads();   // all reads before in-place writes
 #pragma unroll
     for (int b = 0; b < PER; ++b) {
@@ -430,35 +445,72 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
 
 // The whole forward (MODE 0: global rows -> spectrum in LDS) or inverse
 // (MODE 1: LDS -> global rows; MODE 2: LDS -> the two real rows in LDS)
-// transform of one pair of rows.
-template <int N, int MODE, class OutT, int S>
+// transform of one pair of rows.  PF (the fused filter's prefetch): the forward
+// transform reads its rows from the staging buffer and calls after_first() once
+// its first pass is done with them; the inverse transform's first pass waits
+// for the staging copies before its barrier.
+template <int N, int MODE, class OutT, int S, bool PF = false, class F = void (*)()>
 __device__ __forceinline__ void wave_fft(float2* lds, const float2* __restrict__ twg,
-                                         const RowCtx& c, const RowOut<OutT>& o) {
-    constexpr int S0 = (MODE == 0) ? SRC_GLOBAL : SRC_LDS;
+                                         const RowCtx& c, const RowOut<OutT>& o,
+                                         F&& after_first = [] {}) {
+    constexpr int S0 = (MODE == 0) ? (PF ? SRC_STAGED : SRC_GLOBAL) : SRC_LDS;
     constexpr int DL = (MODE == 0) ? DST_LDS_SPEC : (MODE == 1 ? DST_GLOBAL : DST_LDS_ROWS);
+    constexpr bool W0 = PF && MODE == 2;
     if constexpr (N == 2048 && S == 1) {
-        wave_pass<N, 8, 1, S0, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 8, 1, S0, DST_LDS, OutT, S, W0>(lds, twg, c, o);
+        if constexpr (MODE == 0) after_first();
         wave_pass<N, 8, 8, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
         wave_pass<N, 8, 64, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
         wave_pass<N, 4, 512, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
     } else if constexpr (N == 2048) {
-        wave_pass<N, 16, 1, S0, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 16, 1, S0, DST_LDS, OutT, S, W0>(lds, twg, c, o);
+        if constexpr (MODE == 0) after_first();
         wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
         wave_pass<N, 8, 256, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
     } else if constexpr (N == 1024) {
-        wave_pass<N, 16, 1, S0, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 16, 1, S0, DST_LDS, OutT, S, W0>(lds, twg, c, o);
+        if constexpr (MODE == 0) after_first();
         wave_pass<N, 16, 16, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
         wave_pass<N, 4, 256, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
     } else if constexpr (N == 512) {
-        wave_pass<N, 8, 1, S0, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 8, 1, S0, DST_LDS, OutT, S, W0>(lds, twg, c, o);
+        if constexpr (MODE == 0) after_first();
         wave_pass<N, 8, 8, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
         wave_pass<N, 8, 64, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
     } else {
         static_assert(N == 256, "wave FFT supports N = 256 .. 2048");
-        wave_pass<N, 4, 1, S0, DST_LDS, OutT, S>(lds, twg, c, o);
+        wave_pass<N, 4, 1, S0, DST_LDS, OutT, S, W0>(lds, twg, c, o);
+        if constexpr (MODE == 0) after_first();
         wave_pass<N, 4, 4, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
         wave_pass<N, 4, 16, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
         wave_pass<N, 4, 64, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
+    }
+}
+
+// Copies rows k, k + 1 of a view (`rows` points at row k) and their preweights
+// (`w` at row k of the preweight table) into fstage with direct-to-LDS 16-byte
+// buffer loads: no registers, no wait here.  A row of YL floats (YL % 4 == 0,
+// 16-byte aligned) is ceil(YL / 256) loads; the tail's lanes are switched off,
+// so nothing lands past the row.  Row k + 1 and its weights only when has_b.
+template <int T>
+__device__ __forceinline__ void fstage_copy(const float* rows, const float* w, bool has_b,
+                                            int YL) {
+    typedef __attribute__((address_space(3))) float lds_float;
+    typedef __attribute__((address_space(3))) void lds_void;
+    constexpr int NW = T / 64;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned rowb = 4u * (unsigned)YL;
+    for (int r = wave; r < 4; r += NW) {       // 0, 1: rows k, k + 1; 2, 3: their weights
+        if ((r & 1) && !has_b) continue;
+        const float* src = (r < 2 ? rows : w) + (r & 1) * YL;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float*>(src), (short)0, (int)rowb, 0x00020000);
+        lds_float* dst = (lds_float*)fstage + r * YL;
+        for (unsigned off = 0; off < rowb; off += 1024u)
+            if (off + 16u * lane < rowb)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void*)(dst + off / 4u), 16,
+                                                        off + 16u * lane, 0, 0, 0);
     }
 }
 
@@ -658,7 +710,11 @@ __global__ __launch_bounds__(256) void expand_quads(OutT* __restrict__ out,
 // k-1, kept from the previous pair) and k.  The differences are the same FP32
 // ones that expand_quads / expand_rowdiff form from stored rows, rounded once to
 // OutT, so the output is identical; the FP32 rows never reach HBM.
-template <int N, class OutT, int S, int OUTL>
+// PF (YL % 4 == 0): the next pair's rows and preweights are copied into the
+// dynamic LDS buffer fstage (4 YL floats) while this pair's transforms run, and
+// the first pass reads them from there: its global loads (rows from HBM,
+// preweights from L2) otherwise wait once per pair.
+template <int N, class OutT, int S, int OUTL, bool PF>
 __global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
     OutT* __restrict__ out, const float* in, const float2* __restrict__ twg,
     const float* __restrict__ Hg, const float* __restrict__ Wg, int YL, int ZL, int ViewN,
@@ -672,6 +728,11 @@ __global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
     RowOut<OutT> o{};                            // the rows stay in LDS
     for (int v = blockIdx.x; v < ViewN; v += gridDim.x) {
         OutT* ov = out + (long long)v * (ZL - 1) * (OUTL == ARECON_FP_QUADS ? nc * 4 : YL * 2);
+        if constexpr (PF) {
+            fstage_copy<T>(in + (long long)v * ZL * YL, Wg, true, YL);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
         for (int k = 0; k < ZL; k += 2) {
             RowCtx c;
             c.has_b = k + 1 < ZL;
@@ -680,8 +741,15 @@ __global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
             c.wa = Wg + (long long)k * YL;
             c.wb = Wg + (long long)(c.has_b ? k + 1 : 0) * YL;
             c.YL = YL; c.PLC = PLC; c.dYL = dYL; c.SO = SO; c.H = Hg;
-            wave_fft<N, 0, OutT, S>(lds, twg, c, o);
-            wave_fft<N, 2, OutT, S>(lds, twg, c, o);
+            // the next pair's copies start once the first pass has read this
+            // pair's (its barrier is behind)
+            wave_fft<N, 0, OutT, S, PF>(lds, twg, c, o, [&] {
+                if constexpr (PF)
+                    if (k + 2 < ZL)
+                        fstage_copy<T>(in + ((long long)v * ZL + k + 2) * YL,
+                                       Wg + (long long)(k + 2) * YL, k + 3 < ZL, YL);
+            });
+            wave_fft<N, 2, OutT, S, PF>(lds, twg, c, o);
             // rows k (la) and k+1 (lb) are in LDS, row k-1 in prev[(k/2) & 1]
             // (k > 0); row k+1 goes to the other buffer for the next pair, which
             // reads it only after the barriers of its transforms
@@ -773,28 +841,46 @@ static void launch_filter_rows(void* out, int out_dtype, int out_layout, const f
                                int YL, int ZL, int ViewN, float PLC, float ZLC, float dYL,
                                float dZL, float SO, hipStream_t s, const TableSet* fixed);
 
-// Launch of the fused per-view filter for layout OUTL.
+// Launch of the fused per-view filter for layout OUTL (PF: rows staged in LDS
+// a pair ahead, dynamic LDS of 4 YL floats).
+template <int OUTL, class OT, bool PF>
+static void launch_filter_fused_pf(OT* outp, const float* in, const TableSet& t, const float* W,
+                                   int N, int YL, int ZL, int ViewN, float PLC, float dYL,
+                                   float SO, hipStream_t s) {
+    const dim3 grid((unsigned)ViewN);
+    const unsigned dyn = PF ? 16u * (unsigned)YL : 0u;
+    switch (N) {
+    case 256: hipLaunchKernelGGL((filter_quads_wave<256, OT, 0, OUTL, PF>), grid, dim3(tpt<256>()),
+                                 dyn, s, outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
+    case 512: hipLaunchKernelGGL((filter_quads_wave<512, OT, 0, OUTL, PF>), grid, dim3(tpt<512>()),
+                                 dyn, s, outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
+    case 1024: hipLaunchKernelGGL((filter_quads_wave<1024, OT, 0, OUTL, PF>), grid,
+                                  dim3(tpt<1024>()), dyn, s, outp, in, t.tw, t.H, W, YL, ZL, ViewN,
+                                  PLC, dYL, SO); break;
+    default:
+        if (filter_schedule() == 1)
+            hipLaunchKernelGGL((filter_quads_wave<2048, OT, 1, OUTL, PF>), grid,
+                               dim3(tpt<2048, 1>()), dyn, s, outp, in, t.tw, t.H, W, YL, ZL,
+                               ViewN, PLC, dYL, SO);
+        else
+            hipLaunchKernelGGL((filter_quads_wave<2048, OT, 0, OUTL, PF>), grid,
+                               dim3(tpt<2048, 0>()), dyn, s, outp, in, t.tw, t.H, W, YL, ZL,
+                               ViewN, PLC, dYL, SO);
+        break;
+    }
+}
+
 template <int OUTL, class OT>
 static void launch_filter_fused(OT* outp, const float* in, const TableSet& t, const float* W,
                                 int N, int YL, int ZL, int ViewN, float PLC, float dYL, float SO,
                                 hipStream_t s) {
-    const dim3 grid((unsigned)ViewN);
-    switch (N) {
-    case 256: hipLaunchKernelGGL((filter_quads_wave<256, OT, 0, OUTL>), grid, dim3(tpt<256>()), 0, s,
-                                 outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
-    case 512: hipLaunchKernelGGL((filter_quads_wave<512, OT, 0, OUTL>), grid, dim3(tpt<512>()), 0, s,
-                                 outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
-    case 1024: hipLaunchKernelGGL((filter_quads_wave<1024, OT, 0, OUTL>), grid, dim3(tpt<1024>()), 0, s,
-                                  outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO); break;
-    default:
-        if (filter_schedule() == 1)
-            hipLaunchKernelGGL((filter_quads_wave<2048, OT, 1, OUTL>), grid, dim3(tpt<2048, 1>()), 0, s,
-                               outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO);
-        else
-            hipLaunchKernelGGL((filter_quads_wave<2048, OT, 0, OUTL>), grid, dim3(tpt<2048, 0>()), 0, s,
-                               outp, in, t.tw, t.H, W, YL, ZL, ViewN, PLC, dYL, SO);
-        break;
-    }
+    // the staging copies move 16-byte pieces of 16-byte aligned rows
+    const bool pf = YL % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(W) & 15) == 0;
+    if (pf)
+        launch_filter_fused_pf<OUTL, OT, true>(outp, in, t, W, N, YL, ZL, ViewN, PLC, dYL, SO, s);
+    else
+        launch_filter_fused_pf<OUTL, OT, false>(outp, in, t, W, N, YL, ZL, ViewN, PLC, dYL, SO, s);
 }
 
 // fixed: the tables to read (the allocation-free entry point's scratch), or
