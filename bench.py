@@ -261,10 +261,13 @@ def bench_clinical(args):
         "stages_ms": {"rebin": round(float(plain["stages"][0]), 3),
                       "filter": round(float(plain["stages"][1]), 3),
                       "backproject": round(float(plain["stages"][2]), 3)},
+        # algorithmic bytes: the rebinning reads and writes 4 B per sample; the
+        # filter reads 4 B per sample and writes its output layout (FP32 row
+        # differences: 8 B per record)
         "stage_hbm_GBs": {
             "rebin": round(8.0 * p.YL * p.ZL * fdk.ViewN / (plain["stages"][0] * 1e-3) / 1e9, 1),
-            "filter": round(8.0 * p.YL * p.ZL * fdk.ViewN / (plain["stages"][1] * 1e-3) / 1e9,
-                            1)},
+            "filter": round((4.0 * p.YL * p.ZL + fdk.filtered[0].numel() * 4.0) * fdk.ViewN /
+                            (plain["stages"][1] * 1e-3) / 1e9, 1)},
         "chosen": {"form": (f"FP16 filtered storage + {len(CHOSEN_SLABS)} z-slabs "
                             f"{list(CHOSEN_SLABS)} in one batched launch, pass replayed from a "
                             f"HIP graph") if chosen["recon"] is not None else "FP16 storage",
