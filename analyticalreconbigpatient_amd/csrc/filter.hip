@@ -332,7 +332,9 @@ template <int R> __device__ __forceinline__ void twiddle(float2* v, float2 w1) {
 // [3 YL, 4 YL), copied there by fstage_copy while the previous pair ran.
 enum { SRC_LDS = 0, SRC_GLOBAL = 1, SRC_STAGED = 2 };
 extern __shared__ __attribute__((aligned(16))) float fstage[];
-enum { DST_LDS = 0, DST_LDS_SPEC = 1, DST_GLOBAL = 2, DST_LDS_ROWS = 3 };
+// DST_ROWDIFF: the fused filter's row differences straight from the last
+// inverse pass (RowCtx's rowdiff fields).
+enum { DST_LDS = 0, DST_LDS_SPEC = 1, DST_GLOBAL = 2, DST_LDS_ROWS = 3, DST_ROWDIFF = 4 };
 
 struct RowCtx {
     const float* ia; const float* ib;      // input rows (ib valid when has_b)
@@ -341,6 +343,32 @@ struct RowCtx {
     int YL;
     float PLC, dYL, SO;
     const float* H;
+    // DST_ROWDIFF: the view's output (OutT), the pair's first row k, the
+    // filtered row k - 1 (LDS, k > 0) and where row k + 1 goes for the next pair
+    void* ov;
+    int krow;
+    const float* pr;
+    float* pn;
+};
+
+// Two values as one 8-byte (FP32) or 4-byte (16-bit) store.
+template <class OutT> struct Pair2;
+template <> struct Pair2<float> {
+    __device__ static __forceinline__ void put(float* o, float a, float b) {
+        *reinterpret_cast<float2*>(o) = make_float2(a, b);
+    }
+};
+template <> struct Pair2<__half> {
+    __device__ static __forceinline__ void put(__half* o, float a, float b) {
+        *reinterpret_cast<unsigned*>(o) = (unsigned)__half_as_ushort(__float2half(a)) |
+                                          ((unsigned)__half_as_ushort(__float2half(b)) << 16);
+    }
+};
+template <> struct Pair2<__hip_bfloat16> {
+    __device__ static __forceinline__ void put(__hip_bfloat16* o, float a, float b) {
+        auto bits = [](float v) { return (unsigned)__bfloat16_as_ushort(__float2bfloat16(v)); };
+        *reinterpret_cast<unsigned*>(o) = bits(a) | (bits(b) << 16);
+    }
 };
 
 // Threads per transform: one wavefront up to N = 1024.  At N = 2048, schedule
@@ -435,6 +463,20 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
                     rows[k] = v[b][r].x;
                     rows[N + k] = -v[b][r].y;
                 }
+            } else if (DST == DST_ROWDIFF) {    // rows krow, krow + 1 of channel k
+                if (k < c.YL) {
+                    const float a0 = v[b][r].x, b0 = -v[b][r].y;
+                    OutT* ov = static_cast<OutT*>(c.ov);
+                    if (c.krow > 0) {
+                        const float p0 = c.pr[k];
+                        Pair2<OutT>::put(ov + ((long long)(c.krow - 1) * c.YL + k) * 2, p0,
+                                         a0 - p0);
+                    }
+                    if (c.has_b) {
+                        c.pn[k] = b0;
+                        Pair2<OutT>::put(ov + ((long long)c.krow * c.YL + k) * 2, a0, b0 - a0);
+                    }
+                }
             } else if (k < c.YL) {
                 o.put(k, v[b][r].x, -v[b][r].y);
             }
@@ -444,8 +486,8 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
 }
 
 // The whole forward (MODE 0: global rows -> spectrum in LDS) or inverse
-// (MODE 1: LDS -> global rows; MODE 2: LDS -> the two real rows in LDS)
-// transform of one pair of rows.  PF (the fused filter's prefetch): the forward
+// (MODE 1: LDS -> global rows; MODE 2: LDS -> the two real rows in LDS; MODE 3:
+// LDS -> row differences in global memory) transform of one pair of rows.  PF (the fused filter's prefetch): the forward
 // transform reads its rows from the staging buffer and calls after_first() once
 // its first pass is done with them; the inverse transform's first pass waits
 // for the staging copies before its barrier.
@@ -454,8 +496,10 @@ __device__ __forceinline__ void wave_fft(float2* lds, const float2* __restrict__
                                          const RowCtx& c, const RowOut<OutT>& o,
                                          F&& after_first = [] {}) {
     constexpr int S0 = (MODE == 0) ? (PF ? SRC_STAGED : SRC_GLOBAL) : SRC_LDS;
-    constexpr int DL = (MODE == 0) ? DST_LDS_SPEC : (MODE == 1 ? DST_GLOBAL : DST_LDS_ROWS);
-    constexpr bool W0 = PF && MODE == 2;
+    constexpr int DL = MODE == 0 ? DST_LDS_SPEC
+                       : MODE == 1 ? DST_GLOBAL
+                       : MODE == 2 ? DST_LDS_ROWS : DST_ROWDIFF;
+    constexpr bool W0 = PF && MODE >= 2;
     if constexpr (N == 2048 && S == 1) {
         wave_pass<N, 8, 1, S0, DST_LDS, OutT, S, W0>(lds, twg, c, o);
         if constexpr (MODE == 0) after_first();
@@ -666,25 +710,6 @@ template <> struct Quad4<__hip_bfloat16> {
 
 // The two values of a row-difference element, f(r, c) and f(r+1, c) - f(r, c),
 // in one store (8 bytes in FP32, 4 in FP16 / BF16).
-template <class OutT> struct Pair2;
-template <> struct Pair2<float> {
-    __device__ static __forceinline__ void put(float* o, float a, float b) {
-        *reinterpret_cast<float2*>(o) = make_float2(a, b);
-    }
-};
-template <> struct Pair2<__half> {
-    __device__ static __forceinline__ void put(__half* o, float a, float b) {
-        *reinterpret_cast<unsigned*>(o) = (unsigned)__half_as_ushort(__float2half(a)) |
-                                          ((unsigned)__half_as_ushort(__float2half(b)) << 16);
-    }
-};
-template <> struct Pair2<__hip_bfloat16> {
-    __device__ static __forceinline__ void put(__hip_bfloat16* o, float a, float b) {
-        auto bits = [](float v) { return (unsigned)__bfloat16_as_ushort(__float2bfloat16(v)); };
-        *reinterpret_cast<unsigned*>(o) = bits(a) | (bits(b) << 16);
-    }
-};
-
 template <class OutT>
 __global__ __launch_bounds__(256) void expand_quads(OutT* __restrict__ out,
                                                     const float* __restrict__ rows, int YL,
@@ -749,24 +774,25 @@ __global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
                         fstage_copy<T>(in + ((long long)v * ZL + k + 2) * YL,
                                        Wg + (long long)(k + 2) * YL, k + 3 < ZL, YL);
             });
+            const float* pr = prev[(k >> 1) & 1];
+            float* pn = prev[((k >> 1) & 1) ^ 1];
+            if constexpr (OUTL == ARECON_FP_ROWDIFF) {
+                // row differences from the last inverse pass: rows k and k + 1 of
+                // a channel are one thread's, row k - 1 comes from prev, and row
+                // k + 1 goes to the other prev buffer for the next pair (which
+                // reads it only after the barriers of its transforms)
+                c.ov = ov; c.krow = k; c.pr = pr; c.pn = pn;
+                wave_fft<N, 3, OutT, S, PF>(lds, twg, c, o);
+                continue;
+            }
             wave_fft<N, 2, OutT, S, PF>(lds, twg, c, o);
             // rows k (la) and k+1 (lb) are in LDS, row k-1 in prev[(k/2) & 1]
             // (k > 0); row k+1 goes to the other buffer for the next pair, which
             // reads it only after the barriers of its transforms
-            const float* pr = prev[(k >> 1) & 1];
-            float* pn = prev[((k >> 1) & 1) ^ 1];
             for (int ch = threadIdx.x; ch < YL; ch += T) {
                 const float a0 = la[ch];
                 const float b0 = c.has_b ? lb[ch] : 0.f;
                 if (c.has_b) pn[ch] = b0;
-                if constexpr (OUTL == ARECON_FP_ROWDIFF) {
-                    if (k > 0)
-                        Pair2<OutT>::put(ov + ((long long)(k - 1) * YL + ch) * 2, pr[ch],
-                                         a0 - pr[ch]);
-                    if (c.has_b)
-                        Pair2<OutT>::put(ov + ((long long)k * YL + ch) * 2, a0, b0 - a0);
-                    continue;
-                }
                 if (ch >= nc) continue;
                 const float a1 = la[ch + 1];
                 if (k > 0) {
