@@ -48,8 +48,16 @@ namespace {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// A product that is never contracted into an FMA with the sum it feeds.
+__device__ __forceinline__ float mul_nc(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+// Complex products with their FMAs spelled out: left to contraction, which of
+// the two products of a component is fused depends on the code around the
+// call, so the same transform inlined in two kernels could round differently.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(fmaf(a.x, b.x, -mul_nc(a.y, b.y)), fmaf(a.x, b.y, mul_nc(a.y, b.x)));
 }
 
 // Radix-4 DFT butterfly (forward sign).
@@ -122,7 +130,7 @@ __device__ __forceinline__ void emit(float2* dst, int d, float2 v, const float* 
         dst[d] = v;
     } else if constexpr (EPI == 1) {
         const float hs = H[d];
-        dst[d] = make_float2(hs * v.x, -hs * v.y);
+        dst[d] = make_float2(mul_nc(hs, v.x), mul_nc(-hs, v.y));
     } else {
         if (d < YL) o.put(d, v.x, -v.y);
     }
@@ -226,13 +234,13 @@ __global__ __launch_bounds__(256) void filter_rows(
         for (int j = threadIdx.x; j < q; j += blockDim.x) {
             float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
             if (j < YL) {
-                v0.x = ia[j] * preweight(j, ba, PLC, dYL, SO);
-                if (has_b) v0.y = ib[j] * preweight(j, bb, PLC, dYL, SO);
+                v0.x = mul_nc(ia[j], preweight(j, ba, PLC, dYL, SO));
+                if (has_b) v0.y = mul_nc(ib[j], preweight(j, bb, PLC, dYL, SO));
             }
             const int j1 = j + q;
             if (j1 < YL) {
-                v1.x = ia[j1] * preweight(j1, ba, PLC, dYL, SO);
-                if (has_b) v1.y = ib[j1] * preweight(j1, bb, PLC, dYL, SO);
+                v1.x = mul_nc(ia[j1], preweight(j1, ba, PLC, dYL, SO));
+                if (has_b) v1.y = mul_nc(ib[j1], preweight(j1, bb, PLC, dYL, SO));
             }
             const float2 nv1 = make_float2(v1.y, -v1.x);   // -i * v1
             bufA[4 * j + 0] = cadd(v0, v1);
@@ -266,7 +274,7 @@ __global__ __launch_bounds__(256) void filter_rows(
 // spectrum H is read through L1.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float2 cscale(float2 a, float c, float s) {   // a * (c + i s)
-    return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+    return make_float2(fmaf(a.x, c, -mul_nc(a.y, s)), fmaf(a.x, s, mul_nc(a.y, c)));
 }
 
 template <int R> __device__ __forceinline__ void dft(float2* v);
@@ -417,12 +425,13 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
             } else if (r < R / 2) {                          // n < N/2; n >= YL reads zero
                 float2 x = make_float2(0.f, 0.f);
                 if (n < c.YL) {
+                    // preweighted samples, rounded before the butterflies add them
                     if (SRC == SRC_STAGED) {
-                        x.x = fstage[n] * fstage[2 * c.YL + n];
-                        if (c.has_b) x.y = fstage[c.YL + n] * fstage[3 * c.YL + n];
+                        x.x = mul_nc(fstage[n], fstage[2 * c.YL + n]);
+                        if (c.has_b) x.y = mul_nc(fstage[c.YL + n], fstage[3 * c.YL + n]);
                     } else {
-                        x.x = c.ia[n] * c.wa[n];
-                        if (c.has_b) x.y = c.ib[n] * c.wb[n];
+                        x.x = mul_nc(c.ia[n], c.wa[n]);
+                        if (c.has_b) x.y = mul_nc(c.ib[n], c.wb[n]);
                     }
                 }
                 v[b][r] = x;
@@ -456,7 +465,7 @@ __device__ __forceinline__ void wave_pass(float2* lds, const float2* __restrict_
                 lds[kp] = v[b][r];
             } else if (DST == DST_LDS_SPEC) {
                 const float hs = c.H[k];
-                lds[kp] = make_float2(hs * v[b][r].x, -hs * v[b][r].y);
+                lds[kp] = make_float2(mul_nc(hs, v[b][r].x), mul_nc(-hs, v[b][r].y));
             } else if (DST == DST_LDS_ROWS) {   // unpadded rows (2 N floats)
                 if (k < c.YL) {
                     float* rows = reinterpret_cast<float*>(lds);
@@ -529,6 +538,19 @@ __device__ __forceinline__ void wave_fft(float2* lds, const float2* __restrict__
         wave_pass<N, 4, 16, SRC_LDS, DST_LDS, OutT, S>(lds, twg, c, o);
         wave_pass<N, 4, 64, SRC_LDS, DL, OutT, S>(lds, twg, c, o);
     }
+}
+
+// A pair's forward transform (its first pass then after_first()) and inverse
+// transform MODE_INV (2 or 3 of wave_fft).  (Fusing the forward's last pass
+// with the inverse's first, whose elements are the same threads' at N = 2048,
+// saves an LDS round trip and two barriers per pair but measured slower: 2.53
+// against 2.35 ms on the clinical sinogram.)
+template <int N, int MODE_INV, class OutT, int S, bool PF, class F>
+__device__ __forceinline__ void wave_fft_pair(float2* lds, const float2* __restrict__ twg,
+                                              const RowCtx& c, const RowOut<OutT>& o,
+                                              F&& after_first) {
+    wave_fft<N, 0, OutT, S, PF>(lds, twg, c, o, after_first);
+    wave_fft<N, MODE_INV, OutT, S, PF>(lds, twg, c, o);
 }
 
 // Copies rows k, k + 1 of a view (`rows` points at row k) and their preweights
@@ -766,26 +788,23 @@ __global__ __launch_bounds__((tpt<N, S>())) void filter_quads_wave(
             c.wa = Wg + (long long)k * YL;
             c.wb = Wg + (long long)(c.has_b ? k + 1 : 0) * YL;
             c.YL = YL; c.PLC = PLC; c.dYL = dYL; c.SO = SO; c.H = Hg;
-            // the next pair's copies start once the first pass has read this
-            // pair's (its barrier is behind)
-            wave_fft<N, 0, OutT, S, PF>(lds, twg, c, o, [&] {
-                if constexpr (PF)
-                    if (k + 2 < ZL)
-                        fstage_copy<T>(in + ((long long)v * ZL + k + 2) * YL,
-                                       Wg + (long long)(k + 2) * YL, k + 3 < ZL, YL);
-            });
             const float* pr = prev[(k >> 1) & 1];
             float* pn = prev[((k >> 1) & 1) ^ 1];
-            if constexpr (OUTL == ARECON_FP_ROWDIFF) {
-                // row differences from the last inverse pass: rows k and k + 1 of
-                // a channel are one thread's, row k - 1 comes from prev, and row
-                // k + 1 goes to the other prev buffer for the next pair (which
-                // reads it only after the barriers of its transforms)
-                c.ov = ov; c.krow = k; c.pr = pr; c.pn = pn;
-                wave_fft<N, 3, OutT, S, PF>(lds, twg, c, o);
-                continue;
-            }
-            wave_fft<N, 2, OutT, S, PF>(lds, twg, c, o);
+            // Row differences come from the last inverse pass: rows k and k + 1
+            // of a channel are one thread's, row k - 1 comes from prev, and row
+            // k + 1 goes to the other prev buffer for the next pair (which reads
+            // it only after the barriers of its transforms).  The next pair's
+            // copies start once the first pass has read this pair's (its barrier
+            // is behind).
+            c.ov = ov; c.krow = k; c.pr = pr; c.pn = pn;
+            wave_fft_pair<N, OUTL == ARECON_FP_ROWDIFF ? 3 : 2, OutT, S, PF>(
+                lds, twg, c, o, [&] {
+                    if constexpr (PF)
+                        if (k + 2 < ZL)
+                            fstage_copy<T>(in + ((long long)v * ZL + k + 2) * YL,
+                                           Wg + (long long)(k + 2) * YL, k + 3 < ZL, YL);
+                });
+            if constexpr (OUTL == ARECON_FP_ROWDIFF) continue;
             // rows k (la) and k+1 (lb) are in LDS, row k-1 in prev[(k/2) & 1]
             // (k > 0); row k+1 goes to the other buffer for the next pair, which
             // reads it only after the barriers of its transforms
