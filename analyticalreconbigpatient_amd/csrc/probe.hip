@@ -4,7 +4,7 @@
 // The LDS-staged backprojector reads every voxel-update's samples from LDS
 // (ds_read2_b32 / ds_read2_b64: 8 or 16 bytes per lane), after the footprint
 // of each view has been staged from L2/HBM.  Its algorithmic sample traffic
-// (16 B per voxel-update in FP32, SURVEY's gather roof) is therefore bounded by
+// (16 B per voxel-update in FP32, the gather roof) is therefore bounded by
 // LDS read bandwidth, and the gathering z-column kernel's by L2 gathers.  These
 // kernels measure what the device delivers for those two access kinds:
 //

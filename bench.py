@@ -937,7 +937,7 @@ def bench_batch(args):
 
 
 def bench_cnn(args):
-    """The learned correction: the 3-level 32/64/128 network
+    """The learned correction (512x512x256 workload): the 3-level 32/64/128 network
     with its BF16 MFMA kernels, trained on 64^3 patches of pairs the method
     makes, one network per scanner:
 

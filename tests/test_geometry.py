@@ -37,7 +37,7 @@ def test_derived_numbers(wl, exp):
     assert oracle.filter_nn(p.YL) == exp["NN"]
     assert G.seg_overlap(w.hdr) == exp["seg"]
     assert p.YL * p.ZL * p.PN == pytest.approx(exp["samples"], rel=1e-3)
-    # rebinning step is the segment's (BetaE-BetaS)/(ViewN-1) (Q4 of the survey)
+    # rebinning step is the segment's (BetaE-BetaS)/(ViewN-1) (analyticalRecon.m's segment arithmetic)
     assert float(a.DeltaFai) == pytest.approx(2 * math.pi / p.N_2pi * p.PN / (p.PN - 1), rel=1e-6)
     assert p.YLC == (p.YL - 1) / 2
 
