@@ -1,0 +1,105 @@
+"""Seeded sweep of random scan geometries through the LDS-staged backprojector
+(variant 3): bit-identical to the z-column kernel (variant 2) in every storage
+and layout, and within 1e-4 relative L2 of the FP64 oracle for FP32 storage.
+
+Each case draws from its own seed the detector (channels, rows, centre
+offsets, pitches), the grid (size, centre, voxel size, slice spacing, the
+slices' place on the helix), the helix (views per rotation, pitch, k1), the
+number of views and the slices per thread; the case's parameters are printed
+before it runs.  The ranges give ragged last slice blocks, windows cut by the
+first or the last view, partial tiles, columns projecting past the detector's
+edges, and footprints too wide or too tall for the staging slots (those
+workgroups run the gathering body).  The comparison with the oracle leaves out
+the few voxels whose sum the reference itself changes by a whole sample when
+the detector moves by a rounding error (edge_stable).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+
+from .test_gpu_bp_lds import STORAGE_LAYOUT, run_bp
+from .test_gpu_stages import rel_l2
+from .test_oracle_fdk import geom as bp_geom_dict
+
+pytestmark = pytest.mark.gpu
+
+N_CASES = 16
+
+
+def draw(seed):
+    r = np.random.default_rng(7000 + seed)
+    N_2pi = int(r.choice([64, 96, 128]))
+    XN, YN, ZN = int(r.integers(8, 49)), int(r.integers(8, 49)), int(r.integers(3, 38))
+    dx = float(r.uniform(1.2, 4.6))
+    dz = float(r.uniform(0.3, 1.2))
+    h = float(r.uniform(12.0, 22.0))
+    PN = int(r.integers(150, 261))
+    YL, ZL = int(r.integers(64, 161)), int(r.integers(8, 21))
+    # the grid's middle slice sits at view s0c of the helix
+    DeltaFai = 2 * math.pi / N_2pi
+    s0c = float(r.uniform(0.15, 0.85)) * PN
+    zc = h * (math.pi + s0c * DeltaFai) / (2 * math.pi)
+    g = bp_geom_dict(N_2pi=N_2pi, XN=XN, YN=YN, ZN=ZN,
+                     XNC=(XN - 1) / 2 + float(r.uniform(-3, 3)),
+                     YNC=(YN - 1) / 2 + float(r.uniform(-3, 3)),
+                     dx=dx, dy=dx, dz=dz, ZNC=(ZN - 1) / 2 - zc / dz,
+                     YLC=(YL - 1) / 2 + float(r.uniform(-4, 4)),
+                     ZLC=(ZL - 1) / 2 + float(r.uniform(-1, 1)),
+                     dYL=float(r.uniform(2.5, 3.5)), dZL=float(r.uniform(1.5, 2.3)), h=h,
+                     k1=float(r.choice([0.0, 2.85, 5.0])))
+    # every storage / layout pair with both block heights
+    nz = 8 if (seed // len(STORAGE_LAYOUT)) % 2 == 0 else 16
+    storage, layout = STORAGE_LAYOUT[seed % len(STORAGE_LAYOUT)]
+    return g, (PN, ZL, YL), nz, storage, layout
+
+
+@pytest.mark.parametrize("case", range(N_CASES))
+def test_random_geometry_lds_kernel(gpu, case):
+    g, shape, nz, storage, layout = draw(case)
+    print(f"bp sweep case {case}: fp {shape} nz={nz} {storage} {layout} "
+          f"grid {g['XN']}x{g['YN']}x{g['ZN']} dx={g['dx']:.3f} dz={g['dz']:.3f} "
+          f"ZNC={g['ZNC']:.2f} N_2pi={g['N_2pi']} h={g['h']:.2f} k1={g['k1']}", flush=True)
+    fp = np.random.default_rng(case).normal(size=shape).astype(np.float32)
+    got, w, (staged, direct) = run_bp(fp, g, 3, nz, storage, layout)
+    want, _, _ = run_bp(fp, g, 2, nz, storage, layout)
+    print(f"  workgroups staged {staged}, gathering {direct}", flush=True)
+    assert not np.isnan(got).any()
+    assert np.array_equal(got, want)
+    if storage == torch.float32:
+        kw = {k: v for k, v in g.items() if k not in ("YL", "ZL")}
+        fp64 = fp.astype(np.float64)
+        ref = oracle.fdk_core(fp64, w, **kw)
+        if np.abs(ref).max() == 0:
+            assert np.abs(got).max() == 0
+            return
+        keep = edge_stable(fp64, w, kw, ref)
+        print(f"  voxels at a detector edge within rounding: {int((~keep).sum())} "
+              f"of {keep.size}", flush=True)
+        assert keep.mean() > 0.99
+        assert rel_l2(got[keep], ref[keep]) < 1e-4
+
+
+def edge_stable(fp64, w, kw, ref, delta=1e-4):
+    """Voxels whose oracle value stays put when the detector's centre moves by
+    +-delta rows or channels.  The reference keeps a view's sample only while
+    ceil(U1) and ceil(V1) lie strictly inside the detector
+    (src/Parallel_FDK_Helical_3DWeighting.cpp:86), and just inside its first
+    row or channel the sample is nearly that row's value: the sum jumps by a
+    whole sample where U1 or V1 crosses 0.  FP32 coordinates differ from FP64
+    by ~1e-6 rows, so a voxel with a view that close to the edge may keep or
+    drop that sample either way; a shift of delta (100x that) finds them.  The
+    shift itself moves every other voxel by ~delta times its gradient, far
+    below the 1e-3 of the volume's peak that marks a jump."""
+    tol = 1e-3 * np.abs(ref).max()
+    keep = np.ones(ref.shape, dtype=bool)
+    for key in ("YLC", "ZLC"):
+        for d in (-delta, delta):
+            k2 = dict(kw)
+            k2[key] = k2[key] + d
+            keep &= np.abs(oracle.fdk_core(fp64, w, **k2) - ref) <= tol
+    return keep
+
