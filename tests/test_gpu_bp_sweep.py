@@ -20,6 +20,7 @@ import pytest
 import torch
 
 import oracle
+from analyticalreconbigpatient_amd import _lib
 
 from .test_gpu_bp_lds import STORAGE_LAYOUT, run_bp
 from .test_gpu_stages import rel_l2
@@ -103,3 +104,23 @@ def edge_stable(fp64, w, kw, ref, delta=1e-4):
             keep &= np.abs(oracle.fdk_core(fp64, w, **k2) - ref) <= tol
     return keep
 
+
+@pytest.mark.parametrize("case", range(8))
+def test_random_geometry_double_drop_in(gpu, case):
+    """The FP64 drop-in (the reference's per-voxel loop in double precision)
+    on the same random scans: within 1e-10 of the FP64 oracle."""
+    g, (PN, ZL, YL), _, _, _ = draw(case)
+    print(f"bp sweep double case {case}: fp {(PN, ZL, YL)} grid {g['XN']}x{g['YN']}x{g['ZN']}",
+          flush=True)
+    fp = np.random.default_rng(case).normal(size=(PN, ZL, YL))
+    w = oracle.fdk_weights(g["N_2pi"], g["HSCoef"], 8)
+    want = oracle.fdk_core(fp, w, **{k: v for k, v in g.items() if k not in ("YL", "ZL")})
+    args = (g["N_2pi"], g["XN"], g["YN"], g["ZN"], g["XNC"], g["YNC"], g["ZNC"], g["dx"],
+            g["dy"], g["dz"], YL, ZL, PN, g["YLC"], g["ZLC"], g["dYL"], g["dZL"], g["h"],
+            g["BetaS"], g["DeltaFai"], g["N_pi"], g["HSCoef"], g["SO"], g["SD"], g["k1"])
+    img = np.zeros(g["XN"] * g["YN"] * g["ZN"])
+    _lib.lib().ParallekFDKHelical3DWeighting_GPU_double(_lib.dptr(img), _lib.dptr(fp),
+                                                         _lib.dptr(w), *args)
+    _lib.check(None, "double drop-in")
+    assert np.abs(want).max() > 0
+    assert rel_l2(img.reshape(want.shape), want) < 1e-10
